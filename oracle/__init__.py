@@ -1,0 +1,21 @@
+"""ORACLE — test infrastructure only.
+
+CPU restatements of the reference (esa/torchquad 0.5.0) algorithms on the
+integration hot path, used as the checker by tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg.  Nothing in the torchquad_amd package
+imports, links or calls anything here.
+
+Modules:
+  families  FP64 numpy definitions of the built-in integrand families and
+            their closed-form integrals over boxes.
+  grid      Newton-Cotes / Gauss-Legendre restatement (materialised grid,
+            per-axis composite collapse) and separable FP64 product sums.
+  mc        Monte Carlo restatement and a numpy Philox-4x32-10.
+  vegas     VEGAS map / stratification / driver restatement.
+  native    ctypes binding of liboracle.so (oracle.c: FP64 brute force
+            tensor-product sums, Philox-4x32-10, VEGAS integer primitives).
+
+Pinning status of each function is listed in the module docstrings; every
+function is pinned by at least one check in tests/test_oracle.py unless its
+docstring says "parity unpinned".
+"""

@@ -1,0 +1,274 @@
+"""Grid rules on the GPU vs the oracle (FP64 C brute force / numpy restatement)."""
+import math
+import warnings
+
+import numpy as np
+import pytest
+import torch
+
+import torchquad_amd as tq
+from oracle import families as fam
+from oracle import grid as og
+from oracle import native
+
+pytestmark = pytest.mark.gpu
+
+RULES = {"trapezoid": tq.Trapezoid, "simpson": tq.Simpson, "boole": tq.Boole, "gauss_legendre": tq.GaussLegendre}
+RTOL = {torch.float32: 1e-6, torch.float64: 1e-12}
+
+
+@pytest.fixture(autouse=True)
+def _quiet():
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        yield
+
+
+def make_family(name, d, rng):
+    a = rng.uniform(1.0, 5.0, d)
+    u = rng.uniform(-0.5, 0.5, d)
+    c = rng.uniform(0.0, 2.0, d)
+    w = float(rng.uniform())
+    if name == "gaussian_peak":
+        return tq.GaussianPeak(a, u), dict(a=a, u=u), 1.0
+    if name == "oscillatory":
+        return tq.Oscillatory(c, w), dict(c1=w, a=c), 1.0
+    if name == "product_peak":
+        a = rng.uniform(0.5, 2.0, d)
+        return tq.ProductPeak(a, u), dict(a=a, u=u), 1.0
+    if name == "exp_linear":
+        return tq.ExpLinear(d, a=c, c=0.1), dict(a=c, c1=0.1), 1.0
+    if name == "sin_sum":
+        return tq.SinSum(d, a=c, u=u), dict(a=c, u=u), float(d)
+    if name == "double_gaussian":
+        f = tq.DoubleGaussian(d, width=0.3)
+        return f, dict(c0=f.norm, c1=1 / 3, c2=2 / 3, c3=1 / 0.09), f.norm * 2
+    if name == "constant":
+        return tq.Constant(d, 2.5), dict(c0=2.5), 2.5
+    raise ValueError(name)
+
+
+def domain_for(name, d, rng):
+    if name == "double_gaussian":
+        return [[0.0, 1.0]] * d
+    lo = rng.uniform(-1.0, 0.0, d)
+    return [[float(l), float(l + rng.uniform(0.5, 1.5))] for l in lo]
+
+
+def oracle_value(rule, fname, d, n, dom, kw):
+    return native.grid_sum(rule, fname, d, n, dom, **kw)
+
+
+def check(val, ref, dtype, vol_sup):
+    tol = RTOL[dtype] * max(abs(ref), vol_sup)
+    assert abs(val - ref) <= tol, f"got {val!r} ref {ref!r} diff {abs(val-ref):.3e} tol {tol:.3e}"
+
+
+FAMS = ["gaussian_peak", "oscillatory", "product_peak", "exp_linear", "sin_sum", "double_gaussian", "constant"]
+SHAPES = [(1, 33), (2, 17), (3, 13), (4, 9), (5, 9), (6, 5)]
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("rule", list(RULES))
+@pytest.mark.parametrize("fname", FAMS)
+def test_fused_vs_oracle(dtype, rule, fname, dev):
+    rng = np.random.default_rng(hash((rule, fname)) % 2**32)
+    for d, n in SHAPES:
+        if rule == "boole":
+            n = 4 * ((n - 1) // 4) + 1 if n >= 5 else 5
+        if rule == "simpson" and n % 2 == 0:
+            n += 1
+        f, kw, sup = make_family(fname, d, rng)
+        dom = domain_for(fname, d, rng)
+        vol = float(np.prod([b - a for a, b in dom]))
+        ref = oracle_value(rule, fname, d, n, dom, kw)
+        domain = torch.tensor(dom, dtype=dtype, device=dev)
+        val = RULES[rule]().integrate(f, d, N=n**d, integration_domain=domain)
+        assert val.dtype == dtype and val.shape == ()
+        check(float(val), ref, dtype, vol * sup)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("rule", list(RULES))
+def test_user_callable_vs_oracle(dtype, rule, dev):
+    rng = np.random.default_rng(11)
+    for d, n in [(1, 21), (2, 9), (3, 9), (4, 5)]:
+        if rule == "simpson" and n % 2 == 0:
+            n += 1
+        f, kw, sup = make_family("gaussian_peak", d, rng)
+        dom = domain_for("gaussian_peak", d, rng)
+        ref = oracle_value(rule, "gaussian_peak", d, n, dom, kw)
+        domain = torch.tensor(dom, dtype=dtype, device=dev)
+        integrator = RULES[rule]()
+        val = integrator.integrate(lambda x: f(x), d, N=n**d, integration_domain=domain)
+        assert integrator._nr_of_fevals == n**d
+        check(float(val), ref, dtype, 1.0)
+        fused = RULES[rule]().integrate(f, d, N=n**d, integration_domain=domain)
+        check(float(val), float(fused), dtype, 1.0)
+
+
+def test_baseline_boole_config(configs, dev):
+    c = configs["C2a"]
+    f = tq.GaussianPeak(c["a"], c["u"])
+    dom = torch.tensor(c["domain"], dtype=torch.float32, device=dev)
+    for n in (17, 33, 65):
+        val = float(tq.Boole().integrate(f, 5, N=n**5, integration_domain=dom))
+        ref = og.separable_gaussian_peak("boole", c["a"], c["u"], c["domain"], n)
+        assert abs(val - ref) <= 1e-6 * abs(ref), (n, val, ref)
+    # n=65 Boole is within 1e-6 of the analytic value
+    assert abs(val - c["analytic"]) <= 1e-6 * c["analytic"]
+
+
+def test_baseline_c1_config(configs, dev):
+    c = configs["C1"]
+    dom = torch.tensor(c["domain"], dtype=torch.float64, device=dev)
+    f = tq.ExpLinear(3)
+    for rule in ("trapezoid", "simpson"):
+        val = float(RULES[rule]().integrate(f, 3, N=c["N"], integration_domain=dom))
+        ref = og.separable_product(rule, [np.exp] * 3, c["domain"], 101)
+        assert abs(val - ref) <= 1e-12 * ref
+    assert abs(val - c["analytic"]) <= 2e-10 * c["analytic"]
+
+
+@pytest.mark.parametrize("case_i", range(5))
+def test_tutorial_outputs(ref_vectors, case_i, dev):
+    case = ref_vectors["tutorial_outputs"]["cases"][case_i]
+    if case["integrand"] == "exp(x)*x^2":
+        f = lambda x: torch.exp(x[:, 0]) * x[:, 0] ** 2  # noqa: E731
+    else:
+        f = lambda x: torch.sum(torch.sin(x), dim=1)  # noqa: E731
+    dom = torch.tensor(case["domain"], dtype=torch.float32, device=dev)
+    val = float(RULES[case["rule"]]().integrate(f, case["dim"], N=case["N"], integration_domain=dom))
+    assert abs(val - case["value"]) <= 1e-6 * abs(case["value"])
+    if case["integrand"] == "sum sin(x)":
+        fused = float(RULES[case["rule"]]().integrate(tq.SinSum(case["dim"]), case["dim"], N=case["N"],
+                                                    integration_domain=dom))
+        assert abs(fused - case["value"]) <= 1e-6 * abs(case["value"])
+
+
+@pytest.mark.parametrize("rule,N,max_deg", [("trapezoid", 2, 1), ("simpson", 3, 3), ("boole", 5, 5),
+                                            ("gauss_legendre", 4, 7)])
+def test_polynomial_exactness(rule, N, max_deg, dev):
+    rng = np.random.default_rng(4)
+    dom = torch.tensor([[-1.3, 2.1]], dtype=torch.float64, device=dev)
+    for deg in range(max_deg + 1):
+        coeffs = rng.uniform(-2, 2, deg + 1)
+        ct = torch.tensor(coeffs, device=dev)
+        val = float(RULES[rule]().integrate(lambda x: sum(ct[k] * x[:, 0] ** k for k in range(deg + 1)), 1, N=N,
+                                            integration_domain=dom))
+        P = np.polynomial.polynomial.Polynomial(coeffs).integ()
+        exact = P(2.1) - P(-1.3)
+        assert abs(val - exact) < 1e-13 * max(1.0, abs(exact))
+
+
+def test_deterministic(configs, dev):
+    c = configs["C2a"]
+    f = tq.GaussianPeak(c["a"], c["u"])
+    dom = torch.tensor(c["domain"], dtype=torch.float32, device=dev)
+    vals = [tq.Boole().integrate(f, 5, N=33**5, integration_domain=dom).item() for _ in range(3)]
+    assert vals[0] == vals[1] == vals[2]
+
+
+def test_split_api_matches_integrate(dev):
+    # trapezoid_test.py:141-174
+    dom = torch.tensor([[0.0, 1.0]], device=dev)
+    integ = tq.Trapezoid()
+    pts, hs, n = integ.calculate_grid(101, dom)
+    assert pts.shape == (101, 1) and n == 101
+    vals, _ = integ.evaluate_integrand(lambda x: torch.exp(x[:, 0]), pts)
+    r1 = integ.calculate_result(vals, 1, n, hs, dom)
+    r2 = integ.calculate_result(function_values=vals, dim=1, n_per_dim=n, hs=hs, integration_domain=dom)
+    r3 = integ.calculate_result(vals, dim=1, n_per_dim=n, hs=hs, integration_domain=dom)
+    assert torch.allclose(r1, r2, rtol=1e-10) and torch.allclose(r1, r3, rtol=1e-10)
+    assert abs(float(r1) - og.integrate("trapezoid", lambda x: np.exp(x[:, 0]), 1, 101, [[0, 1]])) < 1e-6
+    # Gauss-Legendre split API: values are pre-weighted by evaluate_integrand
+    gl = tq.GaussLegendre()
+    dom64 = torch.tensor([[0.0, 2.0], [-1.0, 1.0]], dtype=torch.float64, device=dev)
+    pts, hs, n = gl.calculate_grid(8**2, dom64)
+    vals, _ = gl.evaluate_integrand(lambda x: torch.exp(x[:, 0]) * x[:, 1] ** 2, pts, weights=gl._weights(n, 2, "torch"))
+    r = gl.calculate_result(vals, 2, n, hs, dom64)
+    assert abs(float(r) - (math.e**2 - 1) * 2 / 3) < 1e-12
+
+
+def test_grid_points_layout(dev):
+    dom = torch.tensor([[0.0, 2.0], [-2.0, 1.0], [0.5, 1.0]], dtype=torch.float64, device=dev)
+    g = tq.IntegrationGrid(4**3, dom)
+    ref, axes = og.grid_points("trapezoid", dom.cpu().numpy(), 4)
+    assert g._N == 4 and g.points.shape == (64, 3)
+    np.testing.assert_allclose(g.points.cpu().numpy(), ref, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(g.h.cpu().numpy(), [ax[1] - ax[0] for ax in axes], rtol=1e-15)
+
+
+def test_multidim_integrand(dev):
+    dom = torch.tensor([[0.0, 1.0], [0.0, 2.0]], dtype=torch.float64, device=dev)
+
+    def f(x):
+        base = torch.exp(x[:, 0] + 0.5 * x[:, 1])
+        return torch.stack([torch.stack([base, 2 * base], 1), torch.stack([x[:, 0], x[:, 1] ** 2], 1)], 1)
+
+    val = tq.Simpson().integrate(f, 2, N=21**2, integration_domain=dom)
+    assert val.shape == (2, 2)
+    ref = og.integrate("simpson", lambda x: np.stack([np.stack([np.exp(x[:, 0] + 0.5 * x[:, 1]),
+                                                                  2 * np.exp(x[:, 0] + 0.5 * x[:, 1])], 1),
+                                                        np.stack([x[:, 0], x[:, 1] ** 2], 1)], 1),
+                       2, 21**2, [[0, 1], [0, 2]])
+    np.testing.assert_allclose(val.cpu().numpy(), ref, rtol=1e-12)
+
+
+def test_gradients_reference_answers(ref_vectors, dev):
+    G = ref_vectors["gradients"]
+    for integ, N1, N2 in zip([tq.Trapezoid(), tq.Simpson(), tq.Boole()], G["Ns_1d"], G["Ns_2d"]):
+        dom = torch.tensor(G["v_function_domain"]["domain"], dtype=torch.float64, device=dev, requires_grad=True)
+        val = integ.integrate(lambda x: 2 * torch.abs(x), 1, N=N1, integration_domain=dom)
+        val.backward()
+        assert abs(float(val) - 2.0) < 1e-2
+        np.testing.assert_array_less(np.abs(dom.grad.cpu().numpy() - np.array([[-2.0, 2.0]])), 2e-2)
+        dom = torch.tensor(G["poly_domain"]["domain"], dtype=torch.float64, device=dev, requires_grad=True)
+        val = integ.integrate(lambda x: 1.0 + 2.0 * x[:, 0] + 3.0 * x[:, 1] ** 2, 2, N=N2, integration_domain=dom)
+        val.backward()
+        assert abs(float(val) - 12.0) < 8e-2
+        np.testing.assert_array_less(np.abs(dom.grad.cpu().numpy() - np.array(G["poly_domain"]["grad"])), 0.1)
+        param = torch.tensor([1.0, 2.0, 3.0], dtype=torch.float64, device=dev, requires_grad=True)
+        val = integ.integrate(lambda x: param[0] + param[1] * x[:, 0] + param[2] * x[:, 1] ** 2, 2, N=N2,
+                              integration_domain=[[0.0, 1.0], [0.0, 2.0]])
+        val.backward()
+        np.testing.assert_array_less(np.abs(param.grad.cpu().numpy() - np.array([2.0, 1.0, 8.0 / 3.0])), 5e-2)
+    # Gauss-Legendre (never gradient-tested by the reference: gradient_test.py:168-178)
+    gl = tq.GaussLegendre()
+    dom = torch.tensor([[0.0, 1.0], [0.0, 2.0]], dtype=torch.float64, device=dev, requires_grad=True)
+    val = gl.integrate(lambda x: 1.0 + 2.0 * x[:, 0] + 3.0 * x[:, 1] ** 2, 2, N=8**2, integration_domain=dom)
+    val.backward()
+    assert abs(float(val) - 12.0) < 1e-12
+    np.testing.assert_allclose(dom.grad.cpu().numpy(), [[-10.0, 14.0], [-2.0, 14.0]], atol=1e-11)
+
+
+def test_gl_generator_matches_leggauss(dev):
+    from torchquad_amd.gl_nodes import gauss_legendre
+
+    for n in (1, 2, 5, 32, 255, 600):
+        x, w = gauss_legendre(n, dev)
+        rx, rw = np.polynomial.legendre.leggauss(n)
+        np.testing.assert_allclose(x.cpu().numpy(), rx, atol=2e-15 * max(1, n / 50))
+        np.testing.assert_allclose(w.cpu().numpy(), rw, rtol=1e-12)
+    x, w = gauss_legendre(100000, dev)
+    assert abs(float(w.sum()) - 2.0) < 1e-11
+    assert float((x[1:] - x[:-1]).min()) > 0
+    assert abs(float(torch.sum(w * x**2)) - 2.0 / 3.0) < 1e-11
+
+
+def test_large_one_dimensional(dev):
+    dom = torch.tensor([[0.0, 3.0]], dtype=torch.float64, device=dev)
+    f = tq.ExpLinear(1, a=[-0.7])
+    for rule, n in (("trapezoid", 1_000_000), ("simpson", 999_999), ("gauss_legendre", 20_000)):
+        val = float(RULES[rule]().integrate(f, 1, N=n, integration_domain=dom))
+        ref = og.separable_product(rule, [lambda x: np.exp(-0.7 * x)], [[0.0, 3.0]], n) if rule != "gauss_legendre" \
+            else fam.exp_linear_integral([-0.7], 0.0, [0.0], [3.0])
+        assert abs(val - ref) <= 1e-12 * abs(ref), rule
+
+
+def test_max_dim(dev):
+    d = 16
+    dom = torch.tensor([[0.0, 1.0]] * d, dtype=torch.float64, device=dev)
+    val = float(tq.Trapezoid().integrate(tq.ExpLinear(d, a=[0.1] * d), d, N=2**d, integration_domain=dom))
+    ref = og.separable_product("trapezoid", [lambda x: np.exp(0.1 * x)] * d, [[0, 1]] * d, 2)
+    assert abs(val - ref) <= 1e-12 * ref

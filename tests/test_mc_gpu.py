@@ -1,0 +1,98 @@
+"""Monte Carlo and the Philox stream on the GPU vs the oracle."""
+import math
+import warnings
+
+import numpy as np
+import pytest
+import torch
+
+import torchquad_amd as tq
+from torchquad_amd import _ops
+from oracle import families as fam
+from oracle import native
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _quiet():
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        yield
+
+
+@pytest.mark.parametrize("cols", [1, 3, 4, 8, 9])
+def test_uniform_stream_bit_exact(cols, dev):
+    for dtype, npd in ((torch.float32, np.float32), (torch.float64, np.float64)):
+        for seed, off in ((0, 0), (12345678901234, 2**33 + 5)):
+            g = _ops.uniform(seed, off, 1000, cols, dtype, dev).cpu().numpy()
+            r = native.uniform(seed, off, 1000, cols, npd)
+            assert np.array_equal(g, r)
+
+
+def test_rng_class_reproducible(dev):
+    a = tq.RNG(seed=3).uniform([100, 4], dtype=torch.float32)
+    r = tq.RNG(seed=3)
+    b1 = r.uniform([50, 4], dtype=torch.float32)
+    b2 = r.uniform([50, 4], dtype=torch.float32)
+    assert torch.equal(a, torch.cat([b1, b2]))
+    assert not torch.equal(b1, b2)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("fname", ["gaussian_peak", "oscillatory", "product_peak", "exp_linear", "constant"])
+def test_fused_mc_vs_same_stream_oracle(dtype, fname, dev):
+    rng = np.random.default_rng(5)
+    d, N, seed = 5, 200_000, 77
+    a, u, c = rng.uniform(1, 3, d), rng.uniform(-0.5, 0.5, d), rng.uniform(0, 2, d)
+    dom = [[-0.5, 1.0]] * d
+    f, kw = {
+        "gaussian_peak": (tq.GaussianPeak(a, u), dict(a=a, u=u)),
+        "oscillatory": (tq.Oscillatory(c, 0.2), dict(c1=0.2, a=c)),
+        "product_peak": (tq.ProductPeak(a, u), dict(a=a, u=u)),
+        "exp_linear": (tq.ExpLinear(d, a=c), dict(a=c)),
+        "constant": (tq.Constant(d, 1.0), dict()),
+    }[fname]
+    s1, _ = native.mc_sums("float32" if dtype == torch.float32 else "float64", fname, d, N, seed, 0, dom, **kw)
+    vol = 1.5**d
+    ref = vol * s1 / N
+    domain = torch.tensor(dom, dtype=dtype, device=dev)
+    val = float(tq.MonteCarlo().integrate(f, d, N=N, integration_domain=domain, seed=seed))
+    tol = (1e-6 if dtype == torch.float32 else 1e-12) * max(abs(ref), vol)
+    assert abs(val - ref) <= tol, (val, ref)
+    # the user-callable path sees the same points
+    user = float(tq.MonteCarlo().integrate(lambda x: f(x), d, N=N, integration_domain=domain, seed=seed))
+    assert abs(user - ref) <= tol
+
+
+def test_constant_integrand_exact(dev):
+    dom = torch.tensor([[0.0, 3.0], [1.0, 2.0]], dtype=torch.float32, device=dev)
+    assert float(tq.MonteCarlo().integrate(tq.Constant(2, 2.0), 2, N=10007, integration_domain=dom, seed=0)) == 6.0
+    assert float(tq.MonteCarlo().integrate(lambda x: x[:, 0] * 0 + 2.0, 2, N=10007, integration_domain=dom,
+                                           seed=0)) == 6.0
+
+
+def test_baseline_mc_config_statistics(configs, dev):
+    c = configs["C2b"]
+    mc = tq.MonteCarlo()
+    f = tq.Oscillatory(c["c"], c["w"])
+    dom = torch.tensor(c["domain"], dtype=torch.float32, device=dev)
+    val = float(mc.integrate(f, 8, N=10_000_000, integration_domain=dom, seed=0))
+    se = mc.std_error()
+    assert se > 0
+    assert abs(val - c["analytic"]) <= 3 * se + 1e-6, (val, c["analytic"], se)
+
+
+def test_mc_gradients_reference_answers(ref_vectors, dev):
+    G = ref_vectors["gradients"]
+    mc = tq.MonteCarlo()
+    dom = torch.tensor(G["poly_domain"]["domain"], dtype=torch.float64, device=dev, requires_grad=True)
+    val = mc.integrate(lambda x: 1.0 + 2.0 * x[:, 0] + 3.0 * x[:, 1] ** 2, 2, N=99997, integration_domain=dom, seed=0)
+    val.backward()
+    assert abs(float(val) - 12.0) < 8e-2
+    np.testing.assert_array_less(np.abs(dom.grad.cpu().numpy() - np.array(G["poly_domain"]["grad"])), 0.1)
+    param = torch.tensor([1.0, 2.0, 3.0], dtype=torch.float64, device=dev, requires_grad=True)
+    val = mc.integrate(lambda x: param[0] + param[1] * x[:, 0] + param[2] * x[:, 1] ** 2, 2, N=99997,
+                       integration_domain=[[0.0, 1.0], [0.0, 2.0]], seed=0)
+    val.backward()
+    np.testing.assert_array_less(np.abs(param.grad.cpu().numpy() - np.array([2.0, 1.0, 8.0 / 3.0])), 5e-2)

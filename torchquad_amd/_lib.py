@@ -1,0 +1,214 @@
+"""ctypes binding of ``libtorchquad_amd.so`` (declared in include/torchquad_amd.h).
+
+The library is the only compute path of this package: every integrator
+launches its kernels through the functions bound here.  If the library is
+missing or no ROCm device is visible, compute calls raise
+:class:`NativeUnavailableError` instead of computing anything elsewhere.
+"""
+import ctypes
+import os
+import threading
+from pathlib import Path
+
+import torch
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libtorchquad_amd.so"
+
+TQ_MAX_DIM = 16
+TQ_F32, TQ_F64 = 0, 1
+RULE_TRAPEZOID, RULE_SIMPSON, RULE_BOOLE, RULE_GAUSS_LEGENDRE = 0, 1, 2, 3
+
+FAM_GAUSSIAN_PEAK = 0
+FAM_OSCILLATORY = 1
+FAM_PRODUCT_PEAK = 2
+FAM_EXP_LINEAR = 3
+FAM_SIN_SUM = 4
+FAM_DOUBLE_GAUSSIAN = 5
+FAM_CONSTANT = 6
+
+
+class NativeUnavailableError(RuntimeError):
+    """The HIP library or a ROCm GPU is not available for a compute call."""
+
+
+class NativeError(RuntimeError):
+    """A C ABI call returned a non-zero status."""
+
+    def __init__(self, fn, code, msg):
+        super().__init__(f"{fn} failed with status {code}: {msg}")
+        self.code = code
+
+
+class Integrand(ctypes.Structure):
+    _fields_ = [
+        ("family", ctypes.c_int32),
+        ("dim", ctypes.c_int32),
+        ("c0", ctypes.c_double),
+        ("c1", ctypes.c_double),
+        ("c2", ctypes.c_double),
+        ("c3", ctypes.c_double),
+        ("a", ctypes.c_double * TQ_MAX_DIM),
+        ("u", ctypes.c_double * TQ_MAX_DIM),
+    ]
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [
+        ("dtype", ctypes.c_int32),
+        ("rule", ctypes.c_int32),
+        ("dim", ctypes.c_int32),
+        ("n", ctypes.c_int32),
+        ("lane_axes", ctypes.c_int32),
+        ("items_per_wave", ctypes.c_int32),
+        ("gl_nodes", ctypes.c_void_p),
+        ("gl_weights", ctypes.c_void_p),
+    ]
+
+
+class McDesc(ctypes.Structure):
+    _fields_ = [
+        ("dtype", ctypes.c_int32),
+        ("dim", ctypes.c_int32),
+        ("N", ctypes.c_int64),
+        ("seed", ctypes.c_uint64),
+        ("offset", ctypes.c_uint64),
+        ("samples_per_thread", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+_P = ctypes.c_void_p
+_SZ = ctypes.c_size_t
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_U64 = ctypes.c_uint64
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "tq_last_error": (ctypes.c_char_p, []),
+    "tq_version": (ctypes.c_int, []),
+    "tq_device_count": (ctypes.c_int, []),
+    "tq_grid_workspace_size": (_SZ, [ctypes.POINTER(GridDesc)]),
+    "tq_grid_integrate": (
+        ctypes.c_int,
+        [ctypes.POINTER(GridDesc), ctypes.POINTER(Integrand), _P, _P, _SZ, _P, _P, _P],
+    ),
+    "tq_grid_points": (ctypes.c_int, [ctypes.POINTER(GridDesc), _P, _P, _P]),
+    "tq_grid_h": (ctypes.c_int, [ctypes.POINTER(GridDesc), _P, _P, _P]),
+    "tq_grid_reduce_workspace_size": (_SZ, [ctypes.POINTER(GridDesc), _I64]),
+    "tq_grid_reduce": (ctypes.c_int, [ctypes.POINTER(GridDesc), _P, _P, _I32, _P, _I64, _P, _SZ, _P, _P]),
+    "tq_grid_reduce_backward": (ctypes.c_int, [ctypes.POINTER(GridDesc), _P, _P, _I32, _P, _I64, _P, _P]),
+    "tq_grid_points_backward_workspace_size": (_SZ, [ctypes.POINTER(GridDesc)]),
+    "tq_grid_points_backward": (ctypes.c_int, [ctypes.POINTER(GridDesc), _P, _P, _P, _SZ, _P, _P]),
+    "tq_gauss_legendre": (ctypes.c_int, [_I32, _P, _P, _P]),
+    "tq_mc_workspace_size": (_SZ, [ctypes.POINTER(McDesc)]),
+    "tq_mc_integrate": (
+        ctypes.c_int,
+        [ctypes.POINTER(McDesc), ctypes.POINTER(Integrand), _P, _P, _SZ, _P, _P, _P],
+    ),
+    "tq_mc_points": (ctypes.c_int, [ctypes.POINTER(McDesc), _P, _P, _P]),
+    "tq_mc_reduce_workspace_size": (_SZ, [_I64, _I64]),
+    "tq_mc_reduce": (ctypes.c_int, [_I32, _P, _I64, _I64, _P, _I32, _P, _SZ, _P, _P]),
+    "tq_uniform": (ctypes.c_int, [_U64, _U64, _I32, _I64, _I32, _P, _P]),
+}
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def load_library(path=None):
+    """Load and bind the shared library (no GPU needed to load it)."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None and path is None:
+            return _lib
+        p = Path(path) if path else LIB_PATH
+        if not p.exists():
+            raise NativeUnavailableError(
+                f"{p} is missing; build it with `python -m torchquad_amd._build`"
+            )
+        lib = ctypes.CDLL(str(p))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        if path is None:
+            _lib = lib
+        return lib
+
+
+def lib():
+    return load_library()
+
+
+def require_device(tensor):
+    """Raise unless `tensor` lives on a ROCm GPU that the library can use."""
+    if not torch.cuda.is_available():
+        raise NativeUnavailableError(
+            "torchquad_amd computes on a ROCm GPU only and none is visible to torch"
+        )
+    if tensor.device.type != "cuda":
+        raise NativeUnavailableError(
+            "torchquad_amd computes on a ROCm GPU only; the integration domain is on "
+            f"{tensor.device}. Call enable_cuda() / set_up_backend('torch') or pass a "
+            "domain tensor on the GPU."
+        )
+    lib()
+
+
+def call(name, *args):
+    """Call a status-returning entry point and raise NativeError on failure."""
+    fn = getattr(lib(), name)
+    rc = fn(*args)
+    if rc != 0:
+        msg = lib().tq_last_error().decode(errors="replace")
+        raise NativeError(name, rc, msg)
+    return rc
+
+
+def stream_ptr(device=None):
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def dtype_code(dtype):
+    if dtype == torch.float32:
+        return TQ_F32
+    if dtype == torch.float64:
+        return TQ_F64
+    raise ValueError(f"Unsupported dtype {dtype}; torchquad_amd computes in float32 or float64")
+
+
+class _Workspace:
+    """Per (device, stream) scratch buffers grown on demand.
+
+    The kernels of one call finish reading a workspace before any later call
+    on the same stream writes it, so reuse within a stream is safe.
+    """
+
+    def __init__(self):
+        self._bufs = {}
+        self._lock = threading.Lock()
+
+    def get(self, nbytes, device, tag="main"):
+        key = (device.index if device.index is not None else torch.cuda.current_device(),
+               torch.cuda.current_stream(device).cuda_stream, tag)
+        with self._lock:
+            buf = self._bufs.get(key)
+            if buf is None or buf.numel() < nbytes:
+                size = max(int(nbytes), 256)
+                size = 1 << (size - 1).bit_length()
+                buf = torch.empty(size, dtype=torch.uint8, device=device)
+                self._bufs[key] = buf
+            return buf
+
+
+workspace = _Workspace()
+
+
+def env_flag(name, default="0"):
+    return os.environ.get(name, default) not in ("0", "", "false", "False")

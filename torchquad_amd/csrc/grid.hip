@@ -1,0 +1,819 @@
+// Tensor-product grid rules (Trapezoid, Simpson, Boole, Gauss-Legendre).
+//
+// Index space.  The n^d grid points (ij order, last axis fastest, as
+// IntegrationGrid.points in integration_grid.py:98-99) are split into
+//   * lane axes  [d-m, d): one combination per lane, fixed for the lane's
+//     lifetime (its partial fold and weight are computed once);
+//   * loop axis  L = d-m-1: wave-uniform, iterated in the hot loop with its
+//     nodes and weights read from an LDS table (broadcast reads);
+//   * row axes   [0, L): wave-uniform, advanced by an odometer once per row.
+// A wave owns one group of 64 lane combinations and a contiguous range of
+// (row, loop-index) items.  Per point the kernel does: fold of the loop axis
+// into the shared row+lane partial, the family's finish (transcendental),
+// and one FMA into an FP32 partial that is flushed to FP64 every 32 points.
+// Block partials are FP64 and the final sum runs in a fixed order, so the
+// result is deterministic.
+#include "tq_host.hpp"
+
+namespace tq {
+
+constexpr int kMaxDimPoints = 64;  // point writer / weighted reduce
+constexpr int kLoopTable = 2048;   // largest n kept in the LDS loop table
+constexpr int kFlush = 32;         // FP32 partial length before FP64 flush
+
+// Per-axis affine map.  Newton-Cotes: x_j = a + j*h (x_{n-1} = b exactly, as
+// linspace).  Gauss-Legendre: x_j = alpha*r_j + beta (gaussian.py:159-162).
+struct Axis {
+    double a, h, b;
+};
+
+template <typename T>
+__device__ inline Axis make_axis(const T* domain, int k, int n, int rule) {
+    const double a = (double)domain[2 * k], b = (double)domain[2 * k + 1];
+    Axis ax;
+    if (rule == TQ_RULE_GAUSS_LEGENDRE) {
+        ax.a = 0.5 * (b - a);
+        ax.h = 0.5 * (a + b);
+        ax.b = b;
+    } else {
+        ax.a = a;
+        ax.h = (b - a) / (double)(n - 1);
+        ax.b = b;
+    }
+    return ax;
+}
+
+__device__ inline double node_of(const Axis& ax, int j, int n, int rule, const double* glx) {
+    if (rule == TQ_RULE_GAUSS_LEGENDRE) return fma(ax.a, glx[j], ax.h);
+    return (j == n - 1) ? ax.b : fma((double)j, ax.h, ax.a);
+}
+
+// Composite Newton-Cotes weight pattern without the per-axis scale factor
+// (trapezoid.py:34-36 -> h/2*[1,2,..,2,1]; simpson.py:35-45 -> h/3*[1,4,2,..,4,1];
+// boole.py:35-47 -> h/22.5*[7,32,12,32,14,..,32,7]); Gauss-Legendre weight.
+__device__ __host__ inline double weight_of(int rule, int j, int n, const double* glw) {
+    const bool end = (j == 0) || (j == n - 1);
+    switch (rule) {
+        case -1: return 1.0;
+        case TQ_RULE_TRAPEZOID: return end ? 1.0 : 2.0;
+        case TQ_RULE_SIMPSON: return end ? 1.0 : ((j & 1) ? 4.0 : 2.0);
+        case TQ_RULE_BOOLE: return end ? 7.0 : ((j & 1) ? 32.0 : (((j & 3) == 2) ? 12.0 : 14.0));
+        default: return glw[j];
+    }
+}
+
+// Per-axis scale so that integral = prod_k scale_k * sum W f.
+__device__ inline double axis_scale(double a, double b, int n, int rule) {
+    switch (rule) {
+        case TQ_RULE_TRAPEZOID: return (b - a) / (double)(n - 1) / 2.0;
+        case TQ_RULE_SIMPSON: return (b - a) / (double)(n - 1) / 3.0;
+        case TQ_RULE_BOOLE: return (b - a) / (double)(n - 1) / 22.5;
+        default: return 0.5 * (b - a);
+    }
+}
+
+struct GridGeom {
+    int d, n, m, L;
+    int rule;
+    int lane_count;       // n^m
+    int lane_waves;       // ceil(lane_count / 64)
+    int use_table;        // loop axis nodes/weights in LDS
+    long long items;      // n^(d-m) (1 when there is no loop axis)
+    long long ipw;        // items per wave
+    long long chunks;     // ceil(items / ipw)
+    long long num_waves;  // lane_waves * chunks
+    long long blocks;
+};
+
+template <typename T>
+struct GridArgs {
+    GridGeom g;
+    const T* domain;
+    const double* glx;
+    const double* glw;
+    double* partials;
+    Params<T> p;
+};
+
+static int ipow_checked(long long base, int e, long long limit, long long* out) {
+    long long r = 1;
+    for (int i = 0; i < e; ++i) {
+        if (r > limit / base) return -1;
+        r *= base;
+    }
+    *out = r;
+    return 0;
+}
+
+static int check_grid_desc(const tq_grid_desc* g, int max_dim) {
+    if (!g) return fail(TQ_E_NULL, "grid descriptor is NULL");
+    if (g->dtype != TQ_F32 && g->dtype != TQ_F64) return fail(TQ_E_ARG, "dtype must be TQ_F32 or TQ_F64");
+    if (g->dim < 1 || g->dim > max_dim) return fail(TQ_E_ARG, "grid dim out of range");
+    switch (g->rule) {
+        case TQ_RULE_TRAPEZOID:
+            if (g->n < 2) return fail(TQ_E_ARG, "Trapezoid needs n >= 2 points per axis");
+            break;
+        case TQ_RULE_SIMPSON:
+            if (g->n < 3 || (g->n % 2) != 1) return fail(TQ_E_ARG, "Simpson needs odd n >= 3");
+            break;
+        case TQ_RULE_BOOLE:
+            if (g->n < 5 || ((g->n - 1) % 4) != 0) return fail(TQ_E_ARG, "Boole needs n = 1 + 4k >= 5");
+            break;
+        case TQ_RULE_GAUSS_LEGENDRE:
+            if (g->n < 1) return fail(TQ_E_ARG, "Gauss-Legendre needs n >= 1");
+            break;
+        default: return fail(TQ_E_ARG, "unknown rule");
+    }
+    long long P;
+    if (ipow_checked(g->n, g->dim, (1LL << 62), &P) != 0) return fail(TQ_E_ARG, "n^dim exceeds 2^62 points");
+    return TQ_OK;
+}
+
+// Gauss-Legendre node/weight tables are needed wherever nodes or weights
+// are formed (everything except unit-weight reductions).
+static int check_gl_tables(const tq_grid_desc* g) {
+    if (g->rule == TQ_RULE_GAUSS_LEGENDRE && (!g->gl_nodes || !g->gl_weights))
+        return fail(TQ_E_NULL, "Gauss-Legendre needs node/weight tables");
+    return TQ_OK;
+}
+
+static int make_geom(const tq_grid_desc* gd, GridGeom& G) {
+    const int d = gd->dim, n = gd->n;
+    G.d = d;
+    G.n = n;
+    G.rule = gd->rule;
+    int m = 0;
+    if (gd->lane_axes > 0) {
+        m = gd->lane_axes < d ? gd->lane_axes : d;
+    } else {
+        long long c = 1;
+        m = d;
+        for (int k = 1; k <= d; ++k) {
+            c *= n;
+            if (c >= 64) {
+                m = k;
+                break;
+            }
+        }
+        // keep a loop axis when the lanes are still reasonably filled without it
+        if (m == d && d > 1) {
+            long long c2;
+            ipow_checked(n, d - 1, 1LL << 62, &c2);
+            if (c2 >= 32) m = d - 1;
+        }
+    }
+    long long lc;
+    if (ipow_checked(n, m, (1LL << 31) - 64, &lc) != 0) return fail(TQ_E_ARG, "lane combinations exceed 2^31");
+    G.m = m;
+    G.lane_count = (int)lc;
+    G.lane_waves = (int)((lc + kWave - 1) / kWave);
+    G.L = d - m - 1;
+    long long items = 1;
+    if (G.L >= 0) ipow_checked(n, d - m, 1LL << 62, &items);
+    G.items = items;
+    G.use_table = (G.L >= 0 && n <= kLoopTable) ? 1 : 0;
+    long long ipw = gd->items_per_wave;
+    if (ipw <= 0) {
+        const long long target_waves = 16384;
+        const long long work = items * (long long)G.lane_waves;
+        ipw = (work + target_waves - 1) / target_waves;
+        const long long min_ipw = items < 64 ? items : 64;
+        if (ipw < min_ipw) ipw = min_ipw;
+    }
+    if (ipw < 1) ipw = 1;
+    // bound the launch size
+    while (((items + ipw - 1) / ipw) * (long long)G.lane_waves > (1LL << 30)) ipw *= 2;
+    G.ipw = ipw;
+    G.chunks = (items + ipw - 1) / ipw;
+    G.num_waves = G.chunks * (long long)G.lane_waves;
+    G.blocks = (G.num_waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    return TQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Fused kernel
+// ---------------------------------------------------------------------------
+template <typename T, int FAM, bool TABLE>
+__global__ __launch_bounds__(kBlock) void k_grid_fused(GridArgs<T> A) {
+    using F = Family<FAM, T>;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ Axis s_axis[TQ_MAX_DIM];
+    __shared__ double s_red[kWavesPerBlock];
+    const GridGeom& g = A.g;
+    const int tid = threadIdx.x;
+    const int n = g.n, d = g.d, L = g.L, rule = g.rule;
+    if (tid < d) s_axis[tid] = make_axis(A.domain, tid, n, rule);
+    __syncthreads();
+    T* s_tab = reinterpret_cast<T*>(smem_raw);
+    if (TABLE) {
+        for (int j = tid; j < n; j += kBlock) {
+            s_tab[2 * j] = (T)node_of(s_axis[L], j, n, rule, A.glx);
+            s_tab[2 * j + 1] = (T)weight_of(rule, j, n, A.glw);
+        }
+        __syncthreads();
+    }
+
+    const long long wave = (long long)blockIdx.x * kWavesPerBlock + (tid >> 6);
+    const int lane = tid & (kWave - 1);
+    double acc = 0.0;
+    if (wave < g.num_waves) {
+        const int lw = (int)(wave % g.lane_waves);
+        const long long chunk = wave / g.lane_waves;
+        const int li = lw * kWave + lane;
+        const bool active = li < g.lane_count;
+
+        // lane axes: fold and weight once
+        State<T> s_lane = F::zero();
+        double w_lane = active ? 1.0 : 0.0;
+        {
+            int rem = active ? li : 0;
+            for (int k = d - 1; k >= d - g.m; --k) {
+                const int j = rem % n;
+                rem /= n;
+                s_lane = F::fold(s_lane, k, (T)node_of(s_axis[k], j, n, rule, A.glx), A.p);
+                w_lane *= weight_of(rule, j, n, A.glw);
+            }
+        }
+
+        if (L < 0) {
+            if (chunk == 0) {
+                const T f = F::finish(F::merge(F::start(A.p), s_lane), A.p);
+                acc = w_lane * (double)f;
+            }
+        } else {
+            long long t = chunk * g.ipw;
+            const long long t_end = (t + g.ipw < g.items) ? (t + g.ipw) : g.items;
+            long long row = t / n;
+            int j = (int)(t - row * n);
+            // row odometer digits for axes [0, L)
+            int rdig[TQ_MAX_DIM];
+            {
+                long long r = row;
+#pragma unroll
+                for (int k = TQ_MAX_DIM - 1; k >= 0; --k) {
+                    if (k < L) {
+                        rdig[k] = (int)(r % n);
+                        r /= n;
+                    } else {
+                        rdig[k] = 0;
+                    }
+                }
+            }
+            while (t < t_end) {
+                State<T> s_row = F::start(A.p);
+                double w_row = w_lane;
+#pragma unroll
+                for (int k = 0; k < TQ_MAX_DIM; ++k) {
+                    if (k < L) {
+                        s_row = F::fold(s_row, k, (T)node_of(s_axis[k], rdig[k], n, rule, A.glx), A.p);
+                        w_row *= weight_of(rule, rdig[k], n, A.glw);
+                    }
+                }
+                const State<T> s_rl = F::merge(s_row, s_lane);
+                const long long left = t_end - t;
+                const int jend = (left < (long long)(n - j)) ? (j + (int)left) : n;
+                double racc = 0.0;
+                for (int jb = j; jb < jend; jb += kFlush) {
+                    const int je = (jb + kFlush < jend) ? (jb + kFlush) : jend;
+                    T fa = T(0);
+                    for (int jj = jb; jj < je; ++jj) {
+                        T xL, wL;
+                        if (TABLE) {
+                            xL = s_tab[2 * jj];
+                            wL = s_tab[2 * jj + 1];
+                        } else {
+                            xL = (T)node_of(s_axis[L], jj, n, rule, A.glx);
+                            wL = (T)weight_of(rule, jj, n, A.glw);
+                        }
+                        const T f = F::finish(F::fold(s_rl, L, xL, A.p), A.p);
+                        fa = fma(wL, f, fa);
+                    }
+                    racc += (double)fa;
+                }
+                acc = fma(w_row, racc, acc);
+                t += jend - j;
+                j = jend;
+                if (j == n) {
+                    j = 0;
+#pragma unroll
+                    for (int k = TQ_MAX_DIM - 1; k >= 0; --k) {
+                        if (k < L) {
+                            if (++rdig[k] < n) break;
+                            rdig[k] = 0;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    const double bs = block_sum(acc, s_red);
+    if (tid == 0) A.partials[blockIdx.x] = bs;
+}
+
+// Deterministic final sum of block partials and the per-axis scale factors.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_grid_finalize(const double* partials, long long nparts,
+                                                         const T* domain, int d, int n, int rule,
+                                                         double c0, T* out, double* out64) {
+    __shared__ double s_red[1024 / kWave];
+    double v = 0.0;
+    for (long long i = threadIdx.x; i < nparts; i += blockDim.x) v += partials[i];
+    v = block_sum(v, s_red);
+    if (threadIdx.x == 0) {
+        double scale = c0;
+        for (int k = 0; k < d; ++k)
+            scale *= axis_scale((double)domain[2 * k], (double)domain[2 * k + 1], n, rule);
+        const double r = v * scale;
+        if (out) *out = (T)r;
+        if (out64) *out64 = r;
+    }
+}
+
+template <typename T>
+static int launch_grid(const tq_grid_desc* gd, const tq_integrand* f, const void* domain,
+                       void* ws, size_t ws_bytes, void* out, double* out64, hipStream_t st) {
+    GridGeom G;
+    int rc = make_geom(gd, G);
+    if (rc) return rc;
+    const size_t need = (size_t)G.blocks * sizeof(double);
+    if (!ws || ws_bytes < need) return fail(TQ_E_WORKSPACE, "grid workspace too small");
+    GridArgs<T> A;
+    A.g = G;
+    A.domain = (const T*)domain;
+    A.glx = gd->gl_nodes;
+    A.glw = gd->gl_weights;
+    A.partials = (double*)ws;
+    A.p = make_params<T>(*f);
+    const size_t shmem = G.use_table ? (size_t)2 * G.n * sizeof(T) : 0;
+    TQ_DISPATCH_FAMILY(f->family, {
+        if (G.use_table)
+            hipLaunchKernelGGL((k_grid_fused<T, FAM, true>), dim3((unsigned)G.blocks), dim3(kBlock), shmem, st, A);
+        else
+            hipLaunchKernelGGL((k_grid_fused<T, FAM, false>), dim3((unsigned)G.blocks), dim3(kBlock), 0, st, A);
+    });
+    rc = check_launch("k_grid_fused");
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_grid_finalize<T>), dim3(1), dim3(1024), 0, st, (const double*)ws,
+                       (long long)G.blocks, (const T*)domain, G.d, G.n, G.rule, f->c0, (T*)out, out64);
+    return check_launch("k_grid_finalize");
+}
+
+// ---------------------------------------------------------------------------
+// Point writer (user integrands)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_grid_points(const T* domain, int d, int n, int rule,
+                                                        const double* glx, long long P, T* pts) {
+    __shared__ Axis s_axis[kMaxDimPoints];
+    for (int k = threadIdx.x; k < d; k += blockDim.x) s_axis[k] = make_axis(domain, k, n, rule);
+    __syncthreads();
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += stride) {
+        unsigned long long rem = (unsigned long long)p;
+        T* row = pts + p * d;
+        for (int k = d - 1; k >= 0; --k) {
+            const int j = (int)(rem % (unsigned long long)n);
+            rem /= (unsigned long long)n;
+            row[k] = (T)node_of(s_axis[k], j, n, rule, glx);
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_grid_h(const T* domain, int d, int n, int rule, const double* glx, T* h) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d) return;
+    const Axis ax = make_axis(domain, k, n, rule);
+    const T x0 = (T)node_of(ax, 0, n, rule, glx);
+    const T x1 = (n > 1) ? (T)node_of(ax, 1, n, rule, glx) : x0;
+    h[k] = x1 - x0;
+}
+
+// ---------------------------------------------------------------------------
+// Weighted reduction of user values f[P, S]
+// Lanes enumerate (inner point over the last m axes, component s); the wave
+// walks the outer index with an odometer; reads are coalesced.
+// ---------------------------------------------------------------------------
+struct ReduceGeom {
+    int d, n, m, rule;
+    long long S;
+    long long inner;       // n^m
+    long long lane_elems;  // inner*S
+    long long lane_waves;
+    long long outer;       // n^(d-m)
+    long long ipw;
+    long long chunks;
+    long long num_waves;
+    long long blocks;
+};
+
+static int make_reduce_geom(const tq_grid_desc* gd, long long S, ReduceGeom& R) {
+    const int d = gd->dim, n = gd->n;
+    R.d = d;
+    R.n = n;
+    R.rule = gd->rule;
+    R.S = S;
+    int m = d;
+    long long c = 1;
+    for (int k = 1; k <= d; ++k) {
+        c *= n;
+        if (c * S >= 64) {
+            m = k;
+            break;
+        }
+    }
+    if (m > d) m = d;
+    long long inner;
+    ipow_checked(n, m, 1LL << 62, &inner);
+    R.m = m;
+    R.inner = inner;
+    R.lane_elems = inner * S;
+    R.lane_waves = (R.lane_elems + kWave - 1) / kWave;
+    long long outer;
+    ipow_checked(n, d - m, 1LL << 62, &outer);
+    R.outer = outer;
+    const long long target_waves = 16384;
+    long long ipw = (outer * R.lane_waves + target_waves - 1) / target_waves;
+    if (ipw < 16) ipw = outer < 16 ? outer : 16;
+    if (ipw < 1) ipw = 1;
+    while (((outer + ipw - 1) / ipw) * R.lane_waves > (1LL << 30)) ipw *= 2;
+    R.ipw = ipw;
+    R.chunks = (outer + ipw - 1) / ipw;
+    R.num_waves = R.chunks * R.lane_waves;
+    R.blocks = (R.num_waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    return TQ_OK;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_grid_reduce(ReduceGeom R, const T* fv, const double* glw,
+                                                        double* partials /* [blocks][S] */) {
+    __shared__ double s_acc[kBlock];
+    __shared__ long long s_comp[kBlock];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1);
+    const long long wave = (long long)blockIdx.x * kWavesPerBlock + (tid >> 6);
+    const int n = R.n, d = R.d, m = R.m, rule = R.rule;
+    double acc = 0.0;
+    long long comp = -1;
+    if (wave < R.num_waves) {
+        const long long lw = wave % R.lane_waves;
+        const long long chunk = wave / R.lane_waves;
+        const long long e = lw * kWave + lane;
+        if (e < R.lane_elems) {
+            comp = e % R.S;
+            long long rem = e / R.S;
+            double w_in = 1.0;
+            for (int k = d - 1; k >= d - m; --k) {
+                const int j = (int)(rem % n);
+                rem /= n;
+                w_in *= weight_of(rule, j, n, glw);
+            }
+            const long long o0 = chunk * R.ipw;
+            const long long o1 = (o0 + R.ipw < R.outer) ? (o0 + R.ipw) : R.outer;
+            int odig[kMaxDimPoints];
+            const int no = d - m;
+            {
+                long long r = o0;
+                for (int k = no - 1; k >= 0; --k) {
+                    odig[k] = (int)(r % n);
+                    r /= n;
+                }
+            }
+            const long long stride = R.lane_elems;
+            for (long long o = o0; o < o1; ++o) {
+                double w = w_in;
+                for (int k = 0; k < no; ++k) w *= weight_of(rule, odig[k], n, glw);
+                acc = fma(w, (double)fv[o * stride + e], acc);
+                for (int k = no - 1; k >= 0; --k) {
+                    if (++odig[k] < n) break;
+                    odig[k] = 0;
+                }
+            }
+        }
+    }
+    s_acc[tid] = acc;
+    s_comp[tid] = comp;
+    __syncthreads();
+    for (long long s = tid; s < R.S; s += blockDim.x) {
+        double v = 0.0;
+        for (int i = 0; i < (int)blockDim.x; ++i)
+            if (s_comp[i] == s) v += s_acc[i];
+        partials[(long long)blockIdx.x * R.S + s] = v;
+    }
+}
+
+// Scale of the weighted reduction: Newton-Cotes uses the caller's mesh
+// widths when given (hs[k]/c as _apply_composite_rule does), otherwise the
+// domain's; Gauss-Legendre uses (b-a)/2 per axis.
+template <typename T>
+__device__ inline double reduce_scale(const T* domain, const T* hs, int d, int n, int rule, int unit) {
+    double scale = 1.0;
+    for (int k = 0; k < d; ++k) {
+        const double a = (double)domain[2 * k], b = (double)domain[2 * k + 1];
+        if (unit || rule == TQ_RULE_GAUSS_LEGENDRE) {
+            scale *= 0.5 * (b - a);
+        } else if (hs) {
+            const double h = (double)hs[k];
+            scale *= (rule == TQ_RULE_TRAPEZOID) ? h / 2.0 : (rule == TQ_RULE_SIMPSON) ? h / 3.0 : h / 22.5;
+        } else {
+            scale *= axis_scale(a, b, n, rule);
+        }
+    }
+    return scale;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_grid_reduce_finalize(const double* partials, long long blocks,
+                                                                 long long S, const T* domain, const T* hs,
+                                                                 int unit, int d, int n, int rule, T* out) {
+    __shared__ double s_red[kWavesPerBlock];
+    const long long s = blockIdx.x;
+    double v = 0.0;
+    for (long long b = threadIdx.x; b < blocks; b += blockDim.x) v += partials[b * S + s];
+    v = block_sum(v, s_red);
+    if (threadIdx.x == 0) out[s] = (T)(v * reduce_scale(domain, hs, d, n, rule, unit));
+}
+
+// grad_f[p, s] = grad_out[s] * scale * W(p)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_grid_reduce_backward(const T* domain, const T* hs, int unit, int d,
+                                                                 int n, int rule, const double* glw,
+                                                                 const T* grad_out, long long S, long long P,
+                                                                 T* grad_f) {
+    const double scale = reduce_scale(domain, hs, d, n, rule, unit);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += stride) {
+        double w = scale;
+        if (!unit) {
+            unsigned long long rem = (unsigned long long)p;
+            for (int k = d - 1; k >= 0; --k) {
+                const int j = (int)(rem % (unsigned long long)n);
+                rem /= (unsigned long long)n;
+                w *= weight_of(rule, j, n, glw);
+            }
+        }
+        for (long long s = 0; s < S; ++s) grad_f[p * S + s] = (T)(w * (double)grad_out[s]);
+    }
+}
+
+// d x_k / d a_k and d x_k / d b_k of node j on axis k.
+__device__ inline void node_grad(int j, int n, int rule, const double* glx, double* da, double* db) {
+    if (rule == TQ_RULE_GAUSS_LEGENDRE) {
+        const double r = glx[j];
+        *da = 0.5 * (1.0 - r);
+        *db = 0.5 * (1.0 + r);
+    } else {
+        const double t = (n > 1) ? (double)j / (double)(n - 1) : 0.0;
+        *da = 1.0 - t;
+        *db = t;
+    }
+}
+
+// Column-fixed threads over grad_points[P, d]: block size is a multiple of d.
+struct Strides {
+    long long s[kMaxDimPoints];
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_grid_points_backward(int d, int n, int rule, const double* glx,
+                                                                 Strides strides, long long P,
+                                                                 const T* gp, double* partials /* [blocks][d][2] */) {
+    __shared__ double s_a[kBlock], s_b[kBlock];
+    const long long bd = blockDim.x;
+    const long long total = P * d;
+    const long long stride = (long long)gridDim.x * bd;
+    double acc_a = 0.0, acc_b = 0.0;
+    const long long e0 = (long long)blockIdx.x * bd + threadIdx.x;
+    const int k = (int)(e0 % d);
+    const long long st = strides.s[k];
+    for (long long e = e0; e < total; e += stride) {
+        const long long p = e / d;
+        const int j = (int)((p / st) % n);
+        double da, db;
+        node_grad(j, n, rule, glx, &da, &db);
+        const double g = (double)gp[e];
+        acc_a = fma(g, da, acc_a);
+        acc_b = fma(g, db, acc_b);
+    }
+    s_a[threadIdx.x] = acc_a;
+    s_b[threadIdx.x] = acc_b;
+    __syncthreads();
+    if (threadIdx.x < d) {
+        double va = 0.0, vb = 0.0;
+        for (long long i = threadIdx.x; i < bd; i += d) {
+            va += s_a[i];
+            vb += s_b[i];
+        }
+        partials[((long long)blockIdx.x * d + threadIdx.x) * 2] = va;
+        partials[((long long)blockIdx.x * d + threadIdx.x) * 2 + 1] = vb;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_grid_points_backward_finalize(const double* partials, long long nb,
+                                                                          int d, T* grad_domain) {
+    __shared__ double s_red[kWavesPerBlock];
+    const int e = blockIdx.x;  // (k, side)
+    double v = 0.0;
+    for (long long b = threadIdx.x; b < nb; b += blockDim.x) v += partials[b * d * 2 + e];
+    v = block_sum(v, s_red);
+    if (threadIdx.x == 0) grad_domain[e] = (T)v;
+}
+
+static void points_backward_shape(int d, long long P, int* bd, long long* nb) {
+    *bd = kBlock - (kBlock % d);
+    long long b = (P * d + (long long)(*bd) * 32 - 1) / ((long long)(*bd) * 32);
+    if (b > 2048) b = 2048;
+    if (b < 1) b = 1;
+    *nb = b;
+}
+
+}  // namespace tq
+
+using namespace tq;
+
+extern "C" {
+
+size_t tq_grid_workspace_size(const tq_grid_desc* g) {
+    if (check_grid_desc(g, TQ_MAX_DIM)) return 0;
+    GridGeom G;
+    if (make_geom(g, G)) return 0;
+    return (size_t)G.blocks * sizeof(double);
+}
+
+int tq_grid_integrate(const tq_grid_desc* g, const tq_integrand* f, const void* domain, void* workspace,
+                      size_t workspace_bytes, void* out, double* out_f64, void* stream) {
+    int rc = check_grid_desc(g, TQ_MAX_DIM);
+    if (rc) return rc;
+    rc = check_gl_tables(g);
+    if (rc) return rc;
+    rc = check_integrand(f, g->dim);
+    if (rc) return rc;
+    if (!domain) return fail(TQ_E_NULL, "domain is NULL");
+    if (!out && !out_f64) return fail(TQ_E_NULL, "no output pointer");
+    if (g->dtype == TQ_F32)
+        return launch_grid<float>(g, f, domain, workspace, workspace_bytes, out, out_f64, as_stream(stream));
+    return launch_grid<double>(g, f, domain, workspace, workspace_bytes, out, out_f64, as_stream(stream));
+}
+
+int tq_grid_points(const tq_grid_desc* g, const void* domain, void* points, void* stream) {
+    int rc = check_grid_desc(g, kMaxDimPoints);
+    if (rc) return rc;
+    rc = check_gl_tables(g);
+    if (rc) return rc;
+    if (!domain || !points) return fail(TQ_E_NULL, "domain/points is NULL");
+    long long P;
+    ipow_checked(g->n, g->dim, 1LL << 62, &P);
+    long long blocks = (P + kBlock - 1) / kBlock;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    if (g->dtype == TQ_F32)
+        hipLaunchKernelGGL((k_grid_points<float>), dim3((unsigned)blocks), dim3(kBlock), 0, as_stream(stream),
+                           (const float*)domain, g->dim, g->n, g->rule, g->gl_nodes, P, (float*)points);
+    else
+        hipLaunchKernelGGL((k_grid_points<double>), dim3((unsigned)blocks), dim3(kBlock), 0, as_stream(stream),
+                           (const double*)domain, g->dim, g->n, g->rule, g->gl_nodes, P, (double*)points);
+    return check_launch("k_grid_points");
+}
+
+int tq_grid_h(const tq_grid_desc* g, const void* domain, void* h_out, void* stream) {
+    int rc = check_grid_desc(g, kMaxDimPoints);
+    if (rc) return rc;
+    rc = check_gl_tables(g);
+    if (rc) return rc;
+    if (!domain || !h_out) return fail(TQ_E_NULL, "domain/h is NULL");
+    const int blocks = (g->dim + 63) / 64;
+    if (g->dtype == TQ_F32)
+        hipLaunchKernelGGL((k_grid_h<float>), dim3(blocks), dim3(64), 0, as_stream(stream), (const float*)domain,
+                           g->dim, g->n, g->rule, g->gl_nodes, (float*)h_out);
+    else
+        hipLaunchKernelGGL((k_grid_h<double>), dim3(blocks), dim3(64), 0, as_stream(stream), (const double*)domain,
+                           g->dim, g->n, g->rule, g->gl_nodes, (double*)h_out);
+    return check_launch("k_grid_h");
+}
+
+size_t tq_grid_reduce_workspace_size(const tq_grid_desc* g, int64_t S) {
+    if (check_grid_desc(g, kMaxDimPoints) || S < 1) return 0;
+    ReduceGeom R;
+    make_reduce_geom(g, S, R);
+    return (size_t)(R.blocks * S) * sizeof(double);
+}
+
+int tq_grid_reduce(const tq_grid_desc* g, const void* domain, const void* hs, int32_t unit_weights,
+                   const void* fvals, int64_t S, void* workspace, size_t workspace_bytes, void* out,
+                   void* stream) {
+    int rc = check_grid_desc(g, kMaxDimPoints);
+    if (rc) return rc;
+    if (!unit_weights) {
+        rc = check_gl_tables(g);
+        if (rc) return rc;
+    }
+    if (S < 1) return fail(TQ_E_ARG, "S must be >= 1");
+    if (!domain || !fvals || !out) return fail(TQ_E_NULL, "domain/fvals/out is NULL");
+    ReduceGeom R;
+    make_reduce_geom(g, S, R);
+    if (R.S > 65535) return fail(TQ_E_UNSUPPORTED, "more than 65535 integrand components");
+    const size_t need = (size_t)(R.blocks * S) * sizeof(double);
+    if (!workspace || workspace_bytes < need) return fail(TQ_E_WORKSPACE, "reduce workspace too small");
+    // unit weights: every point has weight 1 (the reduce kernel still runs the
+    // rule's pattern, so route through the Gauss-Legendre branch with ones).
+    if (unit_weights) R.rule = -1;
+    hipStream_t st = as_stream(stream);
+    const int unit = unit_weights ? 1 : 0;
+    if (g->dtype == TQ_F32) {
+        hipLaunchKernelGGL((k_grid_reduce<float>), dim3((unsigned)R.blocks), dim3(kBlock), 0, st, R,
+                           (const float*)fvals, g->gl_weights, (double*)workspace);
+        rc = check_launch("k_grid_reduce");
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_grid_reduce_finalize<float>), dim3((unsigned)S), dim3(kBlock), 0, st,
+                           (const double*)workspace, R.blocks, (long long)S, (const float*)domain,
+                           (const float*)hs, unit, g->dim, g->n, g->rule, (float*)out);
+    } else {
+        hipLaunchKernelGGL((k_grid_reduce<double>), dim3((unsigned)R.blocks), dim3(kBlock), 0, st, R,
+                           (const double*)fvals, g->gl_weights, (double*)workspace);
+        rc = check_launch("k_grid_reduce");
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_grid_reduce_finalize<double>), dim3((unsigned)S), dim3(kBlock), 0, st,
+                           (const double*)workspace, R.blocks, (long long)S, (const double*)domain,
+                           (const double*)hs, unit, g->dim, g->n, g->rule, (double*)out);
+    }
+    return check_launch("k_grid_reduce_finalize");
+}
+
+int tq_grid_reduce_backward(const tq_grid_desc* g, const void* domain, const void* hs, int32_t unit_weights,
+                            const void* grad_out, int64_t S, void* grad_f, void* stream) {
+    int rc = check_grid_desc(g, kMaxDimPoints);
+    if (rc) return rc;
+    if (!unit_weights) {
+        rc = check_gl_tables(g);
+        if (rc) return rc;
+    }
+    if (S < 1) return fail(TQ_E_ARG, "S must be >= 1");
+    if (!domain || !grad_out || !grad_f) return fail(TQ_E_NULL, "domain/grad_out/grad_f is NULL");
+    long long P;
+    ipow_checked(g->n, g->dim, 1LL << 62, &P);
+    long long blocks = (P + kBlock - 1) / kBlock;
+    if (blocks > 65536) blocks = 65536;
+    const int unit = unit_weights ? 1 : 0;
+    if (g->dtype == TQ_F32)
+        hipLaunchKernelGGL((k_grid_reduce_backward<float>), dim3((unsigned)blocks), dim3(kBlock), 0,
+                           as_stream(stream), (const float*)domain, (const float*)hs, unit, g->dim, g->n, g->rule,
+                           g->gl_weights, (const float*)grad_out, (long long)S, P, (float*)grad_f);
+    else
+        hipLaunchKernelGGL((k_grid_reduce_backward<double>), dim3((unsigned)blocks), dim3(kBlock), 0,
+                           as_stream(stream), (const double*)domain, (const double*)hs, unit, g->dim, g->n,
+                           g->rule, g->gl_weights, (const double*)grad_out, (long long)S, P, (double*)grad_f);
+    return check_launch("k_grid_reduce_backward");
+}
+
+size_t tq_grid_points_backward_workspace_size(const tq_grid_desc* g) {
+    if (check_grid_desc(g, kMaxDimPoints)) return 0;
+    long long P;
+    ipow_checked(g->n, g->dim, 1LL << 62, &P);
+    int bd;
+    long long nb;
+    points_backward_shape(g->dim, P, &bd, &nb);
+    return (size_t)(nb * g->dim * 2) * sizeof(double);
+}
+
+int tq_grid_points_backward(const tq_grid_desc* g, const void* domain, const void* grad_points, void* workspace,
+                            size_t workspace_bytes, void* grad_domain, void* stream) {
+    int rc = check_grid_desc(g, kMaxDimPoints);
+    if (rc) return rc;
+    rc = check_gl_tables(g);
+    if (rc) return rc;
+    if (!domain || !grad_points || !grad_domain) return fail(TQ_E_NULL, "domain/grad_points/grad_domain is NULL");
+    long long P;
+    ipow_checked(g->n, g->dim, 1LL << 62, &P);
+    int bd;
+    long long nb;
+    points_backward_shape(g->dim, P, &bd, &nb);
+    const size_t need = (size_t)(nb * g->dim * 2) * sizeof(double);
+    if (!workspace || workspace_bytes < need) return fail(TQ_E_WORKSPACE, "points backward workspace too small");
+    Strides strides;
+    long long st = 1;
+    for (int k = g->dim - 1; k >= 0; --k) {
+        strides.s[k] = st;
+        st *= g->n;
+    }
+    hipStream_t s = as_stream(stream);
+    if (g->dtype == TQ_F32) {
+        hipLaunchKernelGGL((k_grid_points_backward<float>), dim3((unsigned)nb), dim3(bd), 0, s, g->dim, g->n,
+                           g->rule, g->gl_nodes, strides, P, (const float*)grad_points,
+                           (double*)workspace);
+        rc = check_launch("k_grid_points_backward");
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_grid_points_backward_finalize<float>), dim3(g->dim * 2), dim3(kBlock), 0, s,
+                           (const double*)workspace, nb, g->dim, (float*)grad_domain);
+    } else {
+        hipLaunchKernelGGL((k_grid_points_backward<double>), dim3((unsigned)nb), dim3(bd), 0, s, g->dim, g->n,
+                           g->rule, g->gl_nodes, strides, P, (const double*)grad_points,
+                           (double*)workspace);
+        rc = check_launch("k_grid_points_backward");
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_grid_points_backward_finalize<double>), dim3(g->dim * 2), dim3(kBlock), 0, s,
+                           (const double*)workspace, nb, g->dim, (double*)grad_domain);
+    }
+    return check_launch("k_grid_points_backward_finalize");
+}
+
+}  // extern "C"

@@ -1,0 +1,241 @@
+"""Grid integrators (reference: torchquad/integration/grid_integrator.py:14-255).
+
+``integrate`` dispatches:
+
+* built-in integrand (:mod:`torchquad_amd.integrands`) on a standard rule ->
+  one fused HIP kernel (nodes, weights, integrand and reduction in
+  registers; no points in memory);
+* any other callable -> HIP point writer, the user's ``fn`` on the
+  ``[N, dim]`` points, HIP weighted reduction.  Gradients with respect to
+  the domain and to tensors captured by ``fn`` flow through hand-written
+  backward kernels.
+
+Subclasses that override the rule (``_apply_composite_rule``, ``_grid_func``,
+``_weights``) run their own code on the reference's data layout.
+"""
+import warnings
+
+import torch
+
+from .. import _ops
+from ..integrands import is_builtin
+from ..utils.set_log_level import logger
+from .base_integrator import BaseIntegrator
+from .integration_grid import IntegrationGrid, grid_func as _standard_grid_func
+from .utils import (
+    _float_domain,
+    _setup_integration_domain,
+    _warn_scalar_deprecation,
+    expand_func_values_and_squeeze_integral,
+)
+
+
+class _GridReduceFn(torch.autograd.Function):
+    """out[s] = scale * sum_p W(p) values[p, s] with d/d(values, domain, hs)."""
+
+    @staticmethod
+    def forward(ctx, values, domain, hs, rule, dim, n, unit, gl):
+        out = _ops.grid_reduce(
+            rule, dim, n, domain.detach(), values.detach(),
+            hs.detach() if hs is not None else None, unit, gl,
+        )
+        ctx.meta = (rule, dim, n, unit, gl, values.shape[0], values.dtype, hs is not None)
+        ctx.save_for_backward(domain, hs if hs is not None else domain, out)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        domain, hs_saved, out = ctx.saved_tensors
+        rule, dim, n, unit, gl, npoints, vdtype, has_hs = ctx.meta
+        hs = hs_saved if has_hs else None
+        grad_values = grad_domain = grad_hs = None
+        if ctx.needs_input_grad[0]:
+            grad_values = _ops.grid_reduce_backward(
+                rule, dim, n, domain.detach(), grad_out, npoints, hs, unit, gl
+            ).to(vdtype)
+        gs = torch.sum(grad_out * out)
+        gauss = unit or rule == _ops.RULES["gauss_legendre"]
+        if ctx.needs_input_grad[2] and has_hs and not gauss:
+            grad_hs = gs / hs.detach()
+        if ctx.needs_input_grad[1]:
+            if gauss or not has_hs:
+                width = (domain[:, 1] - domain[:, 0]).detach()
+                gw = gs / width
+                grad_domain = torch.stack([-gw, gw], dim=1).to(domain.dtype)
+            else:
+                grad_domain = torch.zeros_like(domain)
+        return grad_values, grad_domain, grad_hs, None, None, None, None, None
+
+
+def _needs_grad(*tensors):
+    return torch.is_grad_enabled() and any(
+        t is not None and isinstance(t, torch.Tensor) and t.requires_grad for t in tensors
+    )
+
+
+def kernel_reduce(rule, dim, n, domain, values, hs=None, unit=False, gl=None):
+    """Differentiable HIP weighted reduction of values[n^dim, *S]."""
+    if _needs_grad(values, domain, hs):
+        return _GridReduceFn.apply(values, domain, hs, rule, dim, n, unit, gl)
+    return _ops.grid_reduce(rule, dim, n, domain, values, hs, unit, gl)
+
+
+class GridIntegrator(BaseIntegrator):
+    """Abstract grid integrator (Newton-Cotes and Gaussian)."""
+
+    # rule code of the standard rule this class implements (None: abstract)
+    _tq_rule = None
+
+    def __init__(self):
+        super().__init__()
+
+    @property
+    def _grid_func(self):
+        return _standard_grid_func
+
+    def _weights(self, N, dim, backend, requires_grad=False):
+        return None
+
+    # -- dispatch helpers -------------------------------------------------
+    def _tq_standard(self):
+        """True when the class uses its standard rule unchanged."""
+        owner = getattr(type(self), "_tq_owner", None)
+        if owner is None or self._tq_rule is None:
+            return False
+        cls = type(self)
+        return (
+            cls._apply_composite_rule is owner._apply_composite_rule
+            and cls._grid_func is owner._grid_func
+            and cls._weights is owner._weights
+            and cls.calculate_result is owner.calculate_result
+        )
+
+    def _gl_tables(self, n, device):
+        return None
+
+    def _grid_n(self, N, dim):
+        """n per axis after _adjust_N and IntegrationGrid's checks (integration_grid.py:64, 105-124)."""
+        N = self._adjust_N(dim=dim, N=N)
+        if N < 2:
+            raise ValueError("N has to be > 1.")
+        if N ** (1.0 / dim) < 2:
+            raise ValueError(
+                "Cannot create a ", dim, "-dimensional grid with ", N, " points. Too few points per dimension."
+            )
+        return int(N ** (1.0 / dim) + 1e-8)
+
+    # -- public API -------------------------------------------------------
+    def integrate(self, fn, dim, N, integration_domain, backend):
+        """Integrate fn over the domain with the composite rule (grid_integrator.py:32-55)."""
+        if N is None:
+            N = self._get_minimal_N(dim)
+        integration_domain = _setup_integration_domain(dim, integration_domain, backend)
+        self._check_inputs(dim=dim, N=N, integration_domain=integration_domain)
+        domain = _float_domain(integration_domain)
+
+        if self._tq_standard() and is_builtin(fn, dim) and not _needs_grad(domain):
+            n = self._grid_n(N, dim)
+            gl = self._gl_tables(n, domain.device)
+            self._nr_of_fevals = n**dim
+            _warn_scalar_deprecation()
+            return _ops.grid_integrate(self._tq_rule, fn, dim, n, domain, gl)
+
+        if self._tq_standard():
+            # user callable: HIP points -> fn -> HIP weighted reduction with the
+            # rule's own weights (no [N] weight tensor is built)
+            grid_points, hs, n_per_dim = self.calculate_grid(N, domain)
+            logger.debug("Evaluating integrand on the grid.")
+            function_values, num_points = self.evaluate_integrand(fn, grid_points)
+            self._nr_of_fevals = num_points
+            return self._standard_result(function_values, dim, n_per_dim, hs, domain, unit=False)
+
+        grid_points, hs, n_per_dim = self.calculate_grid(N, integration_domain)
+        function_values, num_points = self.evaluate_integrand(
+            fn, grid_points, weights=self._weights(n_per_dim, dim, backend)
+        )
+        self._nr_of_fevals = num_points
+        return self.calculate_result(function_values, dim, n_per_dim, hs, integration_domain)
+
+    @expand_func_values_and_squeeze_integral
+    def _standard_result(self, function_values, dim, n_per_dim, hs, integration_domain, unit=False):
+        return self._kernel_result(function_values, dim, n_per_dim, hs, integration_domain, unit)
+
+    def _kernel_result(self, function_values, dim, n_per_dim, hs, integration_domain, unit):
+        expected = n_per_dim**dim
+        if function_values.shape[0] != expected:
+            raise ValueError(
+                f"reshaping produced shape {list(function_values.shape)}, expected {expected} points "
+                f"({n_per_dim}^{dim})"
+            )
+        domain = _float_domain(integration_domain)
+        gl = self._gl_tables(n_per_dim, domain.device)
+        gauss = self._tq_rule == _ops.RULES["gauss_legendre"]
+        return kernel_reduce(
+            self._tq_rule, dim, n_per_dim, domain, function_values,
+            None if gauss else hs, unit, gl,
+        )
+
+    @expand_func_values_and_squeeze_integral
+    def calculate_result(self, function_values, dim, n_per_dim, hs, integration_domain):
+        """Apply the composite rule to evaluated values (grid_integrator.py:57-91).
+
+        Standard rules reduce with the HIP weighted-reduction kernel; a
+        subclass with its own ``_apply_composite_rule`` gets the reference's
+        ``[*S, n, ..., n]`` layout.
+        """
+        if self._tq_standard() and isinstance(function_values, torch.Tensor) and function_values.is_cuda:
+            unit = self._tq_rule == _ops.RULES["gauss_legendre"]
+            return self._kernel_result(function_values, dim, n_per_dim, hs, integration_domain, unit)
+        integrand_shape = function_values.shape[1:]
+        new_shape = [*integrand_shape, *([n_per_dim] * dim)]
+        perm = list(range(1, function_values.dim())) + [0]
+        reshaped = function_values.permute(*perm).reshape(new_shape)
+        logger.debug("Computing areas.")
+        return self._apply_composite_rule(reshaped, dim, hs, integration_domain)
+
+    def calculate_grid(self, N, integration_domain, disable_integration_domain_check=False):
+        """Grid points, mesh widths and points per axis (grid_integrator.py:93-127)."""
+        N = self._adjust_N(dim=integration_domain.shape[0], N=N)
+        grid = IntegrationGrid(N, integration_domain, self._grid_func, disable_integration_domain_check)
+        return grid.points, grid.h, grid._N
+
+    @staticmethod
+    def _adjust_N(dim, N):
+        return N
+
+    def get_jit_compiled_integrate(self, dim, N=None, integration_domain=None, backend=None):
+        """Integrate function with N, dim and rule fixed (grid_integrator.py:134-255).
+
+        The reference traces the grid and result steps with torch.jit; here
+        both are single kernel launches already, so the returned function
+        caches the validated parameters and launches directly.
+        """
+        if N is None:
+            N = self._get_minimal_N(dim)
+        integration_domain = _setup_integration_domain(dim, integration_domain, backend)
+        self._check_inputs(dim=dim, N=N, integration_domain=integration_domain)
+
+        def compiled_integrate(fn, integration_domain):
+            return self.integrate(fn, dim, N, integration_domain, backend)
+
+        return compiled_integrate
+
+
+def _composite_by_kernel(rule, cur_dim_areas, dim, hs, domain):
+    """_apply_composite_rule on the reference layout [*S, n, ..., n] via the HIP reduction."""
+    n = int(cur_dim_areas.shape[-1])
+    S = list(cur_dim_areas.shape[: cur_dim_areas.dim() - dim])
+    nd = len(S)
+    perm = list(range(nd, cur_dim_areas.dim())) + list(range(nd))
+    values = cur_dim_areas.permute(*perm).reshape([n**dim, *S])
+    dom = _float_domain(domain)
+    unit = rule == _ops.RULES["gauss_legendre"]
+    return kernel_reduce(rule, dim, n, dom, values, None if unit else hs, unit, None)
+
+
+def _mark_owner(cls):
+    cls._tq_owner = cls
+    return cls
+
+
+__all__ = ["GridIntegrator", "kernel_reduce", "_composite_by_kernel", "_mark_owner"]

@@ -1,0 +1,186 @@
+"""Domain handling and helpers shared by the integrators.
+
+Mirrors torchquad/integration/utils.py (the torch backend):
+``_linspace_with_grads`` (:24-58), ``_setup_integration_domain`` (:110-159),
+``_check_integration_domain`` (:162-206) and
+``expand_func_values_and_squeeze_integral`` (:235-277).  Error messages and
+warnings are the reference's.
+"""
+import warnings
+import weakref
+
+import torch
+
+from ..utils.set_log_level import logger
+from ..utils.set_precision import _get_precision
+from ..utils.set_up_backend import _get_default_backend
+
+_SUPPORTED_BACKENDS = ("torch",)
+
+
+def _default_device():
+    """GPU when one is visible (this package computes only there), else CPU."""
+    dev = torch.get_default_device() if hasattr(torch, "get_default_device") else torch.device("cpu")
+    if dev.type == "cuda":
+        return dev
+    if torch.cuda.is_available():
+        return torch.device("cuda", torch.cuda.current_device())
+    return dev
+
+
+def _linspace_with_grads(start, stop, N, requires_grad):
+    """Equally spaced 1-D grid that keeps gradients w.r.t. start/stop (utils.py:24-58)."""
+    if requires_grad:
+        zero = start.new_zeros(())
+        one = start.new_ones(())
+        grid = torch.linspace(zero, one, N, dtype=start.dtype, device=start.device)
+        grid = grid * (stop - start)
+        grid = grid + start
+        return grid
+    return torch.linspace(start, stop, N, dtype=start.dtype, device=start.device)
+
+
+def _setup_integration_domain(dim, integration_domain, backend):
+    """List/None -> tensor [dim, 2]; validate the shape (utils.py:110-159).
+
+    A list becomes a tensor on the GPU with dtype from TORCHQUAD_DTYPE_TORCH
+    or torch's default dtype.
+    """
+    if backend is not None and backend not in _SUPPORTED_BACKENDS:
+        raise ValueError(
+            f"Unsupported numerical backend: {backend}. torchquad_amd computes with backend='torch'."
+        )
+    if integration_domain is None:
+        integration_domain = [[-1.0, 1.0]] * dim
+
+    is_tensor = isinstance(integration_domain, torch.Tensor)
+    if not is_tensor and hasattr(integration_domain, "__array__") and not isinstance(
+        integration_domain, (list, tuple)
+    ):
+        # numpy arrays etc.: the reference keeps their backend; here they are
+        # converted like a list when the torch backend is requested.
+        if backend is None and _get_default_backend() not in _SUPPORTED_BACKENDS:
+            raise ValueError("torchquad_amd computes with backend='torch' only")
+        warning_msg = "integration_domain should be a list when the backend argument is set."
+        logger.warning(warning_msg)
+        warnings.warn(warning_msg, RuntimeWarning)
+        integration_domain = integration_domain.tolist()
+
+    if not isinstance(integration_domain, torch.Tensor):
+        integration_domain = [[float(b) for b in bounds] for bounds in integration_domain]
+        dtype_name = _get_precision("torch")
+        dtype = {"float32": torch.float32, "float64": torch.float64}.get(dtype_name, torch.get_default_dtype())
+        _check_list_domain(integration_domain)
+        integration_domain = torch.tensor(integration_domain, dtype=dtype, device=_default_device())
+        _mark_checked(integration_domain)
+
+    if tuple(integration_domain.shape) != (dim, 2):
+        raise ValueError(
+            "The integration domain has an unexpected shape. "
+            f"Expected {(dim, 2)}, got {tuple(integration_domain.shape)}"
+        )
+    return integration_domain
+
+
+def _check_list_domain(integration_domain):
+    dim = len(integration_domain)
+    if dim < 1:
+        raise ValueError("len(integration_domain) needs to be 1 or larger.")
+    for bounds in integration_domain:
+        if len(bounds) != 2:
+            raise ValueError(bounds, " in ", integration_domain, " does not specify a valid integration bound.")
+        if bounds[0] > bounds[1]:
+            raise ValueError(bounds, " in ", integration_domain, " does not specify a valid integration bound.")
+    return dim
+
+
+# Tensors whose bounds were validated, keyed by identity and in-place
+# version counter: validation of a device tensor needs one device->host read,
+# which is paid once per tensor instead of once per call.
+_checked = {}
+
+
+def _mark_checked(t):
+    key = id(t)
+    try:
+        ref = weakref.ref(t, lambda _r, k=key: _checked.pop(k, None))
+    except TypeError:
+        return
+    _checked[key] = (ref, t.data_ptr(), t._version)
+
+
+def _is_marked(t):
+    entry = _checked.get(id(t))
+    if entry is None:
+        return False
+    ref, p, v = entry
+    return ref() is t and p == t.data_ptr() and v == t._version
+
+
+def _check_integration_domain(integration_domain):
+    """Validate shape and bounds and return the dimension (utils.py:162-206)."""
+    if not isinstance(integration_domain, torch.Tensor):
+        return _check_list_domain(integration_domain)
+    if len(integration_domain.shape) != 2:
+        raise ValueError("The integration_domain tensor has an invalid shape")
+    dim, num_bounds = integration_domain.shape
+    if dim < 1:
+        raise ValueError("integration_domain.shape[0] needs to be 1 or larger.")
+    if num_bounds != 2:
+        raise ValueError("integration_domain must have 2 values per boundary")
+    if torch.jit.is_tracing():
+        return dim
+    if _is_marked(integration_domain):
+        return dim
+    if torch.min(integration_domain[:, 1] - integration_domain[:, 0]).item() < 0.0:
+        raise ValueError("integration_domain has invalid boundary values")
+    _mark_checked(integration_domain)
+    return dim
+
+
+def _float_domain(integration_domain):
+    """Integer domains become float64 (integration_grid.py:52-56); contiguous."""
+    if not integration_domain.dtype.is_floating_point:
+        integration_domain = integration_domain.to(torch.float64)
+    return integration_domain.contiguous()
+
+
+def expand_func_values_and_squeeze_integral(f):
+    """Decorator: [N] values become [N, 1] and the result is squeezed, with the
+    reference's deprecation warning (utils.py:235-277)."""
+
+    def wrap(*args, **kwargs):
+        if len(args) > 1:
+            function_values = args[1]
+        elif "function_values" in kwargs:
+            function_values = kwargs["function_values"]
+        else:
+            raise ValueError(
+                "function_values argument not found in either positional or keyword arguments. "
+                "Please provide function_values as the second positional argument or as a keyword argument."
+            )
+        is_1d = len(function_values.shape) == 1 or (
+            len(function_values.shape) == 2 and function_values.shape[1] == 1
+        )
+        if is_1d:
+            warnings.warn(
+                "DEPRECATION WARNING: In future versions of torchquad, an array-like object will be returned."
+            )
+            if len(args) > 1:
+                args = (args[0], torch.unsqueeze(function_values, 1) if function_values.dim() == 1
+                        else function_values, *args[2:])
+            else:
+                fv = kwargs["function_values"]
+                kwargs["function_values"] = torch.unsqueeze(fv, 1) if fv.dim() == 1 else fv
+            result = f(*args, **kwargs)
+            return torch.squeeze(result)
+        return f(*args, **kwargs)
+
+    return wrap
+
+
+def _warn_scalar_deprecation():
+    """The warning the reference emits for every scalar integrand (utils.py:264-266)."""
+    warnings.warn(
+        "DEPRECATION WARNING: In future versions of torchquad, an array-like object will be returned."
+    )
