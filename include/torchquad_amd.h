@@ -171,6 +171,127 @@ int tq_mc_reduce(int32_t dtype, const void* fvals, int64_t N, int64_t S, const v
 int tq_uniform(uint64_t seed, uint64_t offset, int32_t dtype, int64_t rows, int32_t cols,
                void* out, void* stream);
 
+/* ---- VEGAS ----------------------------------------------------------------
+ * Device-resident VEGAS+ state (reference: vegas_map.py, vegas_stratification.py,
+ * vegas.py).  Every buffer is caller-owned device memory; the library only
+ * reads/writes it.  Map and stratification state are FP64; samples and the
+ * integrand use sample_dtype.  Capacities: jf/x/jac/ids hold max_samples
+ * samples, the partial buffers hold max_blocks sample blocks of
+ * TQ_VEGAS_BLOCK_SAMPLES samples each. */
+#define TQ_VEGAS_BLOCK_SAMPLES 1024
+#define TQ_VEGAS_MAX_ITERS 4096
+
+typedef struct tq_vegas_state {
+    int32_t dim;
+    int32_t n_intervals;     /* map bins per axis (VEGASMap N_intervals)            */
+    int32_t n_strat;         /* stratification steps per axis                       */
+    int32_t sample_dtype;    /* TQ_F32 / TQ_F64                                     */
+    int64_t n_cubes;         /* n_strat^dim                                         */
+    int64_t max_samples;     /* capacity of per-sample buffers                      */
+    int64_t max_blocks;      /* capacity of per-block partial buffers               */
+    double v_cubes;          /* n_strat^-dim                                        */
+    double alpha;            /* map smoothing exponent (vegas_map.py:15)            */
+    double beta;             /* stratification damping (vegas_stratification.py:13) */
+    double* x_edges;         /* [dim][n_intervals+1]                                */
+    double* dx_edges;        /* [dim][n_intervals]                                  */
+    double* weights;         /* [dim][n_intervals]  accumulated (f J)^2             */
+    double* counts;          /* [dim][n_intervals]  accumulated sample counts       */
+    double* dh;              /* [n_cubes] normalised damped counts                  */
+    double* jf_cube;         /* [n_cubes] JF                                        */
+    double* jf2_cube;        /* [n_cubes] JF2                                       */
+    int64_t* nh;             /* [n_cubes] samples per cube                          */
+    int64_t* offsets;        /* [n_cubes+1] exclusive scan of nh                    */
+    int64_t* chunk_cube;     /* [max_blocks+1] cube of the first sample of a block  */
+    double* jf;              /* [max_samples] per-sample f*J                        */
+    double* hist_w;          /* [max(max_blocks*dim*n_intervals, 5*dim*(n_intervals+1))]
+                                block partials, then update_map scratch              */
+    uint32_t* hist_c;        /* [max_blocks][dim][n_intervals] block partials       */
+    double* block_sums;      /* [max(max_blocks, ceil(n_cubes/256))][4] scratch     */
+    int64_t* scan_sums;      /* [ceil(n_cubes/2048)+1] scratch                      */
+    double* iter_stats;      /* [TQ_VEGAS_MAX_ITERS][4] = result, sigma2, evals, dh sum */
+    int32_t* flags;          /* [4] = map not updated, infinite edges, unrepaired, spare */
+} tq_vegas_state;
+
+/* Resets map (uniform edges), stratification (dh = 1/n_cubes) and flags
+ * (VEGASMap.__init__ vegas_map.py:15-42, VEGASStratification.__init__
+ * vegas_stratification.py:13-44). */
+int tq_vegas_init(const tq_vegas_state* st, void* stream);
+
+/* get_NH (vegas_stratification.py:92-103): dh <- dh / dh_sum when
+ * normalize_it >= 0 (the update_DH normalisation, :86-90, of iteration
+ * normalize_it), nh = max(2, floor(dh * n_exp)), then offsets, the block
+ * cube index and iter_stats[it].evals = sum nh.  nh_in != NULL uses the
+ * given counts instead (step API). */
+int tq_vegas_nh(const tq_vegas_state* st, int64_t n_exp, int32_t normalize_it, int32_t it,
+                const int64_t* nh_in, void* stream);
+
+/* One fused VEGAS iteration body after tq_vegas_nh (vegas.py:268-315):
+ * stratified samples y (get_Y, vegas_stratification.py:140-165), map
+ * (get_X/get_Jac, vegas_map.py:44-74), the built-in integrand on
+ * x*(b-a)+a times the volume (vegas.py:104-112), per-sample jf, map
+ * histograms (accumulate_weight, vegas_map.py:99-111), then the per-cube
+ * sums, ih, sigma^2 and dh (vegas_stratification.py:46-90) into
+ * iter_stats[it].  Sample s uses Philox counter offset + s.
+ * warmup != 0: n_warm plain uniform samples y = u*0.999999 instead
+ * (vegas.py:211-266), no stratification. */
+int tq_vegas_iterate(const tq_vegas_state* st, const tq_integrand* f, const void* domain, uint64_t seed,
+                     uint64_t offset, int32_t it, int32_t use_grid_improve, int32_t warmup, int64_t n_warm,
+                     void* stream);
+
+/* Split form for user integrands: write the samples' domain points
+ * x[total, dim] (sample_dtype), Jacobians jac[total] (FP64), bin ids
+ * ids[total, dim] (int32) and optionally y[total, dim].  u_in (optional,
+ * [total, dim] sample_dtype) supplies the uniforms instead of the Philox
+ * stream, for caller-provided RNG objects (vegas.py:98-102) ... */
+int tq_vegas_points(const tq_vegas_state* st, const void* domain, uint64_t seed, uint64_t offset,
+                    int32_t warmup, int64_t n_warm, const void* u_in, void* x_out, double* jac_out,
+                    int32_t* ids_out, void* y_out, void* stream);
+/* ... then consume the user's values f[total] (sample_dtype, already
+ * evaluated at x): jf = f * volume * jac, histograms, per-cube sums. */
+int tq_vegas_consume(const tq_vegas_state* st, const void* fvals, const void* domain, const double* jac,
+                     const int32_t* ids, int32_t it, int32_t use_grid_improve, int32_t warmup, int64_t n_warm,
+                     void* stream);
+
+/* update_map (vegas_map.py:113-261): reduce the histogram partials of the
+ * last n_samples samples (NULL count -> iter_stats[it].evals), smooth,
+ * redistribute edges.  Flags count the "cannot update" and "infinite
+ * edges" cases. */
+int tq_vegas_update_map(const tq_vegas_state* st, int64_t n_samples, int32_t it, void* stream);
+
+/* Map primitives on given y[n, dim] (sample_dtype), for VEGASMap's public
+ * methods: ids (_get_interval_ID :76-85), offsets (_get_interval_offset
+ * :87-97), x (get_X :44-58), jac (get_Jac :60-74).  Any output may be NULL. */
+int tq_vegas_map_eval(const tq_vegas_state* st, const void* y, int64_t n, int64_t* ids_out, void* off_out,
+                      void* x_out, void* jac_out, void* stream);
+/* accumulate_weight (vegas_map.py:99-111) for given y[n, dim] and
+ * jf2[n] (sample_dtype): adds into weights/counts. */
+int tq_vegas_map_accumulate(const tq_vegas_state* st, const void* y, const void* jf2, int64_t n,
+                            void* stream);
+/* update_map from the accumulated weights/counts (no partial reduction),
+ * then zero them (_reset_weight :174-183). */
+int tq_vegas_map_update_accumulated(const tq_vegas_state* st, void* stream);
+/* _smooth_map (vegas_map.py:113-172) on given weights/counts [dim][n]
+ * (FP64): out [dim][n], scratch [dim][2n]; *not_updatable (device int)
+ * becomes 1 when a weight sum is 0 (the reference then returns None). */
+int tq_vegas_smooth(int32_t dim, int32_t n, const double* weights, const double* counts, double alpha,
+                    double* out, double* scratch, int32_t* not_updatable, void* stream);
+
+/* get_Y (vegas_stratification.py:140-165) for the nh/offsets of the last
+ * tq_vegas_nh: y[total, dim] (sample_dtype), Philox counter offset + s, or
+ * the caller's uniforms u_in[total, dim] when given. */
+int tq_vegas_strat_y(const tq_vegas_state* st, uint64_t seed, uint64_t offset, const void* u_in, void* y_out,
+                     void* stream);
+/* accumulate_weight (vegas_stratification.py:46-70) for given per-sample
+ * weights w[total] (FP64) in the offsets of the last tq_vegas_nh: JF, JF2
+ * and iter_stats[it] result/sigma^2. */
+int tq_vegas_strat_accumulate(const tq_vegas_state* st, const double* w, int32_t it, void* stream);
+/* update_DH (vegas_stratification.py:72-90) from JF/JF2/nh except the
+ * normalisation, which the next tq_vegas_nh or tq_vegas_normalize_dh
+ * applies with iter_stats[it].dh_sum. */
+int tq_vegas_update_dh(const tq_vegas_state* st, int32_t it, void* stream);
+/* dh <- dh / iter_stats[it].dh_sum when the sum is non-zero. */
+int tq_vegas_normalize_dh(const tq_vegas_state* st, int32_t it, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

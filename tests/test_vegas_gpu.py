@@ -1,0 +1,227 @@
+"""VEGAS on the GPU vs the oracle restatement and the reference's checks."""
+import math
+import warnings
+from unittest.mock import patch
+
+import numpy as np
+import pytest
+import torch
+
+import torchquad_amd as tq
+from oracle import families as fam
+from oracle import native
+from oracle import vegas as ov
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _quiet():
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        yield
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_map_golden_vectors(ref_vectors, dtype, dev):
+    g = ref_vectors["vegas_map_interval"]
+    m = tq.VEGASMap(g["N_intervals"], 3, "torch", dtype)
+    y = torch.tensor(g["y"], dtype=dtype, device=dev)
+    assert torch.equal(m._get_interval_ID(y).cpu(), torch.tensor(g["ID"]))
+    assert float(torch.max(torch.abs(m._get_interval_offset(y).cpu() - torch.tensor(g["offset"], dtype=dtype)))) <= 6e-5
+    assert float(torch.max(torch.abs(m.get_X(y) - y))) <= 3e-7
+    gc = ref_vectors["vegas_map_counts"]
+    y1 = torch.linspace(0.0, gc["y_max"], gc["N_per_dim"], dtype=dtype, device=dev)
+    yy = torch.stack([a.ravel() for a in torch.meshgrid(y1, y1, y1, indexing="ij")], 1)
+    f = torch.prod(torch.exp(yy), dim=1)
+    jac = m.get_Jac(yy)
+    assert jac.dtype == dtype and float(torch.max(torch.abs(jac - 1.0))) < 1e-14
+    m.accumulate_weight(yy, (f * jac) ** 2)
+    w = m.weights
+    assert float(torch.min(w[:, 1:] - w[:, :-1])) > 0.0
+    assert int(torch.max(torch.abs(m.counts - gc["count"]))) == 0
+    gs = ref_vectors["vegas_smooth_map"]
+    sw = tq.VEGASMap._smooth_map(torch.tensor(gs["weights"], dtype=dtype, device=dev),
+                                 torch.ones((2, 6), dtype=torch.int64, device=dev), 0.5)
+    assert float(torch.max(torch.abs(sw.cpu() - torch.tensor(gs["expected"], dtype=dtype)))) <= 3e-7
+    m.update_map()
+    xe, dxe = m.x_edges, m.dx_edges
+    assert float(torch.max(torch.abs(xe[:, [0, -1]] - torch.tensor([[0.0, 1.0]] * 3, dtype=dtype, device=dev)))) == 0
+    assert float(torch.max(torch.abs(torch.sum(dxe, 1) - 1.0))) < 3e-7
+    assert float(torch.min(dxe)) > 0 and float(torch.max(dxe[:, 1:] - dxe[:, :-1])) < 0
+    assert float(torch.max(torch.abs(m.get_X(yy)[0]))) == 0.0
+
+
+def test_map_update_matches_oracle(dev):
+    rng = np.random.default_rng(3)
+    for N, dim in ((2, 1), (7, 3), (50, 6)):
+        m = tq.VEGASMap(N, dim, "torch", torch.float64)
+        om = ov.Map(N, dim)
+        w = rng.uniform(0.0, 3.0, (dim, N))
+        c = rng.integers(0, 5, (dim, N))
+        w[c == 0] = 0.0
+        m._dev.weights.copy_(torch.tensor(w))
+        m._dev.counts.copy_(torch.tensor(c, dtype=torch.float64))
+        om.weights, om.counts = w.copy(), c.copy()
+        sw_dev = tq.VEGASMap._smooth_map(torch.tensor(w, device=dev), torch.tensor(c, device=dev), 0.5)
+        sw_ref = ov.Map.smooth(w, c, 0.5)
+        if sw_ref is None:
+            assert sw_dev is None
+            continue
+        np.testing.assert_allclose(sw_dev.cpu().numpy(), sw_ref, rtol=1e-14, atol=1e-15)
+        m.update_map()
+        om.update_map()
+        np.testing.assert_allclose(m.x_edges.cpu().numpy(), om.x_edges, rtol=0, atol=1e-14)
+
+
+def test_zero_weights_cannot_update(dev):
+    m = tq.VEGASMap(4, 2, "torch", torch.float64)
+    before = m.x_edges.clone()
+    with pytest.warns(RuntimeWarning, match="Cannot update the VEGASMap"):
+        m.update_map()
+    assert torch.equal(before, m.x_edges)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_stratification_bit_exact(dtype, dev, ref_vectors):
+    g = ref_vectors["vegas_strat_fresh"]
+    rng = tq.RNG(seed=11)
+    st = tq.VEGASStratification(g["N_increment"], g["dim"], rng, "torch", dtype)
+    ost = ov.Strat(g["N_increment"], g["dim"], np.float32 if dtype == torch.float32 else np.float64)
+    assert st.N_strat == ost.N_strat == g["N_strat"]
+    nh = st.get_NH(g["nevals_exp"])
+    assert nh.dtype == torch.int64 and torch.all(nh == g["nh"])
+    y = st.get_Y(nh)
+    u = native.uniform(11, 0, int(nh.sum()), 3, np.float32 if dtype == torch.float32 else np.float64)
+    assert np.array_equal(y.cpu().numpy(), ost.get_Y(nh.cpu().numpy(), u))
+    f = torch.prod(torch.exp(y), dim=1)
+    jf, jf2 = st.accumulate_weight(nh, f)
+    ojf, ojf2 = ost.accumulate_weight(nh.cpu().numpy(), f.double().cpu().numpy())
+    np.testing.assert_allclose(jf.double().cpu().numpy(), ojf, rtol=1e-12 if dtype == torch.float64 else 1e-6)
+    st.update_DH()
+    ost.update_DH()
+    np.testing.assert_allclose(st._dev.dh.cpu().numpy(), ost.dh, rtol=1e-12)
+    # given the same dh, get_NH is bit-exact for several expected counts
+    ost.dh = st._dev.dh.cpu().numpy()
+    for n_exp in (4000, 123457, 10**6):
+        assert np.array_equal(st.get_NH(n_exp).cpu().numpy(), ost.get_NH(n_exp))
+    assert int(st.get_NH(4000)[-1]) > int(st.get_NH(4000)[0])
+
+
+def test_fused_end_to_end_fp64_matches_oracle(dev):
+    dim, N, seed = 3, 60000, 5
+    dom = [[0.0, 1.0], [-0.5, 1.0], [0.2, 0.9]]
+    f = tq.DoubleGaussian(dim, width=0.3)
+    v = tq.VEGAS()
+    val = float(v.integrate(f, dim, N=N, integration_domain=torch.tensor(dom, dtype=torch.float64, device=dev),
+                            seed=seed))
+    r = ov.run(lambda x: fam.double_gaussian(x, 0.3), dim, N, dom, seed=seed)
+    assert v._nr_of_fevals == r["fevals"]
+    assert abs(val - r["result"]) <= 1e-10 * abs(r["result"])
+    # map state after the run
+    np.testing.assert_allclose(v._dev.x_edges.cpu().numpy(), r["map"].x_edges, atol=1e-12)
+
+
+def test_user_callable_end_to_end_fp64_matches_oracle(dev):
+    dim, N, seed = 2, 20000, 7
+    dom = [[0.0, 2.0], [-1.0, 1.0]]
+    fn = lambda x: torch.exp(-((x[:, 0] - 1.2) ** 2) * 4.0 - (x[:, 1] ** 2) * 2.0)  # noqa: E731
+    v = tq.VEGAS()
+    val = float(v.integrate(fn, dim, N=N, integration_domain=torch.tensor(dom, dtype=torch.float64, device=dev),
+                            seed=seed))
+    r = ov.run(lambda x: np.exp(-((x[:, 0] - 1.2) ** 2) * 4.0 - (x[:, 1] ** 2) * 2.0), dim, N, dom, seed=seed)
+    assert v._nr_of_fevals == r["fevals"]
+    assert abs(val - r["result"]) <= 1e-10 * abs(r["result"])
+    # fused built-in of the same function agrees with the callable path
+    g = tq.GaussianPeak([2.0, math.sqrt(2.0)], [1.2, 0.0])
+    val2 = float(tq.VEGAS().integrate(g, dim, N=N, integration_domain=torch.tensor(dom, dtype=torch.float64,
+                                                                                    device=dev), seed=seed))
+    assert abs(val2 - val) <= 1e-9 * abs(val)
+
+
+def test_fp32_statistics(configs, dev):
+    c = configs["C3"]
+    v = tq.VEGAS()
+    f = tq.DoubleGaussian(6, width=0.1)
+    dom = torch.tensor(c["domain"], dtype=torch.float32, device=dev)
+    val = float(v.integrate(f, 6, N=2_000_000, integration_domain=dom, seed=0, N_intervals=50))
+    err = v._get_error()
+    assert abs(val - c["analytic"]) <= max(5 * err, 0.01 * c["analytic"]), (val, c["analytic"], err)
+
+
+def test_reference_special_cases(ref_vectors, dev):
+    g = ref_vectors["vegas_special"]
+    v = tq.VEGAS()
+    val = v.integrate(lambda x: x[:, 0] * 0.0, 2, N=10000, integration_domain=[[0.0, 3.0]] * 2, seed=0)
+    assert float(val) == 0.0
+    d64 = torch.tensor(g["constant_domain"], dtype=torch.float64, device=dev)
+    val = v.integrate(lambda x: x[:, 0] * 0.0 + 10.0, 2, N=10000, integration_domain=d64, seed=0)
+    assert abs(float(val) - 90.0) < 1e-13
+    val = v.integrate(tq.Constant(2, 10.0), 2, N=10000, integration_domain=d64, seed=0)
+    assert abs(float(val) - 90.0) < 1e-13
+
+
+class ModifiedRNG(tq.RNG):
+    """Replaces some uniforms by 0.0 and 1.0 (vegas_test.py:143-156)."""
+
+    def __init__(self, *args, **kargs):
+        super().__init__(*args, **kargs)
+        rng_uniform = self.uniform
+        self.uniform = lambda *a, **k: self.modify_numbers(rng_uniform(*a, **k))
+
+    def modify_numbers(self, numbers):
+        zeros = torch.zeros_like(numbers)
+        ones = torch.ones_like(numbers)
+        return torch.where(numbers < 0.5, numbers * 2.0, torch.where(numbers < 0.75, zeros, ones))
+
+
+def test_edge_random_numbers(dev):
+    with patch("torchquad_amd.integration.vegas.RNG", ModifiedRNG):
+        v = tq.VEGAS()
+        val = v.integrate(lambda x: torch.sum(x, dim=1), 2, N=10000, integration_domain=[[0.0, 1.0]] * 2, seed=0)
+    assert isinstance(v.rng, ModifiedRNG)
+    assert abs(float(val) - 1.0) < 0.1
+
+
+def test_reference_accuracy_checks(dev):
+    # vegas_test.py:71-140
+    for dtype in (torch.float32, torch.float64):
+        dom = torch.tensor([[1.0, 5.0], [-4.0, 4.0], [2.0, 6.0]], dtype=dtype, device=dev)
+
+        def cube_peak(x):
+            return torch.prod((x >= 3.0) * (x < 4.0), dim=1).to(dtype) + 0.001
+
+        ref = float(torch.prod(dom[:, 1] - dom[:, 0])) * 0.001 + 1.0
+        for seed in [0, 1, 2, 3, 41317]:
+            val = tq.VEGAS().integrate(cube_peak, 3, N=30000, integration_domain=dom, seed=seed)
+            assert abs(float(val) - ref) < 0.03
+        pd = 100.0
+        dom2 = torch.tensor([[1.0, 1.0 + pd], [-4.0, -4.0 + pd]], dtype=dtype, device=dev)
+
+        def diag(x):
+            return torch.exp(torch.sum(dom2[:, 0] - x, dim=1)) + torch.exp(torch.sum(x - dom2[:, 1], dim=1))
+
+        ref2 = 2.0 - 4.0 * math.exp(-pd) + 2.0 * math.exp(-2 * pd)
+        for seed in [0, 1, 2, 3, 41317]:
+            val = tq.VEGAS().integrate(diag, 2, N=30000, integration_domain=dom2, seed=seed)
+            assert abs(float(val) - ref2) < 0.03
+
+
+def test_vegas_gradients(ref_vectors, dev):
+    G = ref_vectors["gradients"]
+    v = tq.VEGAS()
+    dom = torch.tensor(G["v_function_domain"]["domain"], dtype=torch.float64, device=dev, requires_grad=True)
+    val = v.integrate(lambda x: 2 * torch.abs(x), 1, N=99997, integration_domain=dom, seed=0)
+    val.backward()
+    assert abs(float(val) - 2.0) < 1e-2
+    np.testing.assert_array_less(np.abs(dom.grad.cpu().numpy() - np.array([[-2.0, 2.0]])), 2e-2)
+    dom = torch.tensor(G["poly_domain"]["domain"], dtype=torch.float64, device=dev, requires_grad=True)
+    val = v.integrate(lambda x: 1.0 + 2.0 * x[:, 0] + 3.0 * x[:, 1] ** 2, 2, N=99997, integration_domain=dom, seed=0)
+    val.backward()
+    assert abs(float(val) - 12.0) < 8e-2
+    np.testing.assert_array_less(np.abs(dom.grad.cpu().numpy() - np.array(G["poly_domain"]["grad"])), 0.1)
+    param = torch.tensor([1.0, 2.0, 3.0], dtype=torch.float64, device=dev, requires_grad=True)
+    val = v.integrate(lambda x: param[0] + param[1] * x[:, 0] + param[2] * x[:, 1] ** 2, 2, N=99997,
+                      integration_domain=[[0.0, 1.0], [0.0, 2.0]], seed=0)
+    val.backward()
+    np.testing.assert_array_less(np.abs(param.grad.cpu().numpy() - np.array([2.0, 1.0, 8.0 / 3.0])), 5e-2)

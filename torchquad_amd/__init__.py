@@ -17,6 +17,9 @@ from .integration.trapezoid import Trapezoid
 from .integration.simpson import Simpson
 from .integration.boole import Boole
 from .integration.gaussian import GaussLegendre, Gaussian
+from .integration.vegas import VEGAS
+from .integration.vegas_map import VEGASMap
+from .integration.vegas_stratification import VEGASStratification
 from .integration.grid_integrator import GridIntegrator
 from .integration.base_integrator import BaseIntegrator
 from .integration.rng import RNG
@@ -44,6 +47,9 @@ __all__ = [
     "Trapezoid",
     "Simpson",
     "Boole",
+    "VEGAS",
+    "VEGASMap",
+    "VEGASStratification",
     "GaussLegendre",
     "Gaussian",
     "RNG",

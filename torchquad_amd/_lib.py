@@ -113,6 +113,64 @@ SIGNATURES = {
     "tq_uniform": (ctypes.c_int, [_U64, _U64, _I32, _I64, _I32, _P, _P]),
 }
 
+class VegasState(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32),
+        ("n_intervals", ctypes.c_int32),
+        ("n_strat", ctypes.c_int32),
+        ("sample_dtype", ctypes.c_int32),
+        ("n_cubes", ctypes.c_int64),
+        ("max_samples", ctypes.c_int64),
+        ("max_blocks", ctypes.c_int64),
+        ("v_cubes", ctypes.c_double),
+        ("alpha", ctypes.c_double),
+        ("beta", ctypes.c_double),
+        ("x_edges", ctypes.c_void_p),
+        ("dx_edges", ctypes.c_void_p),
+        ("weights", ctypes.c_void_p),
+        ("counts", ctypes.c_void_p),
+        ("dh", ctypes.c_void_p),
+        ("jf_cube", ctypes.c_void_p),
+        ("jf2_cube", ctypes.c_void_p),
+        ("nh", ctypes.c_void_p),
+        ("offsets", ctypes.c_void_p),
+        ("chunk_cube", ctypes.c_void_p),
+        ("jf", ctypes.c_void_p),
+        ("hist_w", ctypes.c_void_p),
+        ("hist_c", ctypes.c_void_p),
+        ("block_sums", ctypes.c_void_p),
+        ("scan_sums", ctypes.c_void_p),
+        ("iter_stats", ctypes.c_void_p),
+        ("flags", ctypes.c_void_p),
+    ]
+
+
+_P = ctypes.c_void_p
+_I32, _I64, _U64, _D = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double
+_S = ctypes.POINTER(VegasState)
+VEGAS_SIGNATURES = {
+    "tq_vegas_init": (ctypes.c_int, [_S, _P]),
+    "tq_vegas_nh": (ctypes.c_int, [_S, _I64, _I32, _I32, _P, _P]),
+    "tq_vegas_iterate": (
+        ctypes.c_int,
+        [_S, ctypes.POINTER(Integrand), _P, _U64, _U64, _I32, _I32, _I32, _I64, _P],
+    ),
+    "tq_vegas_points": (ctypes.c_int, [_S, _P, _U64, _U64, _I32, _I64, _P, _P, _P, _P, _P, _P]),
+    "tq_vegas_consume": (ctypes.c_int, [_S, _P, _P, _P, _P, _I32, _I32, _I32, _I64, _P]),
+    "tq_vegas_update_map": (ctypes.c_int, [_S, _I64, _I32, _P]),
+    "tq_vegas_map_eval": (ctypes.c_int, [_S, _P, _I64, _P, _P, _P, _P, _P]),
+    "tq_vegas_map_accumulate": (ctypes.c_int, [_S, _P, _P, _I64, _P]),
+    "tq_vegas_map_update_accumulated": (ctypes.c_int, [_S, _P]),
+    "tq_vegas_smooth": (ctypes.c_int, [_I32, _I32, _P, _P, _D, _P, _P, _P, _P]),
+    "tq_vegas_strat_y": (ctypes.c_int, [_S, _U64, _U64, _P, _P, _P]),
+    "tq_vegas_strat_accumulate": (ctypes.c_int, [_S, _P, _I32, _P]),
+    "tq_vegas_update_dh": (ctypes.c_int, [_S, _I32, _P]),
+    "tq_vegas_normalize_dh": (ctypes.c_int, [_S, _I32, _P]),
+}
+
+
+SIGNATURES.update(VEGAS_SIGNATURES)
+
 _lib = None
 _lib_lock = threading.Lock()
 

@@ -1,0 +1,1159 @@
+// VEGAS+ on the device: adaptive map, stratification and the fused
+// iteration body, with all state resident in caller-owned buffers.
+//
+// Iteration pipeline (no host synchronisation inside):
+//   k_vegas_nh      nh = max(2, floor(dh * n_exp)) per cube + block scan
+//   k_vegas_scan    scan of the block totals
+//   k_vegas_offsets cube offsets, total, cube of the first sample of every
+//                   1024-sample block (so blocks find their cubes in LDS)
+//   k_vegas_sample  y, map, Jacobian, integrand, jf, map histograms
+//   k_vegas_strat   per-cube JF/JF2 (segmented, fixed order), ih, sigma^2, dh
+//   k_vegas_strat_finalize   iteration result and dh normaliser
+//   k_vegas_map_reduce + k_vegas_map_update   update_map
+// Map histograms are accumulated per wave in LDS by looping over the distinct
+// bins present in the wave (ballot + fixed-order wave reduction), combined in
+// wave order per block and reduced over blocks in block order: the whole
+// iteration is deterministic.  Integer-valued and rounding-sensitive steps
+// (bin ids, y, nh) are written operation by operation as the reference does
+// them, with floating-point contraction disabled in this file.
+#pragma clang fp contract(off)
+#include "tq_host.hpp"
+
+namespace tq {
+
+constexpr int kSB = TQ_VEGAS_BLOCK_SAMPLES;
+constexpr int kSPT = kSB / kBlock;
+constexpr int kScanItems = 8;
+constexpr int kScanBlockCubes = kBlock * kScanItems;
+constexpr int kHistLds = 768;         // dim*n_intervals handled with LDS histograms
+constexpr int kMapLds = 1024;         // doubles of map tables staged in LDS
+constexpr int kOffLds = kSB / 2 + 8;  // cube offsets of one sample block
+
+static int check_state(const tq_vegas_state* st) {
+    if (!st) return fail(TQ_E_NULL, "vegas state is NULL");
+    if (st->dim < 1 || st->dim > TQ_MAX_DIM) return fail(TQ_E_ARG, "vegas dim out of range (1..16)");
+    if (st->n_intervals < 2) return fail(TQ_E_ARG, "n_intervals must be >= 2");
+    if (st->n_strat < 1 || st->n_cubes < 1) return fail(TQ_E_ARG, "n_strat/n_cubes must be >= 1");
+    if (st->sample_dtype != TQ_F32 && st->sample_dtype != TQ_F64) return fail(TQ_E_ARG, "bad sample dtype");
+    if (st->max_blocks < 4) return fail(TQ_E_ARG, "max_blocks must be >= 4");
+    if (!st->x_edges || !st->dx_edges || !st->weights || !st->counts || !st->dh || !st->jf_cube ||
+        !st->jf2_cube || !st->nh || !st->offsets || !st->chunk_cube || !st->jf || !st->hist_w || !st->hist_c ||
+        !st->block_sums || !st->scan_sums || !st->iter_stats || !st->flags)
+        return fail(TQ_E_NULL, "vegas state has a NULL buffer");
+    return TQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// init
+// ---------------------------------------------------------------------------
+__global__ void k_vegas_init(tq_vegas_state st) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ni = st.n_intervals;
+    const double step = 1.0 / (double)ni;
+    for (long long i = i0; i < (long long)st.dim * (ni + 1); i += stride) {
+        const int j = (int)(i % (ni + 1));
+        st.x_edges[i] = (j == ni) ? 1.0 : (double)j * step;
+    }
+    for (long long i = i0; i < (long long)st.dim * ni; i += stride) {
+        st.dx_edges[i] = 1.0 / (double)ni;
+        st.weights[i] = 0.0;
+        st.counts[i] = 0.0;
+    }
+    const double dh0 = 1.0 / (double)st.n_cubes;
+    for (long long h = i0; h < st.n_cubes; h += stride) st.dh[h] = dh0;
+    if (i0 < 4) st.flags[i0] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// get_NH + scan
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_vegas_nh(tq_vegas_state st, double n_exp, int normalize_it,
+                                                     const int64_t* nh_in) {
+    __shared__ long long s_scan[kBlock];
+    const int tid = threadIdx.x;
+    double dsum = 0.0;
+    bool norm = false;
+    if (normalize_it >= 0) {
+        dsum = st.iter_stats[4 * normalize_it + 3];
+        norm = (dsum != 0.0);
+    }
+    const long long base = (long long)blockIdx.x * kScanBlockCubes + (long long)tid * kScanItems;
+    long long local[kScanItems];
+    long long tsum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const long long h = base + i;
+        long long v = 0;
+        if (h < st.n_cubes) {
+            if (nh_in) {
+                v = nh_in[h];
+                if (v < 0) v = 0;
+            } else {
+                double dh = st.dh[h];
+                if (norm) {
+                    dh = dh / dsum;
+                    st.dh[h] = dh;
+                }
+                double x = floor(dh * n_exp);
+                if (!(x >= 2.0)) x = 2.0;  // clip(., 2) ; NaN -> 2 keeps indices in range
+                if (x > 4.0e18) x = 4.0e18;
+                v = (long long)x;
+            }
+            st.nh[h] = v;
+        }
+        local[i] = tsum;
+        tsum += v;
+    }
+    s_scan[tid] = tsum;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        const long long t = (tid >= off) ? s_scan[tid - off] : 0;
+        __syncthreads();
+        s_scan[tid] += t;
+        __syncthreads();
+    }
+    const long long excl = s_scan[tid] - tsum;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const long long h = base + i;
+        if (h < st.n_cubes) st.offsets[h] = excl + local[i];
+    }
+    if (tid == kBlock - 1) st.scan_sums[blockIdx.x] = s_scan[tid];
+}
+
+__global__ __launch_bounds__(1024) void k_vegas_scan(int64_t* sums, int nb) {
+    __shared__ long long s[1024];
+    const int tid = threadIdx.x;
+    const int per = (nb + 1023) / 1024;
+    const int b0 = tid * per;
+    long long loc = 0;
+    for (int i = 0; i < per; ++i)
+        if (b0 + i < nb) loc += sums[b0 + i];
+    s[tid] = loc;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const long long t = (tid >= off) ? s[tid - off] : 0;
+        __syncthreads();
+        s[tid] += t;
+        __syncthreads();
+    }
+    long long run = s[tid] - loc;
+    for (int i = 0; i < per; ++i) {
+        if (b0 + i < nb) {
+            const long long v = sums[b0 + i];
+            sums[b0 + i] = run;
+            run += v;
+        }
+    }
+    if (tid == 1023) sums[nb] = s[1023];
+}
+
+__global__ __launch_bounds__(kBlock) void k_vegas_offsets(tq_vegas_state st, int it, int nb_scan) {
+    const long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < st.n_cubes) {
+        const long long off = st.offsets[h] + st.scan_sums[h / kScanBlockCubes];
+        st.offsets[h] = off;
+        const long long n = st.nh[h];
+        if (n > 0) {
+            const long long c0 = (off + kSB - 1) / kSB;
+            const long long c1 = (off + n - 1) / kSB;
+            for (long long c = c0; c <= c1 && c <= st.max_blocks; ++c) st.chunk_cube[c] = h;
+        }
+    }
+    if (h == 0) {
+        const long long total = st.scan_sums[nb_scan];
+        st.offsets[st.n_cubes] = total;
+        st.iter_stats[4 * it + 2] = (double)total;
+        if (total > st.max_samples || (total + kSB - 1) / kSB > st.max_blocks) st.flags[3] = 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// histograms
+// ---------------------------------------------------------------------------
+// Adds v into bin `id` of axis k for every lane with id >= 0, one distinct
+// bin per loop trip; the leader lane writes the wave-private slot.
+__device__ inline void wave_hist_add(double* hw, uint32_t* hc, int k, int ni, int id, double v) {
+    const int lane = threadIdx.x & (kWave - 1);
+    unsigned long long pending = __ballot(id >= 0);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int b = __shfl(id, leader, kWave);
+        const bool mine = (id == b);
+        const unsigned long long mm = __ballot(mine);
+        const double s = wave_sum(mine ? v : 0.0);
+        if (lane == leader) {
+            hw[k * ni + b] += s;
+            hc[k * ni + b] += (uint32_t)__popcll(mm);
+        }
+        pending &= ~mm;
+    }
+}
+
+// Same aggregation, written with FP64 atomics to the global accumulators
+// (maps too large for LDS; the summation order is then not fixed).
+__device__ inline void wave_hist_add_global(double* w, double* c, int k, int ni, int id, double v) {
+    const int lane = threadIdx.x & (kWave - 1);
+    unsigned long long pending = __ballot(id >= 0);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int b = __shfl(id, leader, kWave);
+        const bool mine = (id == b);
+        const unsigned long long mm = __ballot(mine);
+        const double s = wave_sum(mine ? v : 0.0);
+        if (lane == leader) {
+            atomicAdd(&w[k * ni + b], s);
+            atomicAdd(&c[k * ni + b], (double)__popcll(mm));
+        }
+        pending &= ~mm;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// sample kernel
+// ---------------------------------------------------------------------------
+enum { MODE_FUSED = 0, MODE_POINTS = 1, MODE_Y = 2 };
+
+template <typename T>
+struct SampleArgs {
+    tq_vegas_state st;
+    const T* domain;
+    uint64_t seed, offset;
+    int warmup;
+    long long n_warm;
+    int hist;  // accumulate map histograms
+    int it;
+    T* x_out;
+    double* jac_out;
+    int32_t* ids_out;
+    T* y_out;
+    Params<T> p;
+    const T* u_in;  // optional caller uniforms [samples][dim] (MODE_POINTS)
+};
+
+template <typename T>
+__device__ inline T uniform_coord(uint64_t seed, uint64_t s, int k);
+template <>
+__device__ inline float uniform_coord<float>(uint64_t seed, uint64_t s, int k) {
+    const u32x4 r = philox_block(seed, s, (uint32_t)(k >> 2));
+    return u01f(r.v[k & 3]);
+}
+template <>
+__device__ inline double uniform_coord<double>(uint64_t seed, uint64_t s, int k) {
+    const u32x4 r = philox_block(seed, s, (uint32_t)(k >> 1));
+    return (k & 1) ? u01d(r.v[2], r.v[3]) : u01d(r.v[0], r.v[1]);
+}
+
+template <typename T, int FAM, int MODE, bool LDS_HIST>
+__global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
+    using F = Family<FAM, T>;
+    __shared__ long long s_off[kOffLds];
+    __shared__ double s_map[kMapLds];
+    __shared__ double s_hw[LDS_HIST ? kWavesPerBlock * kHistLds : 1];
+    __shared__ uint32_t s_hc[LDS_HIST ? kWavesPerBlock * kHistLds : 1];
+    __shared__ long long s_hrange[2];
+    const tq_vegas_state& st = A.st;
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const int d = st.dim, ni = st.n_intervals;
+    const int W = d * ni;
+
+    long long total;
+    if (A.warmup) {
+        total = A.n_warm;
+    } else {
+        total = st.offsets[st.n_cubes];
+        if (total > st.max_samples) total = st.max_samples;
+    }
+    const long long s0 = (long long)blockIdx.x * kSB;
+    if (s0 >= total) return;
+    const long long nblocks = (total + kSB - 1) / kSB;
+
+    // map tables
+    const bool map_lds = (d * (2 * ni + 1) <= kMapLds);
+    const double* xe = st.x_edges;
+    const double* dxe = st.dx_edges;
+    if (MODE != MODE_Y && map_lds) {
+        for (int i = tid; i < d * (ni + 1); i += kBlock) s_map[i] = st.x_edges[i];
+        for (int i = tid; i < W; i += kBlock) s_map[d * (ni + 1) + i] = st.dx_edges[i];
+        xe = s_map;
+        dxe = s_map + d * (ni + 1);
+    }
+    if (MODE == MODE_FUSED && LDS_HIST && A.hist) {
+        for (int i = tid; i < kWavesPerBlock * W; i += kBlock) {
+            s_hw[(i / W) * kHistLds + (i % W)] = 0.0;
+            s_hc[(i / W) * kHistLds + (i % W)] = 0u;
+        }
+    }
+    // cube range of this block
+    long long h_lo = 0, n_off = 0;
+    bool off_lds = false;
+    if (!A.warmup) {
+        if (tid == 0) {
+            s_hrange[0] = st.chunk_cube[blockIdx.x];
+            s_hrange[1] = (blockIdx.x + 1 < nblocks) ? st.chunk_cube[blockIdx.x + 1] : (st.n_cubes - 1);
+        }
+        __syncthreads();
+        h_lo = s_hrange[0];
+        const long long h_hi = s_hrange[1];
+        n_off = h_hi - h_lo + 2;  // offsets h_lo .. h_hi+1
+        off_lds = (n_off <= kOffLds);
+        if (off_lds)
+            for (long long i = tid; i < n_off; i += kBlock) s_off[i] = st.offsets[h_lo + i];
+    }
+    __syncthreads();
+
+    // domain transform (vegas.py:104-112)
+    T start[TQ_MAX_DIM], size[TQ_MAX_DIM];
+    T vol = T(1);
+#pragma unroll
+    for (int k = 0; k < TQ_MAX_DIM; ++k) {
+        if (k < d && MODE != MODE_Y) {
+            start[k] = A.domain[2 * k];
+            size[k] = A.domain[2 * k + 1] - A.domain[2 * k];
+            vol = vol * size[k];
+        } else {
+            start[k] = T(0);
+            size[k] = T(0);
+        }
+    }
+    const T tns = (T)st.n_strat;
+    const T tni = (T)ni;
+    const T ymax = (T)0.999999;
+
+    for (int j = 0; j < kSPT; ++j) {
+        const long long s = s0 + (long long)j * kBlock + tid;
+        const bool active = s < total;
+        long long h = 0;
+        if (active && !A.warmup) {
+            if (off_lds) {
+                long long lo = 0, hi = n_off - 1;  // find last i with s_off[i] <= s
+                while (lo < hi) {
+                    const long long mid = (lo + hi + 1) >> 1;
+                    if (s_off[mid] <= s) lo = mid;
+                    else hi = mid - 1;
+                }
+                h = h_lo + lo;
+            } else {
+                long long lo = h_lo, hi = st.n_cubes - 1;
+                while (lo < hi) {
+                    const long long mid = (lo + hi + 1) >> 1;
+                    if (st.offsets[mid] <= s) lo = mid;
+                    else hi = mid - 1;
+                }
+                h = lo;
+            }
+        }
+        T x[TQ_MAX_DIM];
+        int ids[TQ_MAX_DIM];
+        double jac = 1.0;
+        long long rem = h;
+#pragma unroll
+        for (int k = 0; k < TQ_MAX_DIM; ++k) {
+            ids[k] = -1;
+            x[k] = T(0);
+            if (k < d) {
+                const T u = (MODE != MODE_FUSED && A.u_in)
+                                ? (active ? A.u_in[s * d + k] : T(0))
+                                : uniform_coord<T>(A.seed, A.offset + (uint64_t)(active ? s : 0), k);
+                T y;
+                if (A.warmup) {
+                    y = u * ymax;
+                } else {
+                    const long long dig = rem % st.n_strat;
+                    rem /= st.n_strat;
+                    y = ((T)dig + u) / tns;
+                    if (y >= T(1)) y = ymax;
+                }
+                if (MODE == MODE_Y) {
+                    if (active) A.y_out[s * d + k] = y;
+                    continue;
+                }
+                if (MODE == MODE_POINTS && A.y_out && active) A.y_out[s * d + k] = y;
+                const T yn = y * tni;
+                const T fl = floor(yn);
+                int id = (int)fl;
+                id = id < 0 ? 0 : (id >= ni ? ni - 1 : id);
+                const T off = yn - fl;
+                const double xm = xe[k * (ni + 1) + id] + dxe[k * ni + id] * (double)off;
+                jac *= (double)ni * dxe[k * ni + id];
+                x[k] = (T)xm * size[k] + start[k];
+                ids[k] = id;
+            }
+        }
+        if (MODE == MODE_Y) continue;
+        if (MODE == MODE_POINTS) {
+            if (active) {
+                for (int k = 0; k < d; ++k) {
+                    A.x_out[s * d + k] = x[k];
+                    A.ids_out[s * d + k] = ids[k];
+                }
+                A.jac_out[s] = jac;
+            }
+            continue;
+        }
+        // fused: integrand
+        T fv = T(0);
+        if (active) fv = (T)(A.p.c0 * eval_point<FAM, T, TQ_MAX_DIM>(x, d, A.p)) * vol;
+        const double jf = (double)fv * jac;
+        if (active && !A.warmup) st.jf[s] = jf;
+        if (A.hist) {
+            const double jf2 = jf * jf;
+            for (int k = 0; k < d; ++k) {
+                const int id = active ? ids[k] : -1;
+                if (LDS_HIST)
+                    wave_hist_add(s_hw + wave * kHistLds, s_hc + wave * kHistLds, k, ni, id, jf2);
+                else
+                    wave_hist_add_global(st.weights, st.counts, k, ni, id, jf2);
+            }
+        }
+    }
+    if (MODE == MODE_FUSED && LDS_HIST && A.hist) {
+        __syncthreads();
+        for (int e = tid; e < W; e += kBlock) {
+            double w = 0.0;
+            uint32_t c = 0;
+            for (int wv = 0; wv < kWavesPerBlock; ++wv) {
+                w += s_hw[wv * kHistLds + e];
+                c += s_hc[wv * kHistLds + e];
+            }
+            st.hist_w[(long long)blockIdx.x * W + e] = w;
+            st.hist_c[(long long)blockIdx.x * W + e] = c;
+        }
+    }
+}
+
+// user integrand values -> jf and histograms
+template <typename T, bool LDS_HIST>
+__global__ __launch_bounds__(kBlock) void k_vegas_consume(tq_vegas_state st, const T* fv, const T* domain,
+                                                          const double* jacs, const int32_t* idsg, int warmup,
+                                                          long long n_warm, int hist) {
+    __shared__ double s_hw[LDS_HIST ? kWavesPerBlock * kHistLds : 1];
+    __shared__ uint32_t s_hc[LDS_HIST ? kWavesPerBlock * kHistLds : 1];
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const int d = st.dim, ni = st.n_intervals, W = d * ni;
+    long long total = warmup ? n_warm : st.offsets[st.n_cubes];
+    if (!warmup && total > st.max_samples) total = st.max_samples;
+    const long long s0 = (long long)blockIdx.x * kSB;
+    if (s0 >= total) return;
+    if (LDS_HIST && hist)
+        for (int i = tid; i < kWavesPerBlock * W; i += kBlock) {
+            s_hw[(i / W) * kHistLds + (i % W)] = 0.0;
+            s_hc[(i / W) * kHistLds + (i % W)] = 0u;
+        }
+    T vol = T(1);
+    for (int k = 0; k < d; ++k) vol = vol * (domain[2 * k + 1] - domain[2 * k]);
+    __syncthreads();
+    for (int j = 0; j < kSPT; ++j) {
+        const long long s = s0 + (long long)j * kBlock + tid;
+        const bool active = s < total;
+        double jf = 0.0;
+        if (active) {
+            const T f = fv[s] * vol;
+            jf = (double)f * jacs[s];
+            if (!warmup) st.jf[s] = jf;
+        }
+        if (hist) {
+            const double jf2 = jf * jf;
+            for (int k = 0; k < d; ++k) {
+                int id = active ? idsg[s * d + k] : -1;
+                if (id >= ni) id = ni - 1;
+                if (LDS_HIST)
+                    wave_hist_add(s_hw + wave * kHistLds, s_hc + wave * kHistLds, k, ni, id, jf2);
+                else
+                    wave_hist_add_global(st.weights, st.counts, k, ni, id, jf2);
+            }
+        }
+    }
+    if (LDS_HIST && hist) {
+        __syncthreads();
+        for (int e = tid; e < W; e += kBlock) {
+            double w = 0.0;
+            uint32_t c = 0;
+            for (int wv = 0; wv < kWavesPerBlock; ++wv) {
+                w += s_hw[wv * kHistLds + e];
+                c += s_hc[wv * kHistLds + e];
+            }
+            st.hist_w[(long long)blockIdx.x * W + e] = w;
+            st.hist_c[(long long)blockIdx.x * W + e] = c;
+        }
+    }
+}
+
+// accumulate_weight on given y / jf2 (step API): block partials
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_vegas_accum_given(tq_vegas_state st, const T* y, const T* jf2v,
+                                                              long long n) {
+    __shared__ double s_hw[kWavesPerBlock * kHistLds];
+    __shared__ uint32_t s_hc[kWavesPerBlock * kHistLds];
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const int d = st.dim, ni = st.n_intervals, W = d * ni;
+    const long long s0 = (long long)blockIdx.x * kSB;
+    for (int i = tid; i < kWavesPerBlock * W; i += kBlock) {
+        s_hw[(i / W) * kHistLds + (i % W)] = 0.0;
+        s_hc[(i / W) * kHistLds + (i % W)] = 0u;
+    }
+    __syncthreads();
+    const T tni = (T)ni;
+    for (int j = 0; j < kSPT; ++j) {
+        const long long s = s0 + (long long)j * kBlock + tid;
+        const bool active = s < n;
+        const double v = active ? (double)jf2v[s] : 0.0;
+        for (int k = 0; k < d; ++k) {
+            int id = -1;
+            if (active) {
+                id = (int)floor(y[s * d + k] * tni);
+                id = id < 0 ? 0 : (id >= ni ? ni - 1 : id);
+            }
+            wave_hist_add(s_hw + wave * kHistLds, s_hc + wave * kHistLds, k, ni, id, v);
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < W; e += kBlock) {
+        double w = 0.0;
+        uint32_t c = 0;
+        for (int wv = 0; wv < kWavesPerBlock; ++wv) {
+            w += s_hw[wv * kHistLds + e];
+            c += s_hc[wv * kHistLds + e];
+        }
+        st.hist_w[(long long)blockIdx.x * W + e] = w;
+        st.hist_c[(long long)blockIdx.x * W + e] = c;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// stratification: per-cube sums and update_DH
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_vegas_strat(tq_vegas_state st, const double* w, int update_dh) {
+    __shared__ double s_red[kWavesPerBlock];
+    const long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double ih = 0.0, sc = 0.0, dr = 0.0;
+    if (h < st.n_cubes) {
+        long long a = st.offsets[h], b = st.offsets[h + 1];
+        if (b > st.max_samples) b = st.max_samples;
+        if (a > b) a = b;
+        double JF = 0.0, JF2 = 0.0;
+        for (long long s = a; s < b; ++s) {
+            const double v = w[s];
+            JF += v;
+            JF2 += v * v;
+        }
+        st.jf_cube[h] = JF;
+        st.jf2_cube[h] = JF2;
+        const double cnt = (double)st.nh[h];
+        const double V = st.v_cubes;
+        const double ninv = 1.0 / cnt;
+        ih = JF * (ninv * V);
+        sc = fabs(JF2 * ninv * (V * V) - ih * ih) * ninv;
+        if (update_dh) {
+            const double V2 = V * V;
+            const double t = V * JF / cnt;
+            double dt = V2 * JF2 / cnt - t * t;
+            if (dt < 0.0) dt = 0.0;
+            dr = pow(dt, st.beta);
+            st.dh[h] = dr;
+        }
+    }
+    const double b1 = block_sum(ih, s_red);
+    const double b2 = block_sum(sc, s_red);
+    const double b3 = block_sum(dr, s_red);
+    if (threadIdx.x == 0) {
+        st.block_sums[4 * blockIdx.x] = b1;
+        st.block_sums[4 * blockIdx.x + 1] = b2;
+        st.block_sums[4 * blockIdx.x + 2] = b3;
+    }
+}
+
+// update_DH (vegas_stratification.py:72-90) from stored JF/JF2/nh, without
+// the normalisation; block sums of the damped counts into block_sums[.][2].
+__global__ __launch_bounds__(kBlock) void k_vegas_update_dh(tq_vegas_state st) {
+    __shared__ double s_red[kWavesPerBlock];
+    const long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double dr = 0.0;
+    if (h < st.n_cubes) {
+        const double JF = st.jf_cube[h], JF2 = st.jf2_cube[h];
+        const double cnt = (double)st.nh[h];
+        const double V = st.v_cubes;
+        const double V2 = V * V;
+        const double t = V * JF / cnt;
+        double dt = V2 * JF2 / cnt - t * t;
+        if (dt < 0.0) dt = 0.0;
+        dr = pow(dt, st.beta);
+        st.dh[h] = dr;
+    }
+    const double b3 = block_sum(dr, s_red);
+    if (threadIdx.x == 0) {
+        st.block_sums[4 * blockIdx.x] = 0.0;
+        st.block_sums[4 * blockIdx.x + 1] = 0.0;
+        st.block_sums[4 * blockIdx.x + 2] = b3;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_vegas_strat_finalize(tq_vegas_state st, long long nb, int it) {
+    __shared__ double s_red[1024 / kWave];
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (long long i = threadIdx.x; i < nb; i += blockDim.x) {
+        a += st.block_sums[4 * i];
+        b += st.block_sums[4 * i + 1];
+        c += st.block_sums[4 * i + 2];
+    }
+    a = block_sum(a, s_red);
+    b = block_sum(b, s_red);
+    c = block_sum(c, s_red);
+    if (threadIdx.x == 0) {
+        st.iter_stats[4 * it] = a;
+        st.iter_stats[4 * it + 1] = b;
+        st.iter_stats[4 * it + 3] = c;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_vegas_normalize(tq_vegas_state st, int it) {
+    const double s = st.iter_stats[4 * it + 3];
+    if (s == 0.0) return;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x; h < st.n_cubes; h += stride)
+        st.dh[h] = st.dh[h] / s;
+}
+
+// ---------------------------------------------------------------------------
+// map update
+// ---------------------------------------------------------------------------
+// weights/counts [dim][ni] = sum over sample blocks of the block partials
+// (one wave per bin, lanes over blocks, fixed order); add != 0 accumulates.
+__global__ __launch_bounds__(kBlock) void k_vegas_map_reduce(tq_vegas_state st, long long nsamp, int it, int add) {
+    const int W = st.dim * st.n_intervals;
+    const long long pair = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (kWave - 1);
+    long long total = nsamp >= 0 ? nsamp : (long long)st.iter_stats[4 * it + 2];
+    if (total > st.max_samples) total = st.max_samples;
+    long long nb = (total + kSB - 1) / kSB;
+    if (nb > st.max_blocks) nb = st.max_blocks;
+    if (pair >= W) return;
+    double w = 0.0, c = 0.0;
+    for (long long b = lane; b < nb; b += kWave) {
+        w += st.hist_w[b * W + pair];
+        c += (double)st.hist_c[b * W + pair];
+    }
+    w = wave_sum(w);
+    c = wave_sum(c);
+    if (lane == 0) {
+        if (add) {
+            st.weights[pair] += w;
+            st.counts[pair] += c;
+        } else {
+            st.weights[pair] = w;
+            st.counts[pair] = c;
+        }
+    }
+}
+
+// _smooth_map for one axis (vegas_map.py:113-172) up to the weight sum:
+// zero-count fill and division; returns the row sum.  w, z: scratch rows.
+__device__ inline double smooth_fill_axis(const double* weights, const double* counts, int n, double* w,
+                                          double* z, bool any_zero) {
+    for (int j = 0; j < n; ++j) {
+        z[j] = (counts[j] == 0.0) ? 1.0 : 0.0;
+        w[j] = (counts[j] != 0.0) ? weights[j] / counts[j] : weights[j];
+        if (!any_zero) w[j] = weights[j] / counts[j];
+    }
+    if (any_zero) {
+        for (int pass = 0; pass < 10; ++pass) {
+            bool left = false;
+            for (int j = 0; j < n - 1; ++j)
+                if (z[j] != 0.0) w[j] = w[j + 1];
+            for (int j = 0; j < n - 1; ++j) z[j] = (z[j] != 0.0 && z[j + 1] != 0.0) ? 1.0 : 0.0;
+            for (int j = n - 1; j >= 1; --j)
+                if (z[j] != 0.0) w[j] = w[j - 1];
+            for (int j = n - 1; j >= 1; --j) z[j] = (z[j] != 0.0 && z[j - 1] != 0.0) ? 1.0 : 0.0;
+            for (int j = 0; j < n; ++j) left = left || (z[j] != 0.0);
+            if (!left) break;
+        }
+    }
+    double sum = 0.0;
+    for (int j = 0; j < n; ++j) sum += w[j];
+    return sum;
+}
+
+// [1,6,1]/8 smoothing (edges [7,1]/8) and range compression (EQ 18-19).
+__device__ inline void smooth_finish_axis(const double* w, int n, double sum, double alpha, double* out) {
+    for (int j = 0; j < n; ++j) {
+        double d;
+        if (j == 0) d = 7.0 * w[0] + w[1];
+        else if (j == n - 1) d = w[n - 2] + 7.0 * w[n - 1];
+        else d = w[j - 1] + 6.0 * w[j] + w[j + 1];
+        d = d / (8.0 * sum);
+        if (d != 0.0) {
+            const double r = (d - 1.0) / log(d);
+            d = (alpha == 0.5) ? sqrt(r) : pow(r, alpha);
+        }
+        out[j] = d;
+    }
+}
+
+// update_map for one axis from smoothed weights sw (vegas_map.py:198-259).
+// scratch: num/val [n+1], newx [n+1].  Returns flags bit0 = non-finite edges
+// were found, bit1 = some remained after repair.
+__device__ inline int update_axis(double* x_edges, double* dx_edges, const double* sw, int n, double* num,
+                                  double* val, double* newx) {
+    double total = 0.0;
+    for (int j = 0; j < n; ++j) total += sw[j];
+    const double delta_d = total / (double)n;
+    for (int m = 0; m <= n; ++m) {
+        num[m] = 0.0;
+        val[m] = 0.0;
+    }
+    double cum = 0.0;
+    for (int j = 0; j < n - 1; ++j) {
+        cum += sw[j];
+        const double q = cum / delta_d;
+        long long mlt = (q >= 0.0 && q <= (double)n) ? (long long)q : -1;
+        if (mlt < 0) return 3;  // NaN / out of range: edges cannot be formed
+        num[mlt] += 1.0;
+        val[mlt] += sw[j];
+    }
+    long long idx = 0;
+    double accu = 0.0;
+    for (int i = 0; i < n - 1; ++i) {
+        idx += (long long)num[i];
+        accu += delta_d - val[i];
+        long long ix = idx > n - 1 ? n - 1 : idx;
+        newx[i + 1] = x_edges[ix] + accu / sw[ix] * dx_edges[ix];
+    }
+    newx[0] = x_edges[0];
+    newx[n] = x_edges[n];
+    int flags = 0;
+    bool all_finite = true;
+    for (int j = 0; j <= n; ++j) all_finite = all_finite && isfinite(newx[j]);
+    if (!all_finite) {
+        flags |= 1;
+        // replace by the mean of the neighbours (vegas_map.py:252-255)
+        for (int j = 1; j < n; ++j) val[j] = isfinite(newx[j]) ? 1.0 : 0.0;
+        for (int j = 1; j < n; ++j) num[j] = 0.5 * (newx[j - 1] + newx[j + 1]);
+        for (int j = 1; j < n; ++j)
+            if (val[j] == 0.0) newx[j] = num[j];
+        for (int j = 0; j <= n; ++j)
+            if (!isfinite(newx[j])) flags |= 2;
+        if (flags & 2) return flags;
+    }
+    for (int j = 0; j <= n; ++j) x_edges[j] = newx[j];
+    for (int j = 0; j < n; ++j) dx_edges[j] = x_edges[j + 1] - x_edges[j];
+    return flags;
+}
+
+// Whole update_map in one block, thread k handles axis k sequentially.
+// Scratch (hist_w): per axis 5 rows of n+1 doubles.
+__global__ __launch_bounds__(64) void k_vegas_map_update(tq_vegas_state st, int reset) {
+    __shared__ double s_sum[TQ_MAX_DIM];
+    __shared__ int s_anyzero;
+    const int k = threadIdx.x;
+    const int d = st.dim, n = st.n_intervals;
+    if (k == 0) s_anyzero = 0;
+    __syncthreads();
+    if (k < d)
+        for (int j = 0; j < n; ++j)
+            if (st.counts[k * n + j] == 0.0) atomicOr(&s_anyzero, 1);
+    __syncthreads();
+    const bool any_zero = s_anyzero != 0;
+    double* scr = st.hist_w + (long long)k * 5 * (n + 1);
+    double* w = scr;
+    double* z = scr + (n + 1);
+    if (k < d) s_sum[k] = smooth_fill_axis(st.weights + k * n, st.counts + k * n, n, w, z, any_zero);
+    __syncthreads();
+    bool cannot = false;
+    for (int i = 0; i < d; ++i) cannot = cannot || (s_sum[i] == 0.0);
+    if (cannot) {
+        if (k == 0) st.flags[0] += 1;
+    } else if (k < d) {
+        double* sw = scr + 2 * (n + 1);
+        smooth_finish_axis(w, n, s_sum[k], st.alpha, sw);
+        // num/val/newx reuse rows 0,1 and row 3, 4
+        const int fl = update_axis(st.x_edges + k * (n + 1), st.dx_edges + k * n, sw, n, scr, scr + (n + 1),
+                                   scr + 3 * (n + 1));
+        if (fl & 1) atomicAdd(&st.flags[1], 1);
+        if (fl & 2) atomicAdd(&st.flags[2], 1);
+    }
+    if (reset) {
+        __syncthreads();
+        if (k < d)
+            for (int j = 0; j < n; ++j) {
+                st.weights[k * n + j] = 0.0;
+                st.counts[k * n + j] = 0.0;
+            }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_vegas_smooth(int d, int n, const double* weights, const double* counts,
+                                                     double alpha, double* out, double* scratch, int32_t* flag) {
+    __shared__ double s_sum[TQ_MAX_DIM];
+    __shared__ int s_anyzero;
+    const int k = threadIdx.x;
+    if (k == 0) s_anyzero = 0;
+    __syncthreads();
+    if (k < d)
+        for (int j = 0; j < n; ++j)
+            if (counts[k * n + j] == 0.0) atomicOr(&s_anyzero, 1);
+    __syncthreads();
+    double* w = scratch + (long long)k * 2 * n;
+    double* z = w + n;
+    if (k < d) s_sum[k] = smooth_fill_axis(weights + k * n, counts + k * n, n, w, z, s_anyzero != 0);
+    __syncthreads();
+    bool cannot = false;
+    for (int i = 0; i < d; ++i) cannot = cannot || (s_sum[i] == 0.0);
+    if (k == 0) *flag = cannot ? 1 : 0;
+    if (!cannot && k < d) smooth_finish_axis(w, n, s_sum[k], alpha, out + k * n);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_vegas_map_eval(tq_vegas_state st, const T* y, long long n,
+                                                           int64_t* ids_out, T* off_out, T* x_out, T* jac_out) {
+    const int d = st.dim, ni = st.n_intervals;
+    const T tni = (T)ni;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
+        T jac = T(1);
+        for (int k = 0; k < d; ++k) {
+            const T yn = y[s * d + k] * tni;
+            const T fl = floor(yn);
+            int id = (int)fl;
+            const int idc = id < 0 ? 0 : (id >= ni ? ni - 1 : id);
+            const T off = yn - fl;
+            if (ids_out) ids_out[s * d + k] = (int64_t)id;
+            if (off_out) off_out[s * d + k] = off;
+            if (x_out) x_out[s * d + k] = (T)(st.x_edges[k * (ni + 1) + idc] + st.dx_edges[k * ni + idc] * (double)off);
+            jac = jac * (T)((double)ni * st.dx_edges[k * ni + idc]);
+        }
+        if (jac_out) jac_out[s] = jac;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static int nh_blocks(const tq_vegas_state* st) { return (int)((st->n_cubes + kScanBlockCubes - 1) / kScanBlockCubes); }
+
+static int launch_nh(const tq_vegas_state* st, int64_t n_exp, int normalize_it, int it, const int64_t* nh_in,
+                     hipStream_t s) {
+    const int nb = nh_blocks(st);
+    hipLaunchKernelGGL(k_vegas_nh, dim3(nb), dim3(kBlock), 0, s, *st, (double)n_exp, normalize_it, nh_in);
+    int rc = check_launch("k_vegas_nh");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_vegas_scan, dim3(1), dim3(1024), 0, s, st->scan_sums, nb);
+    rc = check_launch("k_vegas_scan");
+    if (rc) return rc;
+    const int ob = (int)((st->n_cubes + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_vegas_offsets, dim3(ob), dim3(kBlock), 0, s, *st, it, nb);
+    return check_launch("k_vegas_offsets");
+}
+
+static long long sample_blocks(const tq_vegas_state* st, int warmup, long long n_warm) {
+    long long b = warmup ? (n_warm + kSB - 1) / kSB : st->max_blocks;
+    if (b > st->max_blocks) b = st->max_blocks;
+    return b < 1 ? 1 : b;
+}
+
+static int launch_strat(const tq_vegas_state* st, const double* w, int it, int update_dh, hipStream_t s) {
+    const long long nb = (st->n_cubes + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_vegas_strat, dim3((unsigned)nb), dim3(kBlock), 0, s, *st, w, update_dh);
+    int rc = check_launch("k_vegas_strat");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_vegas_strat_finalize, dim3(1), dim3(1024), 0, s, *st, nb, it);
+    return check_launch("k_vegas_strat_finalize");
+}
+
+static int launch_map_update(const tq_vegas_state* st, long long nsamp, int it, hipStream_t s) {
+    const int W = st->dim * st->n_intervals;
+    if (W <= kHistLds) {
+        const int rb = (W + kWavesPerBlock - 1) / kWavesPerBlock;
+        hipLaunchKernelGGL(k_vegas_map_reduce, dim3(rb), dim3(kBlock), 0, s, *st, nsamp, it, 0);
+        int rc = check_launch("k_vegas_map_reduce");
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_vegas_map_update, dim3(1), dim3(64), 0, s, *st, W <= kHistLds ? 0 : 1);
+    return check_launch("k_vegas_map_update");
+}
+
+__global__ void k_zero(double* p, long long n) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0;
+}
+
+template <typename T>
+static int vegas_iterate(const tq_vegas_state* st, const tq_integrand* f, const void* domain, uint64_t seed,
+                         uint64_t offset, int it, int use_grid_improve, int warmup, long long n_warm,
+                         hipStream_t s) {
+    const int W = st->dim * st->n_intervals;
+    const bool lds_hist = W <= kHistLds;
+    const int hist = (use_grid_improve || warmup) ? 1 : 0;
+    if (hist && !lds_hist) {
+        hipLaunchKernelGGL(k_zero, dim3(64), dim3(kBlock), 0, s, st->weights, (long long)W);
+        hipLaunchKernelGGL(k_zero, dim3(64), dim3(kBlock), 0, s, st->counts, (long long)W);
+    }
+    SampleArgs<T> A;
+    A.st = *st;
+    A.domain = (const T*)domain;
+    A.seed = seed;
+    A.offset = offset;
+    A.warmup = warmup;
+    A.n_warm = n_warm;
+    A.hist = hist;
+    A.it = it;
+    A.x_out = nullptr;
+    A.jac_out = nullptr;
+    A.ids_out = nullptr;
+    A.y_out = nullptr;
+    A.p = make_params<T>(*f);
+    A.u_in = nullptr;
+    const long long nb = sample_blocks(st, warmup, n_warm);
+    TQ_DISPATCH_FAMILY(f->family, {
+        if (lds_hist)
+            hipLaunchKernelGGL((k_vegas_sample<T, FAM, MODE_FUSED, true>), dim3((unsigned)nb), dim3(kBlock), 0, s, A);
+        else
+            hipLaunchKernelGGL((k_vegas_sample<T, FAM, MODE_FUSED, false>), dim3((unsigned)nb), dim3(kBlock), 0, s, A);
+    });
+    int rc = check_launch("k_vegas_sample");
+    if (rc) return rc;
+    if (!warmup) {
+        rc = launch_strat(st, st->jf, it, 1, s);
+        if (rc) return rc;
+    }
+    if (hist) return launch_map_update(st, warmup ? n_warm : -1, it, s);
+    return TQ_OK;
+}
+
+}  // namespace tq
+
+using namespace tq;
+
+extern "C" {
+
+int tq_vegas_init(const tq_vegas_state* st, void* stream) {
+    int rc = check_state(st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_vegas_init, dim3(256), dim3(kBlock), 0, as_stream(stream), *st);
+    return check_launch("k_vegas_init");
+}
+
+int tq_vegas_nh(const tq_vegas_state* st, int64_t n_exp, int32_t normalize_it, int32_t it, const int64_t* nh_in,
+                void* stream) {
+    int rc = check_state(st);
+    if (rc) return rc;
+    if (it < 0 || it >= TQ_VEGAS_MAX_ITERS || normalize_it >= TQ_VEGAS_MAX_ITERS)
+        return fail(TQ_E_ARG, "iteration index out of range");
+    if (n_exp < 0) return fail(TQ_E_ARG, "n_exp must be >= 0");
+    return launch_nh(st, n_exp, normalize_it, it, nh_in, as_stream(stream));
+}
+
+int tq_vegas_iterate(const tq_vegas_state* st, const tq_integrand* f, const void* domain, uint64_t seed,
+                     uint64_t offset, int32_t it, int32_t use_grid_improve, int32_t warmup, int64_t n_warm,
+                     void* stream) {
+    int rc = check_state(st);
+    if (rc) return rc;
+    rc = check_integrand(f, st->dim);
+    if (rc) return rc;
+    if (!domain) return fail(TQ_E_NULL, "domain is NULL");
+    if (it < 0 || it >= TQ_VEGAS_MAX_ITERS) return fail(TQ_E_ARG, "iteration index out of range");
+    if (warmup && (n_warm < 0 || (n_warm + kSB - 1) / kSB > st->max_blocks))
+        return fail(TQ_E_ARG, "n_warm exceeds the block capacity");
+    if (st->sample_dtype == TQ_F32)
+        return vegas_iterate<float>(st, f, domain, seed, offset, it, use_grid_improve, warmup, n_warm,
+                                    as_stream(stream));
+    return vegas_iterate<double>(st, f, domain, seed, offset, it, use_grid_improve, warmup, n_warm,
+                                 as_stream(stream));
+}
+
+int tq_vegas_points(const tq_vegas_state* st, const void* domain, uint64_t seed, uint64_t offset, int32_t warmup,
+                    int64_t n_warm, const void* u_in, void* x_out, double* jac_out, int32_t* ids_out, void* y_out,
+                    void* stream) {
+    int rc = check_state(st);
+    if (rc) return rc;
+    if (!domain || !x_out || !jac_out || !ids_out) return fail(TQ_E_NULL, "domain/x/jac/ids is NULL");
+    if (warmup && (n_warm < 0 || (n_warm + kSB - 1) / kSB > st->max_blocks))
+        return fail(TQ_E_ARG, "n_warm exceeds the block capacity");
+    const long long nb = sample_blocks(st, warmup, n_warm);
+    hipStream_t s = as_stream(stream);
+    tq_integrand dummy{};
+    dummy.family = TQ_FAM_CONSTANT;
+    dummy.dim = st->dim;
+    if (st->sample_dtype == TQ_F32) {
+        SampleArgs<float> A{*st, (const float*)domain, seed, offset, warmup, n_warm, 0, 0, (float*)x_out, jac_out,
+                            ids_out, (float*)y_out, make_params<float>(dummy), (const float*)u_in};
+        hipLaunchKernelGGL((k_vegas_sample<float, TQ_FAM_CONSTANT, MODE_POINTS, false>), dim3((unsigned)nb),
+                           dim3(kBlock), 0, s, A);
+    } else {
+        SampleArgs<double> A{*st, (const double*)domain, seed, offset, warmup, n_warm, 0, 0, (double*)x_out,
+                             jac_out, ids_out, (double*)y_out, make_params<double>(dummy), (const double*)u_in};
+        hipLaunchKernelGGL((k_vegas_sample<double, TQ_FAM_CONSTANT, MODE_POINTS, false>), dim3((unsigned)nb),
+                           dim3(kBlock), 0, s, A);
+    }
+    return check_launch("k_vegas_sample(points)");
+}
+
+int tq_vegas_consume(const tq_vegas_state* st, const void* fvals, const void* domain, const double* jac,
+                     const int32_t* ids, int32_t it, int32_t use_grid_improve, int32_t warmup, int64_t n_warm,
+                     void* stream) {
+    int rc = check_state(st);
+    if (rc) return rc;
+    if (!fvals || !domain || !jac || !ids) return fail(TQ_E_NULL, "fvals/domain/jac/ids is NULL");
+    if (it < 0 || it >= TQ_VEGAS_MAX_ITERS) return fail(TQ_E_ARG, "iteration index out of range");
+    hipStream_t s = as_stream(stream);
+    const int W = st->dim * st->n_intervals;
+    const bool lds_hist = W <= kHistLds;
+    const int hist = (use_grid_improve || warmup) ? 1 : 0;
+    if (hist && !lds_hist) {
+        hipLaunchKernelGGL(k_zero, dim3(64), dim3(kBlock), 0, s, st->weights, (long long)W);
+        hipLaunchKernelGGL(k_zero, dim3(64), dim3(kBlock), 0, s, st->counts, (long long)W);
+    }
+    const long long nb = sample_blocks(st, warmup, n_warm);
+    if (st->sample_dtype == TQ_F32) {
+        if (lds_hist)
+            hipLaunchKernelGGL((k_vegas_consume<float, true>), dim3((unsigned)nb), dim3(kBlock), 0, s, *st,
+                               (const float*)fvals, (const float*)domain, jac, ids, warmup, (long long)n_warm, hist);
+        else
+            hipLaunchKernelGGL((k_vegas_consume<float, false>), dim3((unsigned)nb), dim3(kBlock), 0, s, *st,
+                               (const float*)fvals, (const float*)domain, jac, ids, warmup, (long long)n_warm, hist);
+    } else {
+        if (lds_hist)
+            hipLaunchKernelGGL((k_vegas_consume<double, true>), dim3((unsigned)nb), dim3(kBlock), 0, s, *st,
+                               (const double*)fvals, (const double*)domain, jac, ids, warmup, (long long)n_warm,
+                               hist);
+        else
+            hipLaunchKernelGGL((k_vegas_consume<double, false>), dim3((unsigned)nb), dim3(kBlock), 0, s, *st,
+                               (const double*)fvals, (const double*)domain, jac, ids, warmup, (long long)n_warm,
+                               hist);
+    }
+    rc = check_launch("k_vegas_consume");
+    if (rc) return rc;
+    if (!warmup) {
+        rc = launch_strat(st, st->jf, it, 1, s);
+        if (rc) return rc;
+    }
+    if (hist) return launch_map_update(st, warmup ? n_warm : -1, it, s);
+    return TQ_OK;
+}
+
+int tq_vegas_update_map(const tq_vegas_state* st, int64_t n_samples, int32_t it, void* stream) {
+    int rc = check_state(st);
+    if (rc) return rc;
+    return launch_map_update(st, n_samples, it, as_stream(stream));
+}
+
+int tq_vegas_map_eval(const tq_vegas_state* st, const void* y, int64_t n, int64_t* ids_out, void* off_out,
+                      void* x_out, void* jac_out, void* stream) {
+    int rc = check_state(st);
+    if (rc) return rc;
+    if (!y) return fail(TQ_E_NULL, "y is NULL");
+    if (n <= 0) return TQ_OK;
+    long long blocks = (n + kBlock - 1) / kBlock;
+    if (blocks > 65536) blocks = 65536;
+    if (st->sample_dtype == TQ_F32)
+        hipLaunchKernelGGL((k_vegas_map_eval<float>), dim3((unsigned)blocks), dim3(kBlock), 0, as_stream(stream),
+                           *st, (const float*)y, (long long)n, ids_out, (float*)off_out, (float*)x_out,
+                           (float*)jac_out);
+    else
+        hipLaunchKernelGGL((k_vegas_map_eval<double>), dim3((unsigned)blocks), dim3(kBlock), 0, as_stream(stream),
+                           *st, (const double*)y, (long long)n, ids_out, (double*)off_out, (double*)x_out,
+                           (double*)jac_out);
+    return check_launch("k_vegas_map_eval");
+}
+
+int tq_vegas_map_accumulate(const tq_vegas_state* st, const void* y, const void* jf2, int64_t n, void* stream) {
+    int rc = check_state(st);
+    if (rc) return rc;
+    if (!y || !jf2) return fail(TQ_E_NULL, "y/jf2 is NULL");
+    if (st->dim * st->n_intervals > kHistLds) return fail(TQ_E_UNSUPPORTED, "dim*n_intervals > 768");
+    if (n <= 0) return TQ_OK;
+    hipStream_t s = as_stream(stream);
+    long long done = 0;
+    while (done < n) {
+        long long chunk = n - done;
+        const long long cap = st->max_blocks * (long long)kSB;
+        if (chunk > cap) chunk = cap;
+        const long long nb = (chunk + kSB - 1) / kSB;
+        if (st->sample_dtype == TQ_F32)
+            hipLaunchKernelGGL((k_vegas_accum_given<float>), dim3((unsigned)nb), dim3(kBlock), 0, s, *st,
+                               (const float*)y + done * st->dim, (const float*)jf2 + done, chunk);
+        else
+            hipLaunchKernelGGL((k_vegas_accum_given<double>), dim3((unsigned)nb), dim3(kBlock), 0, s, *st,
+                               (const double*)y + done * st->dim, (const double*)jf2 + done, chunk);
+        rc = check_launch("k_vegas_accum_given");
+        if (rc) return rc;
+        const int W = st->dim * st->n_intervals;
+        const int rb = (W + kWavesPerBlock - 1) / kWavesPerBlock;
+        hipLaunchKernelGGL(k_vegas_map_reduce, dim3(rb), dim3(kBlock), 0, s, *st, chunk, 0, 1);
+        rc = check_launch("k_vegas_map_reduce");
+        if (rc) return rc;
+        done += chunk;
+    }
+    return TQ_OK;
+}
+
+int tq_vegas_map_update_accumulated(const tq_vegas_state* st, void* stream) {
+    int rc = check_state(st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_vegas_map_update, dim3(1), dim3(64), 0, as_stream(stream), *st, 1);
+    return check_launch("k_vegas_map_update");
+}
+
+int tq_vegas_smooth(int32_t dim, int32_t n, const double* weights, const double* counts, double alpha, double* out,
+                    double* scratch, int32_t* not_updatable, void* stream) {
+    if (dim < 1 || dim > TQ_MAX_DIM || n < 2) return fail(TQ_E_ARG, "dim/n out of range");
+    if (!weights || !counts || !out || !scratch || !not_updatable) return fail(TQ_E_NULL, "NULL argument");
+    hipLaunchKernelGGL(k_vegas_smooth, dim3(1), dim3(64), 0, as_stream(stream), (int)dim, (int)n, weights, counts,
+                       alpha, out, scratch, not_updatable);
+    return check_launch("k_vegas_smooth");
+}
+
+int tq_vegas_strat_y(const tq_vegas_state* st, uint64_t seed, uint64_t offset, const void* u_in, void* y_out,
+                     void* stream) {
+    int rc = check_state(st);
+    if (rc) return rc;
+    if (!y_out) return fail(TQ_E_NULL, "y is NULL");
+    const long long nb = sample_blocks(st, 0, 0);
+    hipStream_t s = as_stream(stream);
+    tq_integrand dummy{};
+    dummy.family = TQ_FAM_CONSTANT;
+    dummy.dim = st->dim;
+    if (st->sample_dtype == TQ_F32) {
+        SampleArgs<float> A{*st, nullptr, seed, offset, 0, 0, 0, 0, nullptr, nullptr, nullptr, (float*)y_out,
+                            make_params<float>(dummy), (const float*)u_in};
+        hipLaunchKernelGGL((k_vegas_sample<float, TQ_FAM_CONSTANT, MODE_Y, false>), dim3((unsigned)nb),
+                           dim3(kBlock), 0, s, A);
+    } else {
+        SampleArgs<double> A{*st, nullptr, seed, offset, 0, 0, 0, 0, nullptr, nullptr, nullptr, (double*)y_out,
+                             make_params<double>(dummy), (const double*)u_in};
+        hipLaunchKernelGGL((k_vegas_sample<double, TQ_FAM_CONSTANT, MODE_Y, false>), dim3((unsigned)nb),
+                           dim3(kBlock), 0, s, A);
+    }
+    return check_launch("k_vegas_sample(y)");
+}
+
+int tq_vegas_strat_accumulate(const tq_vegas_state* st, const double* w, int32_t it, void* stream) {
+    int rc = check_state(st);
+    if (rc) return rc;
+    if (!w) return fail(TQ_E_NULL, "w is NULL");
+    if (it < 0 || it >= TQ_VEGAS_MAX_ITERS) return fail(TQ_E_ARG, "iteration index out of range");
+    return launch_strat(st, w, it, 0, as_stream(stream));
+}
+
+int tq_vegas_update_dh(const tq_vegas_state* st, int32_t it, void* stream) {
+    int rc = check_state(st);
+    if (rc) return rc;
+    if (it < 0 || it >= TQ_VEGAS_MAX_ITERS) return fail(TQ_E_ARG, "iteration index out of range");
+    hipStream_t s = as_stream(stream);
+    const long long nb = (st->n_cubes + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_vegas_update_dh, dim3((unsigned)nb), dim3(kBlock), 0, s, *st);
+    rc = check_launch("k_vegas_update_dh");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_vegas_strat_finalize, dim3(1), dim3(1024), 0, s, *st, nb, it);
+    return check_launch("k_vegas_strat_finalize");
+}
+
+int tq_vegas_normalize_dh(const tq_vegas_state* st, int32_t it, void* stream) {
+    int rc = check_state(st);
+    if (rc) return rc;
+    if (it < 0 || it >= TQ_VEGAS_MAX_ITERS) return fail(TQ_E_ARG, "iteration index out of range");
+    hipLaunchKernelGGL(k_vegas_normalize, dim3(256), dim3(kBlock), 0, as_stream(stream), *st, it);
+    return check_launch("k_vegas_normalize");
+}
+
+}  // extern "C"
