@@ -254,7 +254,7 @@ def test_gl_generator_matches_leggauss(dev):
         # leggauss itself carries ~1e-10 relative error in the endpoint weights
         np.testing.assert_allclose(w.cpu().numpy(), rw, rtol=2e-10)
         xs, ws = x.cpu().numpy(), w.cpu().numpy()
-        for k in sorted({0, 1, n // 3, n - 1}):
+        for k in sorted({0, min(1, n - 1), n // 3, n - 1}):
             mx, mw = ogl.node_weight(n, k)
             assert abs(xs[k] - mx) <= 1e-15, (n, k)
             assert abs(ws[k] - mw) <= 2e-12 * mw, (n, k)

@@ -14,63 +14,12 @@
 // Block partials are FP64 and the final sum runs in a fixed order, so the
 // result is deterministic.
 #include "tq_host.hpp"
+#include "tq_grid_tile.hpp"
 
 namespace tq {
 
 constexpr int kMaxDimPoints = 64;  // point writer / weighted reduce
 constexpr int kLoopTable = 2048;   // largest n kept in the LDS loop table
-constexpr int kFlush = 32;         // FP32 partial length before FP64 flush
-
-// Per-axis affine map.  Newton-Cotes: x_j = a + j*h (x_{n-1} = b exactly, as
-// linspace).  Gauss-Legendre: x_j = alpha*r_j + beta (gaussian.py:159-162).
-struct Axis {
-    double a, h, b;
-};
-
-template <typename T>
-__device__ inline Axis make_axis(const T* domain, int k, int n, int rule) {
-    const double a = (double)domain[2 * k], b = (double)domain[2 * k + 1];
-    Axis ax;
-    if (rule == TQ_RULE_GAUSS_LEGENDRE) {
-        ax.a = 0.5 * (b - a);
-        ax.h = 0.5 * (a + b);
-        ax.b = b;
-    } else {
-        ax.a = a;
-        ax.h = (b - a) / (double)(n - 1);
-        ax.b = b;
-    }
-    return ax;
-}
-
-__device__ inline double node_of(const Axis& ax, int j, int n, int rule, const double* glx) {
-    if (rule == TQ_RULE_GAUSS_LEGENDRE) return fma(ax.a, glx[j], ax.h);
-    return (j == n - 1) ? ax.b : fma((double)j, ax.h, ax.a);
-}
-
-// Composite Newton-Cotes weight pattern without the per-axis scale factor
-// (trapezoid.py:34-36 -> h/2*[1,2,..,2,1]; simpson.py:35-45 -> h/3*[1,4,2,..,4,1];
-// boole.py:35-47 -> h/22.5*[7,32,12,32,14,..,32,7]); Gauss-Legendre weight.
-__device__ __host__ inline double weight_of(int rule, int j, int n, const double* glw) {
-    const bool end = (j == 0) || (j == n - 1);
-    switch (rule) {
-        case -1: return 1.0;
-        case TQ_RULE_TRAPEZOID: return end ? 1.0 : 2.0;
-        case TQ_RULE_SIMPSON: return end ? 1.0 : ((j & 1) ? 4.0 : 2.0);
-        case TQ_RULE_BOOLE: return end ? 7.0 : ((j & 1) ? 32.0 : (((j & 3) == 2) ? 12.0 : 14.0));
-        default: return glw[j];
-    }
-}
-
-// Per-axis scale so that integral = prod_k scale_k * sum W f.
-__device__ inline double axis_scale(double a, double b, int n, int rule) {
-    switch (rule) {
-        case TQ_RULE_TRAPEZOID: return (b - a) / (double)(n - 1) / 2.0;
-        case TQ_RULE_SIMPSON: return (b - a) / (double)(n - 1) / 3.0;
-        case TQ_RULE_BOOLE: return (b - a) / (double)(n - 1) / 22.5;
-        default: return 0.5 * (b - a);
-    }
-}
 
 struct GridGeom {
     int d, n, m, L;
@@ -137,51 +86,67 @@ static int check_gl_tables(const tq_grid_desc* g) {
     return TQ_OK;
 }
 
-static int make_geom(const tq_grid_desc* gd, GridGeom& G) {
-    const int d = gd->dim, n = gd->n;
+// Splits the n^d index space of one integral (see grid_tile).  lane_axes /
+// items_per_wave override the automatic choice (0 = automatic).
+static int plan_geom(int d, int n, int rule, int lane_axes, long long items_per_wave, long long target_waves,
+                     GridGeom& G) {
     G.d = d;
     G.n = n;
-    G.rule = gd->rule;
+    G.rule = rule;
     int m = 0;
-    if (gd->lane_axes > 0) {
-        m = gd->lane_axes < d ? gd->lane_axes : d;
+    if (lane_axes > 0) {
+        m = lane_axes < d ? lane_axes : d;
+    } else if (d == 1 && n >= 128) {
+        m = 0;  // rows of 64 consecutive nodes
     } else {
+        // lane axes: the smallest m whose n^m lane combinations fill >= 90% of
+        // the 64-lane waves (else the best filling m); a loop axis is kept
+        // whenever d > 1
+        const int mmax = d > 1 ? d - 1 : 1;
+        m = mmax;
+        double best = -1.0;
         long long c = 1;
-        m = d;
-        for (int k = 1; k <= d; ++k) {
+        for (int k = 1; k <= mmax; ++k) {
+            if (c > (1LL << 31) / n) break;
             c *= n;
-            if (c >= 64) {
+            const double util = (double)c / (double)(((c + kWave - 1) / kWave) * kWave);
+            if (c >= 32 && util >= 0.9) {
                 m = k;
                 break;
             }
+            if (util > best) {
+                best = util;
+                m = k;
+            }
         }
-        // keep a loop axis when the lanes are still reasonably filled without it
-        if (m == d && d > 1) {
-            long long c2;
-            ipow_checked(n, d - 1, 1LL << 62, &c2);
-            if (c2 >= 32) m = d - 1;
-        }
+        if (d == 1) m = 1;
     }
-    long long lc;
-    if (ipow_checked(n, m, (1LL << 31) - 64, &lc) != 0) return fail(TQ_E_ARG, "lane combinations exceed 2^31");
-    G.m = m;
-    G.lane_count = (int)lc;
-    G.lane_waves = (int)((lc + kWave - 1) / kWave);
-    G.L = d - m - 1;
     long long items = 1;
-    if (G.L >= 0) ipow_checked(n, d - m, 1LL << 62, &items);
+    if (m == 0) {
+        G.m = 0;
+        G.lane_count = kWave;
+        G.lane_waves = 1;
+        G.L = 0;
+        items = ((long long)n + kWave - 1) / kWave;
+    } else {
+        long long lc;
+        if (ipow_checked(n, m, (1LL << 31) - 64, &lc) != 0) return fail(TQ_E_ARG, "lane combinations exceed 2^31");
+        G.m = m;
+        G.lane_count = (int)lc;
+        G.lane_waves = (int)((lc + kWave - 1) / kWave);
+        G.L = d - m - 1;
+        if (G.L >= 0) ipow_checked(n, d - m, 1LL << 62, &items);
+    }
     G.items = items;
-    G.use_table = (G.L >= 0 && n <= kLoopTable) ? 1 : 0;
-    long long ipw = gd->items_per_wave;
+    G.use_table = (G.m > 0 && G.L >= 0 && n <= kLoopTable) ? 1 : 0;
+    long long ipw = items_per_wave;
     if (ipw <= 0) {
-        const long long target_waves = 16384;
         const long long work = items * (long long)G.lane_waves;
         ipw = (work + target_waves - 1) / target_waves;
         const long long min_ipw = items < 64 ? items : 64;
         if (ipw < min_ipw) ipw = min_ipw;
     }
     if (ipw < 1) ipw = 1;
-    // bound the launch size
     while (((items + ipw - 1) / ipw) * (long long)G.lane_waves > (1LL << 30)) ipw *= 2;
     G.ipw = ipw;
     G.chunks = (items + ipw - 1) / ipw;
@@ -190,12 +155,15 @@ static int make_geom(const tq_grid_desc* gd, GridGeom& G) {
     return TQ_OK;
 }
 
+static int make_geom(const tq_grid_desc* gd, GridGeom& G) {
+    return plan_geom(gd->dim, gd->n, gd->rule, gd->lane_axes, gd->items_per_wave, 16384, G);
+}
+
 // ---------------------------------------------------------------------------
 // Fused kernel
 // ---------------------------------------------------------------------------
 template <typename T, int FAM, bool TABLE>
 __global__ __launch_bounds__(kBlock) void k_grid_fused(GridArgs<T> A) {
-    using F = Family<FAM, T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ Axis s_axis[TQ_MAX_DIM];
     __shared__ double s_red[kWavesPerBlock];
@@ -212,99 +180,15 @@ __global__ __launch_bounds__(kBlock) void k_grid_fused(GridArgs<T> A) {
         }
         __syncthreads();
     }
-
-    const long long wave = (long long)blockIdx.x * kWavesPerBlock + (tid >> 6);
-    const int lane = tid & (kWave - 1);
+    // readfirstlane: the wave index is uniform, so the tile loop runs on SGPRs
+    const long long wave = (long long)blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(tid >> 6);
     double acc = 0.0;
     if (wave < g.num_waves) {
         const int lw = (int)(wave % g.lane_waves);
         const long long chunk = wave / g.lane_waves;
-        const int li = lw * kWave + lane;
-        const bool active = li < g.lane_count;
-
-        // lane axes: fold and weight once
-        State<T> s_lane = F::zero();
-        double w_lane = active ? 1.0 : 0.0;
-        {
-            int rem = active ? li : 0;
-            for (int k = d - 1; k >= d - g.m; --k) {
-                const int j = rem % n;
-                rem /= n;
-                s_lane = F::fold(s_lane, k, (T)node_of(s_axis[k], j, n, rule, A.glx), A.p);
-                w_lane *= weight_of(rule, j, n, A.glw);
-            }
-        }
-
-        if (L < 0) {
-            if (chunk == 0) {
-                const T f = F::finish(F::merge(F::start(A.p), s_lane), A.p);
-                acc = w_lane * (double)f;
-            }
-        } else {
-            long long t = chunk * g.ipw;
-            const long long t_end = (t + g.ipw < g.items) ? (t + g.ipw) : g.items;
-            long long row = t / n;
-            int j = (int)(t - row * n);
-            // row odometer digits for axes [0, L)
-            int rdig[TQ_MAX_DIM];
-            {
-                long long r = row;
-#pragma unroll
-                for (int k = TQ_MAX_DIM - 1; k >= 0; --k) {
-                    if (k < L) {
-                        rdig[k] = (int)(r % n);
-                        r /= n;
-                    } else {
-                        rdig[k] = 0;
-                    }
-                }
-            }
-            while (t < t_end) {
-                State<T> s_row = F::start(A.p);
-                double w_row = w_lane;
-#pragma unroll
-                for (int k = 0; k < TQ_MAX_DIM; ++k) {
-                    if (k < L) {
-                        s_row = F::fold(s_row, k, (T)node_of(s_axis[k], rdig[k], n, rule, A.glx), A.p);
-                        w_row *= weight_of(rule, rdig[k], n, A.glw);
-                    }
-                }
-                const State<T> s_rl = F::merge(s_row, s_lane);
-                const long long left = t_end - t;
-                const int jend = (left < (long long)(n - j)) ? (j + (int)left) : n;
-                double racc = 0.0;
-                for (int jb = j; jb < jend; jb += kFlush) {
-                    const int je = (jb + kFlush < jend) ? (jb + kFlush) : jend;
-                    T fa = T(0);
-                    for (int jj = jb; jj < je; ++jj) {
-                        T xL, wL;
-                        if (TABLE) {
-                            xL = s_tab[2 * jj];
-                            wL = s_tab[2 * jj + 1];
-                        } else {
-                            xL = (T)node_of(s_axis[L], jj, n, rule, A.glx);
-                            wL = (T)weight_of(rule, jj, n, A.glw);
-                        }
-                        const T f = F::finish(F::fold(s_rl, L, xL, A.p), A.p);
-                        fa = fma(wL, f, fa);
-                    }
-                    racc += (double)fa;
-                }
-                acc = fma(w_row, racc, acc);
-                t += jend - j;
-                j = jend;
-                if (j == n) {
-                    j = 0;
-#pragma unroll
-                    for (int k = TQ_MAX_DIM - 1; k >= 0; --k) {
-                        if (k < L) {
-                            if (++rdig[k] < n) break;
-                            rdig[k] = 0;
-                        }
-                    }
-                }
-            }
-        }
+        const long long t0 = chunk * g.ipw;
+        const long long t1 = (t0 + g.ipw < g.items) ? (t0 + g.ipw) : g.items;
+        acc = grid_tile<T, FAM, TABLE>(d, n, g.m, L, rule, g.lane_count, A.p, s_axis, s_tab, A.glx, A.glw, lw, t0, t1);
     }
     const double bs = block_sum(acc, s_red);
     if (tid == 0) A.partials[blockIdx.x] = bs;

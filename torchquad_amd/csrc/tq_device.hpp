@@ -158,6 +158,17 @@ struct State {
     T s0, s1;
 };
 
+// Per-axis family constants, loaded once per axis outside the hot loops.
+template <typename T>
+struct AxisC {
+    T a, u, b;
+};
+
+template <typename T>
+__device__ inline AxisC<T> axis_c(const Params<T>& p, int k) {
+    return AxisC<T>{p.a[k], p.u[k], p.b[k]};
+}
+
 template <int FAM, typename T>
 struct Family;
 
@@ -166,8 +177,8 @@ template <typename T>
 struct Family<TQ_FAM_GAUSSIAN_PEAK, T> {
     __device__ static inline State<T> start(const Params<T>&) { return {T(0), T(0)}; }
     __device__ static inline State<T> zero() { return {T(0), T(0)}; }
-    __device__ static inline State<T> fold(State<T> s, int k, T x, const Params<T>& p) {
-        const T t = fma(x, p.a[k], p.b[k]);
+    __device__ static inline State<T> fold_c(State<T> s, const AxisC<T>& c, T x, const Params<T>&) {
+        const T t = fma(x, c.a, c.b);
         s.s0 = fma(-t, t, s.s0);
         return s;
     }
@@ -180,8 +191,8 @@ template <typename T>
 struct Family<TQ_FAM_OSCILLATORY, T> {
     __device__ static inline State<T> start(const Params<T>& p) { return {p.c2, T(0)}; }
     __device__ static inline State<T> zero() { return {T(0), T(0)}; }
-    __device__ static inline State<T> fold(State<T> s, int k, T x, const Params<T>& p) {
-        s.s0 = fma(p.a[k], x, s.s0);
+    __device__ static inline State<T> fold_c(State<T> s, const AxisC<T>& c, T x, const Params<T>&) {
+        s.s0 = fma(c.a, x, s.s0);
         return s;
     }
     __device__ static inline State<T> merge(State<T> a, State<T> b) { return {a.s0 + b.s0, T(0)}; }
@@ -194,9 +205,9 @@ template <typename T>
 struct Family<TQ_FAM_PRODUCT_PEAK, T> {
     __device__ static inline State<T> start(const Params<T>&) { return {T(1), T(0)}; }
     __device__ static inline State<T> zero() { return {T(1), T(0)}; }
-    __device__ static inline State<T> fold(State<T> s, int k, T x, const Params<T>& p) {
-        const T t = x - p.u[k];
-        s.s0 *= fma(t, t, p.b[k]);
+    __device__ static inline State<T> fold_c(State<T> s, const AxisC<T>& c, T x, const Params<T>&) {
+        const T t = x - c.u;
+        s.s0 *= fma(t, t, c.b);
         return s;
     }
     __device__ static inline State<T> merge(State<T> a, State<T> b) { return {a.s0 * b.s0, T(0)}; }
@@ -208,8 +219,8 @@ template <typename T>
 struct Family<TQ_FAM_EXP_LINEAR, T> {
     __device__ static inline State<T> start(const Params<T>& p) { return {p.c1, T(0)}; }
     __device__ static inline State<T> zero() { return {T(0), T(0)}; }
-    __device__ static inline State<T> fold(State<T> s, int k, T x, const Params<T>& p) {
-        s.s0 = fma(p.a[k], x, s.s0);
+    __device__ static inline State<T> fold_c(State<T> s, const AxisC<T>& c, T x, const Params<T>&) {
+        s.s0 = fma(c.a, x, s.s0);
         return s;
     }
     __device__ static inline State<T> merge(State<T> a, State<T> b) { return {a.s0 + b.s0, T(0)}; }
@@ -221,8 +232,8 @@ template <typename T>
 struct Family<TQ_FAM_SIN_SUM, T> {
     __device__ static inline State<T> start(const Params<T>&) { return {T(0), T(0)}; }
     __device__ static inline State<T> zero() { return {T(0), T(0)}; }
-    __device__ static inline State<T> fold(State<T> s, int k, T x, const Params<T>& p) {
-        s.s0 += tq_sin(fma(p.a[k], x, p.u[k]));
+    __device__ static inline State<T> fold_c(State<T> s, const AxisC<T>& c, T x, const Params<T>&) {
+        s.s0 += tq_sin(fma(c.a, x, c.u));
         return s;
     }
     __device__ static inline State<T> merge(State<T> a, State<T> b) { return {a.s0 + b.s0, T(0)}; }
@@ -234,7 +245,7 @@ template <typename T>
 struct Family<TQ_FAM_DOUBLE_GAUSSIAN, T> {
     __device__ static inline State<T> start(const Params<T>&) { return {T(0), T(0)}; }
     __device__ static inline State<T> zero() { return {T(0), T(0)}; }
-    __device__ static inline State<T> fold(State<T> s, int, T x, const Params<T>& p) {
+    __device__ static inline State<T> fold_c(State<T> s, const AxisC<T>&, T x, const Params<T>& p) {
         const T t1 = x - p.c1, t2 = x - p.c2;
         s.s0 = fma(t1, t1, s.s0);
         s.s1 = fma(t2, t2, s.s1);
@@ -252,10 +263,15 @@ template <typename T>
 struct Family<TQ_FAM_CONSTANT, T> {
     __device__ static inline State<T> start(const Params<T>&) { return {T(0), T(0)}; }
     __device__ static inline State<T> zero() { return {T(0), T(0)}; }
-    __device__ static inline State<T> fold(State<T> s, int, T, const Params<T>&) { return s; }
+    __device__ static inline State<T> fold_c(State<T> s, const AxisC<T>&, T, const Params<T>&) { return s; }
     __device__ static inline State<T> merge(State<T> a, State<T>) { return a; }
     __device__ static inline T finish(State<T>, const Params<T>&) { return T(1); }
 };
+
+template <int FAM, typename T>
+__device__ inline State<T> fold(State<T> s, int k, T x, const Params<T>& p) {
+    return Family<FAM, T>::fold_c(s, axis_c(p, k), x, p);
+}
 
 // Full evaluation at one point (VEGAS, Monte Carlo): fold every axis.
 template <int FAM, typename T, int MAXD>
@@ -264,7 +280,7 @@ __device__ inline T eval_point(const T* x, int d, const Params<T>& p) {
     State<T> s = F::start(p);
 #pragma unroll
     for (int k = 0; k < MAXD; ++k)
-        if (k < d) s = F::fold(s, k, x[k], p);
+        if (k < d) s = F::fold_c(s, axis_c(p, k), x[k], p);
     return F::finish(s, p);
 }
 

@@ -1,0 +1,197 @@
+// Tensor-product grid building blocks shared by the single-integral and the
+// batched kernels: per-axis maps, rule weights and the tile loop.
+#pragma once
+#include "tq_device.hpp"
+
+namespace tq {
+
+constexpr int kFlush = 32;  // FP32 partial length before FP64 flush
+
+// Per-axis affine map.  Newton-Cotes: x_j = a + j*h (x_{n-1} = b exactly, as
+// linspace).  Gauss-Legendre: x_j = alpha*r_j + beta (gaussian.py:159-162).
+struct Axis {
+    double a, h, b;
+};
+
+template <typename T>
+__device__ inline Axis make_axis(const T* domain, int k, int n, int rule) {
+    const double a = (double)domain[2 * k], b = (double)domain[2 * k + 1];
+    Axis ax;
+    if (rule == TQ_RULE_GAUSS_LEGENDRE) {
+        ax.a = 0.5 * (b - a);
+        ax.h = 0.5 * (a + b);
+        ax.b = b;
+    } else {
+        ax.a = a;
+        ax.h = (b - a) / (double)(n - 1);
+        ax.b = b;
+    }
+    return ax;
+}
+
+inline Axis make_axis_host(double a, double b, int n, int rule) {
+    Axis ax;
+    if (rule == TQ_RULE_GAUSS_LEGENDRE) {
+        ax.a = 0.5 * (b - a);
+        ax.h = 0.5 * (a + b);
+        ax.b = b;
+    } else {
+        ax.a = a;
+        ax.h = (b - a) / (double)(n - 1);
+        ax.b = b;
+    }
+    return ax;
+}
+
+__device__ inline double node_of(const Axis& ax, int j, int n, int rule, const double* glx) {
+    if (rule == TQ_RULE_GAUSS_LEGENDRE) return fma(ax.a, glx[j], ax.h);
+    return (j == n - 1) ? ax.b : fma((double)j, ax.h, ax.a);
+}
+
+// Composite Newton-Cotes weight pattern without the per-axis scale factor
+// (trapezoid.py:34-36 -> h/2*[1,2,..,2,1]; simpson.py:35-45 -> h/3*[1,4,2,..,4,1];
+// boole.py:35-47 -> h/22.5*[7,32,12,32,14,..,32,7]); Gauss-Legendre weight.
+__device__ __host__ inline double weight_of(int rule, int j, int n, const double* glw) {
+    const bool end = (j == 0) || (j == n - 1);
+    switch (rule) {
+        case -1: return 1.0;
+        case TQ_RULE_TRAPEZOID: return end ? 1.0 : 2.0;
+        case TQ_RULE_SIMPSON: return end ? 1.0 : ((j & 1) ? 4.0 : 2.0);
+        case TQ_RULE_BOOLE: return end ? 7.0 : ((j & 1) ? 32.0 : (((j & 3) == 2) ? 12.0 : 14.0));
+        default: return glw[j];
+    }
+}
+
+// Per-axis scale so that integral = prod_k scale_k * sum W f.
+__device__ __host__ inline double axis_scale(double a, double b, int n, int rule) {
+    switch (rule) {
+        case TQ_RULE_TRAPEZOID: return (b - a) / (double)(n - 1) / 2.0;
+        case TQ_RULE_SIMPSON: return (b - a) / (double)(n - 1) / 3.0;
+        case TQ_RULE_BOOLE: return (b - a) / (double)(n - 1) / 22.5;
+        default: return 0.5 * (b - a);
+    }
+}
+
+// One tile of the tensor-product index space.  m == 0 (one axis): node
+// 64 t + lane for loop items t in [t_begin, t_end).  Otherwise the lanes of the calling wave
+// take lane combinations lw*64 .. lw*64+63 of the last m axes, the wave walks
+// loop items [t_begin, t_end) (row axes [0, L) x loop axis L).  Returns the
+// lane's weighted sum.  `ax` holds the per-axis maps, `tab` the loop-axis
+// (node, weight) pairs when TABLE.
+template <typename T, int FAM, bool TABLE>
+__device__ inline double grid_tile(int d, int n, int m, int L, int rule, int lane_count, const Params<T>& p,
+                                   const Axis* ax, const T* tab, const double* glx, const double* glw, int lw,
+                                   long long t_begin, long long t_end) {
+    using F = Family<FAM, T>;
+    const int lane = threadIdx.x & (kWave - 1);
+    double acc = 0.0;
+    if (m == 0) {
+        // one axis split into rows of 64 consecutive nodes: node k = 64 t + lane
+        const State<T> s0 = F::start(p);
+        const AxisC<T> ac0 = axis_c(p, 0);
+        for (long long tb = t_begin; tb < t_end; tb += kFlush) {
+            const long long te = (tb + kFlush < t_end) ? (tb + kFlush) : t_end;
+            T fa = T(0);
+            for (long long t = tb; t < te; ++t) {
+                const long long k = t * kWave + lane;
+                if (k < n) {
+                    const int kk = (int)k;
+                    const T x = (T)node_of(ax[0], kk, n, rule, glx);
+                    const T w = (T)weight_of(rule, kk, n, glw);
+                    fa = fma(w, F::finish(F::fold_c(s0, ac0, x, p), p), fa);
+                }
+            }
+            acc += (double)fa;
+        }
+        return acc;
+    }
+    const int li = lw * kWave + lane;
+    const bool active = li < lane_count;
+
+    // lane axes: fold and weight once
+    State<T> s_lane = F::zero();
+    double w_lane = active ? 1.0 : 0.0;
+    {
+        int rem = active ? li : 0;
+        for (int k = d - 1; k >= d - m; --k) {
+            const int j = rem % n;
+            rem /= n;
+            s_lane = F::fold_c(s_lane, axis_c(p, k), (T)node_of(ax[k], j, n, rule, glx), p);
+            w_lane *= weight_of(rule, j, n, glw);
+        }
+    }
+    if (L < 0) {
+        if (t_begin == 0 && t_end > 0) {
+            const T f = F::finish(F::merge(F::start(p), s_lane), p);
+            acc = w_lane * (double)f;
+        }
+        return acc;
+    }
+    const AxisC<T> acL = axis_c(p, L);
+    long long t = t_begin;
+    long long row = t / n;
+    int j = (int)(t - row * n);
+    // row odometer digits for axes [0, L)
+    int rdig[TQ_MAX_DIM];
+    {
+        long long r = row;
+#pragma unroll
+        for (int k = TQ_MAX_DIM - 1; k >= 0; --k) {
+            if (k < L) {
+                rdig[k] = (int)(r % n);
+                r /= n;
+            } else {
+                rdig[k] = 0;
+            }
+        }
+    }
+    while (t < t_end) {
+        State<T> s_row = F::start(p);
+        double w_row = w_lane;
+#pragma unroll
+        for (int k = 0; k < TQ_MAX_DIM; ++k) {
+            if (k < L) {
+                s_row = F::fold_c(s_row, axis_c(p, k), (T)node_of(ax[k], rdig[k], n, rule, glx), p);
+                w_row *= weight_of(rule, rdig[k], n, glw);
+            }
+        }
+        const State<T> s_rl = F::merge(s_row, s_lane);
+        const long long left = t_end - t;
+        const int jend = (left < (long long)(n - j)) ? (j + (int)left) : n;
+        double racc = 0.0;
+        for (int jb = j; jb < jend; jb += kFlush) {
+            const int je = (jb + kFlush < jend) ? (jb + kFlush) : jend;
+            T fa = T(0);
+#pragma unroll 4
+            for (int jj = jb; jj < je; ++jj) {
+                T xL, wL;
+                if (TABLE) {
+                    xL = tab[2 * jj];
+                    wL = tab[2 * jj + 1];
+                } else {
+                    xL = (T)node_of(ax[L], jj, n, rule, glx);
+                    wL = (T)weight_of(rule, jj, n, glw);
+                }
+                const T f = F::finish(F::fold_c(s_rl, acL, xL, p), p);
+                fa = fma(wL, f, fa);
+            }
+            racc += (double)fa;
+        }
+        acc = fma(w_row, racc, acc);
+        t += jend - j;
+        j = jend;
+        if (j == n) {
+            j = 0;
+#pragma unroll
+            for (int k = TQ_MAX_DIM - 1; k >= 0; --k) {
+                if (k < L) {
+                    if (++rdig[k] < n) break;
+                    rdig[k] = 0;
+                }
+            }
+        }
+    }
+    return acc;
+}
+
+}  // namespace tq

@@ -625,7 +625,7 @@ __global__ __launch_bounds__(kBlock) void k_vegas_map_reduce(tq_vegas_state st, 
     const long long pair = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int lane = threadIdx.x & (kWave - 1);
     long long total = nsamp >= 0 ? nsamp : (long long)st.iter_stats[4 * it + 2];
-    if (total > st.max_samples) total = st.max_samples;
+    if (nsamp < 0 && total > st.max_samples) total = st.max_samples;
     long long nb = (total + kSB - 1) / kSB;
     if (nb > st.max_blocks) nb = st.max_blocks;
     if (pair >= W) return;
