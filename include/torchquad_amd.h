@@ -171,6 +171,48 @@ int tq_mc_reduce(int32_t dtype, const void* fvals, int64_t N, int64_t S, const v
 int tq_uniform(uint64_t seed, uint64_t offset, int32_t dtype, int64_t rows, int32_t cols,
                void* out, void* stream);
 
+/* ---- batched form ------------------------------------------------------
+ * Many independent tensor-product integrals (mixed rule, family, dim, n,
+ * domain, parameters) in one persistent kernel.  The reference has no batched
+ * integrator; its nearest relatives are vectorised integrands
+ * (base_integrator.py:77-89) and the tutorial's Batch1DIntegrator
+ * (docs tutorial.rst:800-888).  Items carry the per-axis n already rounded
+ * by the rule (simpson.py:54-81, boole.py:56-84, integration_grid.py:64). */
+typedef struct tq_batch_item {
+    int32_t rule;
+    int32_t family;
+    int32_t dim;
+    int32_t n;               /* points per axis                                   */
+    int64_t gl_offset;       /* first node of this n-point rule in the GL tables  */
+    double domain[2 * TQ_MAX_DIM];
+    double c0, c1, c2, c3;
+    double a[TQ_MAX_DIM];
+    double u[TQ_MAX_DIM];
+} tq_batch_item;
+
+typedef struct tq_batch_tile {
+    int32_t item;            /* integral index                                    */
+    int32_t lane_wave;       /* 64-lane group of the integral's lane axes         */
+    int64_t t0, t1;          /* loop items [t0, t1)                               */
+} tq_batch_tile;
+
+/* Host-side plan sizes: bytes of the device descriptor array and tile count
+ * for tiles of about tile_points points. */
+int tq_batch_plan_sizes(const tq_batch_item* items, int64_t count, int32_t dtype, int64_t tile_points,
+                        int64_t* desc_bytes, int64_t* n_tiles);
+/* Host-side plan: descriptors (desc_out, host memory of desc_bytes), tiles
+ * ordered longest integral first with each integral's tiles contiguous,
+ * tile_begin[i] / tile_count[i] for integral i. */
+int tq_batch_plan(const tq_batch_item* items, int64_t count, int32_t dtype, int64_t tile_points, void* desc_out,
+                  tq_batch_tile* tiles_out, int64_t* tile_begin, int64_t* tile_count);
+/* Runs a plan whose arrays were copied to the device: clears *counter,
+ * persistent kernel of `blocks` workgroups (0 = automatic) pulling tiles,
+ * then out[i] (dtype) / out64[i] = integral i.  tile_partials: [n_tiles]. */
+int tq_batch_run(int32_t dtype, const void* desc, int64_t count, const tq_batch_tile* tiles, int64_t n_tiles,
+                 const int64_t* tile_begin, const int64_t* tile_count, const double* gl_nodes,
+                 const double* gl_weights, double* tile_partials, unsigned int* counter, void* out, double* out64,
+                 int32_t blocks, void* stream);
+
 /* ---- VEGAS ----------------------------------------------------------------
  * Device-resident VEGAS+ state (reference: vegas_map.py, vegas_stratification.py,
  * vegas.py).  Every buffer is caller-owned device memory; the library only

@@ -1,0 +1,123 @@
+"""Batched form on the GPU vs the oracle, the plain path and itself."""
+import math
+import warnings
+
+import numpy as np
+import pytest
+import torch
+
+import torchquad_amd as tq
+from torchquad_amd.batch import Batch, n_per_axis
+from torchquad_amd.workloads import genz_batch
+from oracle import families as fam
+from oracle import grid as og
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _quiet():
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        yield
+
+
+def oracle_value(it, n):
+    dim, rule, dom = it["dim"], it["rule"], it["domain"]
+    a, u, w = it["a"], it["u"], it["w"]
+    if rule == "gauss_legendre" and n > 1000:
+        lo, hi = [0.0] * dim, [1.0] * dim
+        return {"gaussian_peak": lambda: fam.gaussian_peak_integral(a, u, lo, hi),
+                "oscillatory": lambda: fam.oscillatory_integral(a, w, lo, hi),
+                "product_peak": lambda: fam.product_peak_integral(a, u, lo, hi)}[it["family"]]()
+    if it["family"] == "gaussian_peak":
+        return og.separable_gaussian_peak(rule, a, u, dom, n)
+    if it["family"] == "oscillatory":
+        return og.separable_oscillatory(rule, a, w, dom, n)
+    return og.separable_product(rule, [lambda x, ak=ak, uk=uk: 1.0 / (ak**-2.0 + (x - uk) ** 2)
+                                       for ak, uk in zip(a, u)], dom, n)
+
+
+def sup_f(it):
+    if it["family"] == "product_peak":
+        return float(np.prod(it["a"] ** 2))
+    return 1.0
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_batch_vs_oracle(dtype, dev):
+    items = genz_batch(300, seed=3)
+    b = Batch(items, dtype=dtype)
+    _, out64 = b.run()
+    res = out64.cpu().numpy()
+    tol = 1e-6 if dtype == torch.float32 else 1e-11
+    for i, it in enumerate(items):
+        rule = tq.batch.RULE_NAMES[it["rule"]]
+        n = n_per_axis(rule, it["dim"], it["N"])
+        ref = oracle_value(it, n)
+        scale = max(abs(ref), sup_f(it))
+        assert abs(res[i] - ref) <= tol * scale, (i, it["rule"], it["family"], it["dim"], n, res[i], ref)
+
+
+def test_batch_matches_plain_path_and_is_deterministic(dev):
+    items = genz_batch(64, seed=9)
+    b = Batch(items, dtype=torch.float32)
+    out, out64 = b.run()
+    first = out64.clone()
+    b.run()
+    assert torch.equal(first, b.out64)
+    rules = {"trapezoid": tq.Trapezoid, "simpson": tq.Simpson, "gauss_legendre": tq.GaussLegendre}
+    for i, it in enumerate(items[:24]):
+        dom = torch.tensor(it["domain"], dtype=torch.float32, device=dev)
+        plain = float(rules[it["rule"]]().integrate(it["fn"], it["dim"], N=it["N"], integration_domain=dom))
+        assert abs(plain - float(out[i])) <= 2e-6 * max(abs(plain), sup_f(it)), (i, plain, float(out[i]))
+    # a different tiling gives the same values up to FP32 partial rounding
+    b.retile(tile_points=2048, blocks=1024)
+    b.run()
+    np.testing.assert_allclose(b.out64.cpu().numpy(), first.cpu().numpy(), rtol=2e-7, atol=1e-9)
+
+
+def test_batch_graph_replay(dev):
+    items = genz_batch(128, seed=4)
+    b = Batch(items, dtype=torch.float32)
+    _, ref = b.run()
+    ref = ref.clone()
+    g = b.graph()
+    for _ in range(3):
+        b.out64.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(b.out64, ref)
+        # eager work between replays
+        torch.ones(1000, device=dev).sum().item()
+
+
+def test_autotune_picks_a_candidate(dev):
+    from torchquad_amd import autotune
+
+    b = Batch(genz_batch(200, seed=5), dtype=torch.float32)
+    _, ref = b.run()
+    ref = ref.clone()
+    (tp, bl), table = autotune.autotune(b, tiles=(2048, 8192), blocks=(1024, 2048), reps=2, use_cache=False)
+    assert (tp, bl) in {(t, q) for t, q, _ in table}
+    b.run()
+    np.testing.assert_allclose(b.out64.cpu().numpy(), ref.cpu().numpy(), rtol=2e-7, atol=1e-9)
+
+
+def test_gl_generator_large_n(dev):
+    from oracle import gl as ogl
+    from torchquad_amd.gl_nodes import gauss_legendre
+
+    for n in (1000, 5000):
+        x, w = gauss_legendre(n, dev)
+        xs, ws = x.cpu().numpy(), w.cpu().numpy()
+        for k in (0, 31, 40, 200, n // 2 - 1):
+            mx, mw = ogl.node_weight(n, k)
+            assert abs(xs[k] - mx) <= 2e-16 * max(1.0, n / 1000), (n, k, xs[k] - mx)
+            assert abs(ws[k] - mw) <= 2e-12 * mw, (n, k, (ws[k] - mw) / mw)
+    for n in (200_000, 1_000_000):
+        x, w = gauss_legendre(n, dev)
+        assert abs(float(w.sum()) - 2.0) < 1e-12
+        assert abs(float(torch.sum(w * x * x)) - 2.0 / 3.0) < 1e-12
+        assert float((x[1:] - x[:-1]).min()) > 0
+        assert torch.allclose(x, -x.flip(0), atol=1e-16, rtol=0)

@@ -66,6 +66,32 @@ class GridDesc(ctypes.Structure):
     ]
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [
+        ("rule", ctypes.c_int32),
+        ("family", ctypes.c_int32),
+        ("dim", ctypes.c_int32),
+        ("n", ctypes.c_int32),
+        ("gl_offset", ctypes.c_int64),
+        ("domain", ctypes.c_double * (2 * TQ_MAX_DIM)),
+        ("c0", ctypes.c_double),
+        ("c1", ctypes.c_double),
+        ("c2", ctypes.c_double),
+        ("c3", ctypes.c_double),
+        ("a", ctypes.c_double * TQ_MAX_DIM),
+        ("u", ctypes.c_double * TQ_MAX_DIM),
+    ]
+
+
+class BatchTile(ctypes.Structure):
+    _fields_ = [
+        ("item", ctypes.c_int32),
+        ("lane_wave", ctypes.c_int32),
+        ("t0", ctypes.c_int64),
+        ("t1", ctypes.c_int64),
+    ]
+
+
 class McDesc(ctypes.Structure):
     _fields_ = [
         ("dtype", ctypes.c_int32),
@@ -102,6 +128,9 @@ SIGNATURES = {
     "tq_grid_points_backward_workspace_size": (_SZ, [ctypes.POINTER(GridDesc)]),
     "tq_grid_points_backward": (ctypes.c_int, [ctypes.POINTER(GridDesc), _P, _P, _P, _SZ, _P, _P]),
     "tq_gauss_legendre": (ctypes.c_int, [_I32, _P, _P, _P]),
+    "tq_batch_plan_sizes": (ctypes.c_int, [_P, _I64, _I32, _I64, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    "tq_batch_plan": (ctypes.c_int, [_P, _I64, _I32, _I64, _P, _P, _P, _P]),
+    "tq_batch_run": (ctypes.c_int, [_I32, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P]),
     "tq_mc_workspace_size": (_SZ, [ctypes.POINTER(McDesc)]),
     "tq_mc_integrate": (
         ctypes.c_int,

@@ -1,0 +1,312 @@
+// Batched form: thousands of independent tensor-product integrals of mixed
+// rule, family, dimension and size in one persistent kernel.
+//
+// The host plan splits every integral into tiles of ~tile_points points
+// (one 64-lane group x a range of loop items, as grid_tile walks them) and
+// orders the tile list by decreasing integral size (longest first), the
+// tiles of an integral contiguous.  Each wave of the persistent kernel pulls
+// the next tile index from a device counter, stages that integral's axis
+// maps and loop-axis table in its own LDS slice, runs the tile and writes one
+// FP64 partial.  A second kernel reduces each integral's tile partials in
+// tile order, so results are deterministic and identical to the partition
+// chosen by any tuning.  No memset is issued (the counter is cleared by a
+// kernel), so the pair of launches can be captured in a graph and replayed.
+#include <algorithm>
+
+#include "tq_grid_tile.hpp"
+
+namespace tq {
+
+template <typename T>
+struct BatchDev {
+    int rule, fam, d, n, m, L, lane_count, lane_waves;
+    long long items;
+    long long gl_offset;
+    double scale;
+    Axis ax[TQ_MAX_DIM];
+    Params<T> p;
+};
+
+template <typename T>
+struct BatchTab {
+    static constexpr int N = (sizeof(T) == 4) ? 1024 : 512;  // per-wave loop table entries
+};
+
+__global__ void k_batch_reset(unsigned int* counter) {
+    if (threadIdx.x == 0) *counter = 0u;
+}
+
+template <typename T, bool TABLE>
+__device__ inline double batch_tile(const BatchDev<T>& D, const Axis* ax, const T* tab, const double* glx,
+                                    const double* glw, int lw, long long t0, long long t1) {
+    switch (D.fam) {
+#define TQ_BT(F)                                                                                                   \
+    case F:                                                                                                        \
+        return grid_tile<T, F, TABLE>(D.d, D.n, D.m, D.L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw, t0, t1);
+        TQ_BT(TQ_FAM_GAUSSIAN_PEAK)
+        TQ_BT(TQ_FAM_OSCILLATORY)
+        TQ_BT(TQ_FAM_PRODUCT_PEAK)
+        TQ_BT(TQ_FAM_EXP_LINEAR)
+        TQ_BT(TQ_FAM_SIN_SUM)
+        TQ_BT(TQ_FAM_DOUBLE_GAUSSIAN)
+        TQ_BT(TQ_FAM_CONSTANT)
+#undef TQ_BT
+        default: return 0.0;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_batch(const BatchDev<T>* descs, const tq_batch_tile* tiles,
+                                                  long long n_tiles, const double* glx, const double* glw,
+                                                  double* tile_part, unsigned int* counter) {
+    constexpr int NT = BatchTab<T>::N;
+    __shared__ Axis s_ax[kWavesPerBlock][TQ_MAX_DIM];
+    __shared__ T s_tab[kWavesPerBlock][2 * NT];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & (kWave - 1);
+    for (;;) {
+        unsigned int t = 0;
+        if (lane == 0) t = atomicAdd(counter, 1u);
+        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, kWave));
+        if ((long long)t >= n_tiles) break;
+        const tq_batch_tile tile = tiles[t];
+        const BatchDev<T>& D = descs[tile.item];
+        const double* gx = glx ? glx + D.gl_offset : nullptr;
+        const double* gw = glw ? glw + D.gl_offset : nullptr;
+        for (int k = lane; k < D.d; k += kWave) s_ax[wave][k] = D.ax[k];
+        const bool table = D.m > 0 && D.L >= 0 && D.n <= NT;
+        if (table) {
+            const Axis axL = D.ax[D.L];
+            for (int j = lane; j < D.n; j += kWave) {
+                s_tab[wave][2 * j] = (T)node_of(axL, j, D.n, D.rule, gx);
+                s_tab[wave][2 * j + 1] = (T)weight_of(D.rule, j, D.n, gw);
+            }
+        }
+        // make this wave's LDS writes visible to all of its lanes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        double acc = table ? batch_tile<T, true>(D, s_ax[wave], s_tab[wave], gx, gw, tile.lane_wave, tile.t0, tile.t1)
+                           : batch_tile<T, false>(D, s_ax[wave], s_tab[wave], gx, gw, tile.lane_wave, tile.t0, tile.t1);
+        acc = wave_sum(acc);
+        if (lane == 0) tile_part[t] = acc;
+        // all lanes are done with this tile's LDS before the next one writes it
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_batch_finalize(const BatchDev<T>* descs, long long count,
+                                                           const int64_t* tile_begin, const int64_t* tile_count,
+                                                           const double* tile_part, T* out, double* out64) {
+    const long long i = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (i >= count) return;
+    const long long b = tile_begin[i], c = tile_count[i];
+    double v = 0.0;
+    for (long long j = lane; j < c; j += kWave) v += tile_part[b + j];
+    v = wave_sum(v);
+    if (lane == 0) {
+        const double r = v * descs[i].scale;
+        if (out) out[i] = (T)r;
+        if (out64) out64[i] = r;
+    }
+}
+
+static int check_item(const tq_batch_item& it) {
+    if (it.family < 0 || it.family >= TQ_NUM_FAMILIES) return fail(TQ_E_ARG, "batch item: unknown family");
+    if (it.dim < 1 || it.dim > TQ_MAX_DIM) return fail(TQ_E_ARG, "batch item: dim out of range (1..16)");
+    switch (it.rule) {
+        case TQ_RULE_TRAPEZOID:
+            if (it.n < 2) return fail(TQ_E_ARG, "batch item: Trapezoid needs n >= 2");
+            break;
+        case TQ_RULE_SIMPSON:
+            if (it.n < 3 || it.n % 2 != 1) return fail(TQ_E_ARG, "batch item: Simpson needs odd n >= 3");
+            break;
+        case TQ_RULE_BOOLE:
+            if (it.n < 5 || (it.n - 1) % 4 != 0) return fail(TQ_E_ARG, "batch item: Boole needs n = 1 + 4k");
+            break;
+        case TQ_RULE_GAUSS_LEGENDRE:
+            if (it.n < 1 || it.gl_offset < 0) return fail(TQ_E_ARG, "batch item: bad Gauss-Legendre n/offset");
+            break;
+        default: return fail(TQ_E_ARG, "batch item: unknown rule");
+    }
+    long long P;
+    if (ipow_checked(it.n, it.dim, 1LL << 62, &P) != 0) return fail(TQ_E_ARG, "batch item: n^dim too large");
+    return TQ_OK;
+}
+
+static long long tile_ipw(const GridGeom& G, long long tile_points) {
+    long long ipw = tile_points / kWave;
+    if (ipw < 1) ipw = 1;
+    if (ipw > G.items) ipw = G.items;
+    return ipw;
+}
+
+template <typename T>
+static int plan_batch(const tq_batch_item* items, long long count, long long tile_points, BatchDev<T>* desc,
+                      tq_batch_tile* tiles, int64_t* tile_begin, int64_t* tile_count, long long* n_tiles_out) {
+    struct Entry {
+        long long cost;
+        long long idx;
+    };
+    Entry* order = new Entry[count > 0 ? count : 1];
+    GridGeom* geo = new GridGeom[count > 0 ? count : 1];
+    long long total = 0;
+    for (long long i = 0; i < count; ++i) {
+        const tq_batch_item& it = items[i];
+        int rc = check_item(it);
+        if (!rc) rc = plan_geom(it.dim, it.n, it.rule, 0, 1, 1, geo[i]);
+        if (rc) {
+            delete[] order;
+            delete[] geo;
+            return rc;
+        }
+        const long long ipw = tile_ipw(geo[i], tile_points);
+        const long long chunks = (geo[i].items + ipw - 1) / ipw;
+        geo[i].ipw = ipw;
+        geo[i].chunks = chunks;
+        long long P;
+        ipow_checked(it.n, it.dim, 1LL << 62, &P);
+        order[i] = {P, i};
+        total += chunks * geo[i].lane_waves;
+        if (desc) {
+            BatchDev<T>& D = desc[i];
+            D.rule = it.rule;
+            D.fam = it.family;
+            D.d = it.dim;
+            D.n = it.n;
+            D.m = geo[i].m;
+            D.L = geo[i].L;
+            D.lane_count = geo[i].lane_count;
+            D.lane_waves = geo[i].lane_waves;
+            D.items = geo[i].items;
+            D.gl_offset = it.rule == TQ_RULE_GAUSS_LEGENDRE ? it.gl_offset : 0;
+            double scale = it.c0;
+            for (int k = 0; k < TQ_MAX_DIM; ++k) {
+                if (k < it.dim) {
+                    D.ax[k] = make_axis_host(it.domain[2 * k], it.domain[2 * k + 1], it.n, it.rule);
+                    scale *= axis_scale(it.domain[2 * k], it.domain[2 * k + 1], it.n, it.rule);
+                } else {
+                    D.ax[k] = Axis{0.0, 0.0, 0.0};
+                }
+            }
+            D.scale = scale;
+            tq_integrand f{};
+            f.family = it.family;
+            f.dim = it.dim;
+            f.c0 = it.c0;
+            f.c1 = it.c1;
+            f.c2 = it.c2;
+            f.c3 = it.c3;
+            for (int k = 0; k < TQ_MAX_DIM; ++k) {
+                f.a[k] = it.a[k];
+                f.u[k] = it.u[k];
+            }
+            D.p = make_params<T>(f);
+        }
+    }
+    if (tiles) {
+        // longest integrals first; stable on the original index
+        std::sort(order, order + count, [](const Entry& x, const Entry& y) {
+            return x.cost != y.cost ? x.cost > y.cost : x.idx < y.idx;
+        });
+        long long t = 0;
+        for (long long o = 0; o < count; ++o) {
+            const long long i = order[o].idx;
+            const GridGeom& G = geo[i];
+            tile_begin[i] = t;
+            for (int lw = 0; lw < G.lane_waves; ++lw) {
+                for (long long c = 0; c < G.chunks; ++c) {
+                    tq_batch_tile& tl = tiles[t++];
+                    tl.item = (int32_t)i;
+                    tl.lane_wave = lw;
+                    tl.t0 = c * G.ipw;
+                    tl.t1 = (c + 1) * G.ipw < G.items ? (c + 1) * G.ipw : G.items;
+                }
+            }
+            tile_count[i] = t - tile_begin[i];
+        }
+    }
+    *n_tiles_out = total;
+    delete[] order;
+    delete[] geo;
+    return TQ_OK;
+}
+
+template <typename T>
+static int run_batch(const void* desc, long long count, const tq_batch_tile* tiles, long long n_tiles,
+                     const int64_t* tile_begin, const int64_t* tile_count, const double* glx, const double* glw,
+                     double* tile_part, unsigned int* counter, void* out, double* out64, int blocks,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counter);
+    int rc = check_launch("k_batch_reset");
+    if (rc) return rc;
+    if (blocks <= 0) blocks = 2048;
+    long long need = (n_tiles + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (need < 1) need = 1;
+    if (blocks > need) blocks = (int)need;
+    hipLaunchKernelGGL((k_batch<T>), dim3(blocks), dim3(kBlock), 0, s, (const BatchDev<T>*)desc, tiles, n_tiles,
+                       glx, glw, tile_part, counter);
+    rc = check_launch("k_batch");
+    if (rc) return rc;
+    const long long fb = (count + kWavesPerBlock - 1) / kWavesPerBlock;
+    hipLaunchKernelGGL((k_batch_finalize<T>), dim3((unsigned)(fb > 0 ? fb : 1)), dim3(kBlock), 0, s,
+                       (const BatchDev<T>*)desc, count, tile_begin, tile_count, tile_part, (T*)out, out64);
+    return check_launch("k_batch_finalize");
+}
+
+}  // namespace tq
+
+using namespace tq;
+
+extern "C" {
+
+int tq_batch_plan_sizes(const tq_batch_item* items, int64_t count, int32_t dtype, int64_t tile_points,
+                        int64_t* desc_bytes, int64_t* n_tiles) {
+    if (!items || !desc_bytes || !n_tiles) return fail(TQ_E_NULL, "NULL argument");
+    if (count < 0 || tile_points < 1) return fail(TQ_E_ARG, "count/tile_points out of range");
+    if (dtype != TQ_F32 && dtype != TQ_F64) return fail(TQ_E_ARG, "dtype must be TQ_F32 or TQ_F64");
+    long long nt = 0;
+    int rc = dtype == TQ_F32 ? plan_batch<float>(items, count, tile_points, nullptr, nullptr, nullptr, nullptr, &nt)
+                             : plan_batch<double>(items, count, tile_points, nullptr, nullptr, nullptr, nullptr, &nt);
+    if (rc) return rc;
+    *n_tiles = nt;
+    *desc_bytes = (int64_t)count * (dtype == TQ_F32 ? (int64_t)sizeof(BatchDev<float>) : (int64_t)sizeof(BatchDev<double>));
+    return TQ_OK;
+}
+
+int tq_batch_plan(const tq_batch_item* items, int64_t count, int32_t dtype, int64_t tile_points, void* desc_out,
+                  tq_batch_tile* tiles_out, int64_t* tile_begin, int64_t* tile_count) {
+    if (!items || !desc_out || !tiles_out || !tile_begin || !tile_count) return fail(TQ_E_NULL, "NULL argument");
+    if (count < 0 || tile_points < 1) return fail(TQ_E_ARG, "count/tile_points out of range");
+    long long nt = 0;
+    if (dtype == TQ_F32)
+        return plan_batch<float>(items, count, tile_points, (BatchDev<float>*)desc_out, tiles_out, tile_begin,
+                                 tile_count, &nt);
+    if (dtype == TQ_F64)
+        return plan_batch<double>(items, count, tile_points, (BatchDev<double>*)desc_out, tiles_out, tile_begin,
+                                  tile_count, &nt);
+    return fail(TQ_E_ARG, "dtype must be TQ_F32 or TQ_F64");
+}
+
+int tq_batch_run(int32_t dtype, const void* desc, int64_t count, const tq_batch_tile* tiles, int64_t n_tiles,
+                 const int64_t* tile_begin, const int64_t* tile_count, const double* gl_nodes,
+                 const double* gl_weights, double* tile_partials, unsigned int* counter, void* out, double* out64,
+                 int32_t blocks, void* stream) {
+    if (!desc || !tiles || !tile_begin || !tile_count || !tile_partials || !counter)
+        return fail(TQ_E_NULL, "NULL argument");
+    if (count < 0 || n_tiles < 0 || n_tiles > 0xffffffffLL) return fail(TQ_E_ARG, "count/n_tiles out of range");
+    if (!out && !out64) return fail(TQ_E_NULL, "no output pointer");
+    if (dtype == TQ_F32)
+        return run_batch<float>(desc, count, tiles, n_tiles, tile_begin, tile_count, gl_nodes, gl_weights,
+                                tile_partials, counter, out, out64, blocks, as_stream(stream));
+    if (dtype == TQ_F64)
+        return run_batch<double>(desc, count, tiles, n_tiles, tile_begin, tile_count, gl_nodes, gl_weights,
+                                 tile_partials, counter, out, out64, blocks, as_stream(stream));
+    return fail(TQ_E_ARG, "dtype must be TQ_F32 or TQ_F64");
+}
+
+}  // extern "C"

@@ -1,10 +1,20 @@
-// Gauss-Legendre nodes and weights on [-1, 1] for any n, computed on the
-// device in O(n) per node: Newton iteration on P_n from the three-term
-// (Bonnet) recurrence j P_j = (2j-1) x P_{j-1} - (j-1) P_{j-2}, started
-// from Tricomi's asymptotic node estimate
+// Gauss-Legendre nodes and weights on [-1, 1] for any n, on the device.
+//
+// Near the ends (n sin(theta) < 50) and for n < 200: Newton iteration on
+// P_n from the three-term (Bonnet) recurrence
+//     j P_j = (2j-1) x P_{j-1} - (j-1) P_{j-2},
+// started from Tricomi's estimate
 //     x_k ~ (1 - 1/(8 n^2) + 1/(8 n^3)) cos(pi (k - 1/4) / (n + 1/2)),
 // with P_n'(x) = n (x P_n - P_{n-1}) / (x^2 - 1) and weights
 //     w_k = 2 / ((1 - x_k^2) P_n'(x_k)^2).
+// Interior nodes of large n: Newton in theta on the Stieltjes asymptotic
+// expansion used by Hale & Townsend (SIAM J. Sci. Comput. 35(2), 2013)
+//     P_n(cos t) = C_n sum_m h_m cos(a_m) / (2 sin t)^(m+1/2),
+//     a_m = (n+m+1/2) t - (m+1/2) pi/2,  h_0 = 1,
+//     h_m = h_{m-1} (m-1/2)^2 / (m (n+m+1/2)),
+//     C_n = (4/pi) prod_{j=1..n} j/(j+1/2),
+// truncated after kAsymTerms terms (relative error < 1e-17 for
+// n sin t >= 50), with w = 2 / (dP_n/dt)^2.  O(1) work per interior node.
 // Output is in ascending order like numpy.polynomial.legendre.leggauss,
 // the reference's root function (gaussian.py:25).
 #include "tq_host.hpp"
@@ -22,13 +32,64 @@ __device__ inline void legendre_pair(int n, double x, double* pn, double* pn1) {
     *pn1 = p0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_gl_nodes(int n, double* xs, double* ws) {
+constexpr int kAsymTerms = 14;
+constexpr int kAsymMinN = 200;
+constexpr double kAsymMinNSin = 50.0;
+
+// f(t) = sum_m h_m cos(a_m) / (2 sin t)^(m+1/2) and f'(t).
+__device__ inline void stieltjes(int n, double t, double* f, double* fp) {
+    const double s = sin(t), c = cos(t);
+    const double inv2s = 0.5 / s;
+    const double nn = (double)n;
+    double pw = sqrt(inv2s);
+    double h = 1.0;
+    const double a0 = (nn + 0.5) * t - 0.25 * M_PI;
+    double ca = cos(a0), sa = sin(a0);
+    double F = 0.0, FP = 0.0;
+    for (int m = 0; m < kAsymTerms; ++m) {
+        if (m > 0) {
+            h *= ((m - 0.5) * (m - 0.5)) / ((double)m * (nn + m + 0.5));
+            pw *= inv2s;
+            // a_m = a_{m-1} + t - pi/2
+            const double ca2 = ca * s + sa * c;
+            const double sa2 = sa * s - ca * c;
+            ca = ca2;
+            sa = sa2;
+        }
+        const double term = h * pw;
+        F += term * ca;
+        FP += term * (-(nn + m + 0.5) * sa - (m + 0.5) * ca * (c / s));
+    }
+    *f = F;
+    *fp = FP;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gl_nodes(int n, double cn, double* xs, double* ws) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int half = (n + 1) / 2;
     if (k >= half) return;
     const double nn = (double)n;
-    double x = (1.0 - 1.0 / (8.0 * nn * nn) + 1.0 / (8.0 * nn * nn * nn)) *
-               cos(M_PI * ((double)k + 0.75) / (nn + 0.5));
+    const double t0 = M_PI * ((double)k + 0.75) / (nn + 0.5);
+    if (n >= kAsymMinN && nn * sin(t0) >= kAsymMinNSin) {
+        double t = t0, f, fp;
+        for (int it = 0; it < 6; ++it) {
+            stieltjes(n, t, &f, &fp);
+            const double dt = f / fp;
+            t -= dt;
+            if (fabs(dt) <= 1e-17 * t) break;
+        }
+        stieltjes(n, t, &f, &fp);
+        double x = cos(t);
+        const double dpdt = cn * fp;
+        const double w = 2.0 / (dpdt * dpdt);
+        if ((n & 1) && k == half - 1) x = 0.0;
+        xs[n - 1 - k] = x;
+        ws[n - 1 - k] = w;
+        xs[k] = -x;
+        ws[k] = w;
+        return;
+    }
+    double x = (1.0 - 1.0 / (8.0 * nn * nn) + 1.0 / (8.0 * nn * nn * nn)) * cos(t0);
     double pn, pn1, dp;
     for (int it = 0; it < 16; ++it) {
         legendre_pair(n, x, &pn, &pn1);
@@ -56,6 +117,11 @@ extern "C" int tq_gauss_legendre(int32_t n, double* nodes, double* weights, void
     if (!nodes || !weights) return fail(TQ_E_NULL, "nodes/weights is NULL");
     const int half = (n + 1) / 2;
     const int blocks = (half + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_gl_nodes, dim3(blocks), dim3(kBlock), 0, as_stream(stream), (int)n, nodes, weights);
+    // C_n = (4/pi) prod_{j=1..n} j/(j+1/2), in extended precision
+    long double cn = 4.0L / 3.14159265358979323846264338327950288L;
+    if (n >= kAsymMinN)
+        for (int j = 1; j <= n; ++j) cn *= (long double)j / ((long double)j + 0.5L);
+    hipLaunchKernelGGL(k_gl_nodes, dim3(blocks), dim3(kBlock), 0, as_stream(stream), (int)n, (double)cn, nodes,
+                       weights);
     return check_launch("k_gl_nodes");
 }

@@ -19,20 +19,6 @@
 namespace tq {
 
 constexpr int kMaxDimPoints = 64;  // point writer / weighted reduce
-constexpr int kLoopTable = 2048;   // largest n kept in the LDS loop table
-
-struct GridGeom {
-    int d, n, m, L;
-    int rule;
-    int lane_count;       // n^m
-    int lane_waves;       // ceil(lane_count / 64)
-    int use_table;        // loop axis nodes/weights in LDS
-    long long items;      // n^(d-m) (1 when there is no loop axis)
-    long long ipw;        // items per wave
-    long long chunks;     // ceil(items / ipw)
-    long long num_waves;  // lane_waves * chunks
-    long long blocks;
-};
 
 template <typename T>
 struct GridArgs {
@@ -43,16 +29,6 @@ struct GridArgs {
     double* partials;
     Params<T> p;
 };
-
-static int ipow_checked(long long base, int e, long long limit, long long* out) {
-    long long r = 1;
-    for (int i = 0; i < e; ++i) {
-        if (r > limit / base) return -1;
-        r *= base;
-    }
-    *out = r;
-    return 0;
-}
 
 static int check_grid_desc(const tq_grid_desc* g, int max_dim) {
     if (!g) return fail(TQ_E_NULL, "grid descriptor is NULL");
@@ -83,75 +59,6 @@ static int check_grid_desc(const tq_grid_desc* g, int max_dim) {
 static int check_gl_tables(const tq_grid_desc* g) {
     if (g->rule == TQ_RULE_GAUSS_LEGENDRE && (!g->gl_nodes || !g->gl_weights))
         return fail(TQ_E_NULL, "Gauss-Legendre needs node/weight tables");
-    return TQ_OK;
-}
-
-// Splits the n^d index space of one integral (see grid_tile).  lane_axes /
-// items_per_wave override the automatic choice (0 = automatic).
-static int plan_geom(int d, int n, int rule, int lane_axes, long long items_per_wave, long long target_waves,
-                     GridGeom& G) {
-    G.d = d;
-    G.n = n;
-    G.rule = rule;
-    int m = 0;
-    if (lane_axes > 0) {
-        m = lane_axes < d ? lane_axes : d;
-    } else if (d == 1 && n >= 128) {
-        m = 0;  // rows of 64 consecutive nodes
-    } else {
-        // lane axes: the smallest m whose n^m lane combinations fill >= 90% of
-        // the 64-lane waves (else the best filling m); a loop axis is kept
-        // whenever d > 1
-        const int mmax = d > 1 ? d - 1 : 1;
-        m = mmax;
-        double best = -1.0;
-        long long c = 1;
-        for (int k = 1; k <= mmax; ++k) {
-            if (c > (1LL << 31) / n) break;
-            c *= n;
-            const double util = (double)c / (double)(((c + kWave - 1) / kWave) * kWave);
-            if (c >= 32 && util >= 0.9) {
-                m = k;
-                break;
-            }
-            if (util > best) {
-                best = util;
-                m = k;
-            }
-        }
-        if (d == 1) m = 1;
-    }
-    long long items = 1;
-    if (m == 0) {
-        G.m = 0;
-        G.lane_count = kWave;
-        G.lane_waves = 1;
-        G.L = 0;
-        items = ((long long)n + kWave - 1) / kWave;
-    } else {
-        long long lc;
-        if (ipow_checked(n, m, (1LL << 31) - 64, &lc) != 0) return fail(TQ_E_ARG, "lane combinations exceed 2^31");
-        G.m = m;
-        G.lane_count = (int)lc;
-        G.lane_waves = (int)((lc + kWave - 1) / kWave);
-        G.L = d - m - 1;
-        if (G.L >= 0) ipow_checked(n, d - m, 1LL << 62, &items);
-    }
-    G.items = items;
-    G.use_table = (G.m > 0 && G.L >= 0 && n <= kLoopTable) ? 1 : 0;
-    long long ipw = items_per_wave;
-    if (ipw <= 0) {
-        const long long work = items * (long long)G.lane_waves;
-        ipw = (work + target_waves - 1) / target_waves;
-        const long long min_ipw = items < 64 ? items : 64;
-        if (ipw < min_ipw) ipw = min_ipw;
-    }
-    if (ipw < 1) ipw = 1;
-    while (((items + ipw - 1) / ipw) * (long long)G.lane_waves > (1LL << 30)) ipw *= 2;
-    G.ipw = ipw;
-    G.chunks = (items + ipw - 1) / ipw;
-    G.num_waves = G.chunks * (long long)G.lane_waves;
-    G.blocks = (G.num_waves + kWavesPerBlock - 1) / kWavesPerBlock;
     return TQ_OK;
 }
 
