@@ -213,6 +213,20 @@ int tq_batch_run(int32_t dtype, const void* desc, int64_t count, const tq_batch_
                  const double* gl_weights, double* tile_partials, unsigned int* counter, void* out, double* out64,
                  int32_t blocks, void* stream);
 
+/* ---- differentiation ----------------------------------------------------
+ * Parameter sweep of the Gaussian peak f = exp(-sum a_k^2 (x_k-u_k)^2) over B
+ * integrals sharing rule / dim (1..8) / n with their own a[B][dim],
+ * u[B][dim] and domains[B][dim][2] (dtype), and its hand-written backward:
+ * grad_a, grad_u, grad_domains (any may be NULL) = grad_out[b] * d I_b / d.
+ * The reference differentiates with torch autograd through the materialised
+ * grid (gaussian.py:44-162, utils.py:24-58; tests/gradient_test.py). */
+size_t tq_sweep_workspace_size(const tq_grid_desc* g, int64_t B);
+int tq_sweep_gaussian_forward(const tq_grid_desc* g, int64_t B, const void* a, const void* u, const void* domains,
+                              void* workspace, size_t workspace_bytes, void* out, double* out64, void* stream);
+int tq_sweep_gaussian_backward(const tq_grid_desc* g, int64_t B, const void* a, const void* u, const void* domains,
+                               const void* grad_out, void* workspace, size_t workspace_bytes, void* grad_a,
+                               void* grad_u, void* grad_domains, void* stream);
+
 /* ---- VEGAS ----------------------------------------------------------------
  * Device-resident VEGAS+ state (reference: vegas_map.py, vegas_stratification.py,
  * vegas.py).  Every buffer is caller-owned device memory; the library only

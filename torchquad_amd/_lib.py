@@ -128,6 +128,12 @@ SIGNATURES = {
     "tq_grid_points_backward_workspace_size": (_SZ, [ctypes.POINTER(GridDesc)]),
     "tq_grid_points_backward": (ctypes.c_int, [ctypes.POINTER(GridDesc), _P, _P, _P, _SZ, _P, _P]),
     "tq_gauss_legendre": (ctypes.c_int, [_I32, _P, _P, _P]),
+    "tq_sweep_workspace_size": (_SZ, [ctypes.POINTER(GridDesc), _I64]),
+    "tq_sweep_gaussian_forward": (ctypes.c_int, [ctypes.POINTER(GridDesc), _I64, _P, _P, _P, _P, _SZ, _P, _P, _P]),
+    "tq_sweep_gaussian_backward": (
+        ctypes.c_int,
+        [ctypes.POINTER(GridDesc), _I64, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _P],
+    ),
     "tq_batch_plan_sizes": (ctypes.c_int, [_P, _I64, _I32, _I64, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "tq_batch_plan": (ctypes.c_int, [_P, _I64, _I32, _I64, _P, _P, _P, _P]),
     "tq_batch_run": (ctypes.c_int, [_I32, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P]),

@@ -1,0 +1,486 @@
+// Parameter sweeps of the Gaussian peak f(x) = exp(-sum_k a_k^2 (x_k-u_k)^2)
+// over B integrals that share rule, dimension and n but have their own
+// parameters (a_b, u_b) and box [lo_b, hi_b] -- forward value and the
+// hand-written backward.
+//
+// Every rule maps node j of axis k as x = alpha_k r_j + beta_k with
+// alpha = (hi-lo)/2, beta = (hi+lo)/2 and r_j in [-1, 1] (the Gauss-Legendre
+// root, or 2j/(n-1) - 1 for Newton-Cotes), and integral = S * sum W f with
+// S proportional to prod_k alpha_k.  With t = x - u and, per integral,
+//   F = sum W f,  E_k = sum W f t_k,  Q_k = sum W f t_k^2,  R_k = sum W f t_k r_k,
+// the gradients are
+//   dI/du_k = 2 a_k^2 S E_k,    dI/da_k = -2 a_k S Q_k,
+//   dI/dbeta_k = -2 a_k^2 S E_k,  dI/dalpha_k = I/alpha_k - 2 a_k^2 S R_k,
+//   dI/dlo_k = (dI/dbeta_k - dI/dalpha_k)/2,  dI/dhi_k = (dI/dbeta_k + dI/dalpha_k)/2.
+// The backward kernel walks the same tiles as the forward one and forms
+// these sums per point in FP32 (flushed to FP64 every 32 points).
+#include "tq_grid_tile.hpp"
+
+namespace tq {
+
+constexpr int kSweepMaxD = 8;
+
+struct SweepGeom {
+    GridGeom g;
+    long long blocks_x;  // blocks per integral
+};
+
+static int sweep_geom(const tq_grid_desc* gd, SweepGeom& S) {
+    int rc = plan_geom(gd->dim, gd->n, gd->rule, gd->lane_axes, gd->items_per_wave, 256, S.g);
+    if (rc) return rc;
+    S.blocks_x = (S.g.num_waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    return TQ_OK;
+}
+
+template <typename T>
+__device__ inline void load_sweep_params(int d, int n, int rule, long long b, const T* a, const T* u, const T* dom,
+                                         Params<T>* p, Axis* ax) {
+    const double sl2e = 1.2011224087864498;  // sqrt(log2 e), as in make_params
+    const int tid = threadIdx.x;
+    if (tid < TQ_MAX_DIM) {
+        if (tid < d) {
+            const double ak = (double)a[b * d + tid], uk = (double)u[b * d + tid];
+            p->a[tid] = (T)(ak * sl2e);
+            p->b[tid] = (T)(-ak * uk * sl2e);
+            p->u[tid] = (T)uk;
+            ax[tid] = make_axis(dom + b * 2 * d, tid, n, rule);
+        } else {
+            p->a[tid] = T(0);
+            p->b[tid] = T(0);
+            p->u[tid] = T(0);
+        }
+    }
+    if (tid == 0) {
+        p->c0 = T(1);
+        p->c1 = p->c2 = p->c3 = T(0);
+    }
+}
+
+template <typename T, bool TABLE>
+__global__ __launch_bounds__(kBlock) void k_sweep_fwd(SweepGeom S, const T* a, const T* u, const T* dom,
+                                                     const double* glx, const double* glw, double* partials) {
+    __shared__ Params<T> s_p;
+    __shared__ Axis s_axis[TQ_MAX_DIM];
+    __shared__ T s_tab[2 * kLoopTable];
+    __shared__ double s_red[kWavesPerBlock];
+    const GridGeom& g = S.g;
+    const long long b = blockIdx.y;
+    load_sweep_params(g.d, g.n, g.rule, b, a, u, dom, &s_p, s_axis);
+    __syncthreads();
+    if (TABLE) {
+        for (int j = threadIdx.x; j < g.n; j += kBlock) {
+            s_tab[2 * j] = (T)node_of(s_axis[g.L], j, g.n, g.rule, glx);
+            s_tab[2 * j + 1] = (T)weight_of(g.rule, j, g.n, glw);
+        }
+        __syncthreads();
+    }
+    const long long wave = (long long)blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double acc = 0.0;
+    if (wave < g.num_waves) {
+        const int lw = (int)(wave % g.lane_waves);
+        const long long chunk = wave / g.lane_waves;
+        const long long t0 = chunk * g.ipw;
+        const long long t1 = (t0 + g.ipw < g.items) ? (t0 + g.ipw) : g.items;
+        acc = grid_tile<T, TQ_FAM_GAUSSIAN_PEAK, TABLE>(g.d, g.n, g.m, g.L, g.rule, g.lane_count, s_p, s_axis, s_tab,
+                                                        glx, glw, lw, t0, t1);
+    }
+    const double bs = block_sum(acc, s_red);
+    if (threadIdx.x == 0) partials[b * gridDim.x + blockIdx.x] = bs;
+}
+
+// normalised node coordinate r in [-1, 1]
+__device__ inline double node_r(int j, int n, int rule, const double* glx) {
+    if (rule == TQ_RULE_GAUSS_LEGENDRE) return glx[j];
+    return (n > 1) ? 2.0 * (double)j / (double)(n - 1) - 1.0 : 0.0;
+}
+
+// Backward tile for D axes: flat walk of the tile's points (lane axes per
+// lane, the rest decoded per row), full per-point accumulation.
+template <typename T, int D>
+__device__ inline void sweep_bwd_tile(const GridGeom& g, const Params<T>& p, const Axis* ax, const T* tab,
+                                      const double* glx, const double* glw, int lw, long long t_begin,
+                                      long long t_end, double* accF, double* accE, double* accQ, double* accR) {
+    const int n = g.n, m = g.m, L = g.L, rule = g.rule;
+    const int lane = threadIdx.x & (kWave - 1);
+    // per-point coordinates of the current point, per axis
+    T xk[D], rk[D], wk[D];
+    bool active = true;
+    int li = 0;
+    if (m == 0) {
+        // single axis split: node 64 t + lane
+        for (long long t = t_begin; t < t_end; ++t) {
+            const long long k = t * kWave + lane;
+            if (k >= n) continue;
+            const int kk = (int)k;
+            const T x = (T)node_of(ax[0], kk, n, rule, glx);
+            const T w = (T)weight_of(rule, kk, n, glw);
+            const T r = (T)node_r(kk, n, rule, glx);
+            const T tt = x - p.u[0];
+            const T e = fma(x, p.a[0], p.b[0]);
+            const T f = tq_exp2(-e * e);
+            const T wf = w * f;
+            accF[0] += (double)wf;
+            accE[0] += (double)(wf * tt);
+            accQ[0] += (double)(wf * tt * tt);
+            accR[0] += (double)(wf * tt * r);
+        }
+        return;
+    }
+    li = lw * kWave + lane;
+    active = li < g.lane_count;
+    T w_lane = active ? T(1) : T(0);
+    {
+        int rem = active ? li : 0;
+#pragma unroll
+        for (int k = D - 1; k >= 0; --k) {
+            if (k >= D - m) {
+                const int j = rem % n;
+                rem /= n;
+                xk[k] = (T)node_of(ax[k], j, n, rule, glx);
+                rk[k] = (T)node_r(j, n, rule, glx);
+                wk[k] = (T)weight_of(rule, j, n, glw);
+                w_lane *= wk[k];
+            }
+        }
+    }
+    if (L < 0) {
+        // every axis is a lane axis: one point per lane
+        if (t_begin == 0 && t_end > 0) {
+            T s = T(0);
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const T e = fma(xk[k], p.a[k], p.b[k]);
+                s = fma(-e, e, s);
+            }
+            const T wf = w_lane * tq_exp2(s);
+            accF[0] += (double)wf;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const T tt = xk[k] - p.u[k];
+                accE[k] += (double)(wf * tt);
+                accQ[k] += (double)(wf * tt * tt);
+                accR[k] += (double)(wf * tt * rk[k]);
+            }
+        }
+        return;
+    }
+    long long t = t_begin;
+    long long row = t / n;
+    int j = (int)(t - row * n);
+    int rdig[D];
+    {
+        long long r = row;
+#pragma unroll
+        for (int k = D - 1; k >= 0; --k) {
+            rdig[k] = 0;
+            if (k < L) {
+                rdig[k] = (int)(r % n);
+                r /= n;
+            }
+        }
+    }
+    T aL = T(0), bL = T(0), uL = T(0);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        if (k == L) {
+            aL = p.a[k];
+            bL = p.b[k];
+            uL = p.u[k];
+            xk[k] = T(0);
+            rk[k] = T(0);
+        }
+    }
+    while (t < t_end) {
+        T w_row = w_lane;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            if (k < L) {
+                xk[k] = (T)node_of(ax[k], rdig[k], n, rule, glx);
+                rk[k] = (T)node_r(rdig[k], n, rule, glx);
+                w_row *= (T)weight_of(rule, rdig[k], n, glw);
+            }
+        }
+        // exponent and t of the fixed axes
+        T s_fix = T(0), tk[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            tk[k] = xk[k] - p.u[k];
+            if (k != L) {
+                const T e = fma(xk[k], p.a[k], p.b[k]);
+                s_fix = fma(-e, e, s_fix);
+            }
+        }
+        const long long left = t_end - t;
+        const int jend = (left < (long long)(n - j)) ? (j + (int)left) : n;
+        for (int jb = j; jb < jend; jb += kFlush) {
+            const int je = (jb + kFlush < jend) ? (jb + kFlush) : jend;
+            T fF = T(0), fE[D], fQ[D], fR[D];
+#pragma unroll
+            for (int k = 0; k < D; ++k) fE[k] = fQ[k] = fR[k] = T(0);
+            for (int jj = jb; jj < je; ++jj) {
+                T xL, wL;
+                if (tab) {
+                    xL = tab[2 * jj];
+                    wL = tab[2 * jj + 1];
+                } else {
+                    xL = (T)node_of(ax[L], jj, n, rule, glx);
+                    wL = (T)weight_of(rule, jj, n, glw);
+                }
+                const T rL = (T)node_r(jj, n, rule, glx);
+                const T tL = xL - uL;
+                const T e = fma(xL, aL, bL);
+                const T wf = wL * w_row * tq_exp2(fma(-e, e, s_fix));
+                fF += wf;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const bool isL = (k == L);
+                    const T tt = isL ? tL : tk[k];
+                    const T rr = isL ? rL : rk[k];
+                    const T wt = wf * tt;
+                    fE[k] += wt;
+                    fQ[k] = fma(wt, tt, fQ[k]);
+                    fR[k] = fma(wt, rr, fR[k]);
+                }
+            }
+            accF[0] += (double)fF;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                accE[k] += (double)fE[k];
+                accQ[k] += (double)fQ[k];
+                accR[k] += (double)fR[k];
+            }
+        }
+        t += jend - j;
+        j = jend;
+        if (j == n) {
+            j = 0;
+#pragma unroll
+            for (int k = D - 1; k >= 0; --k) {
+                if (k < L) {
+                    if (++rdig[k] < n) break;
+                    rdig[k] = 0;
+                }
+            }
+        }
+    }
+}
+
+// partial layout per (integral, block): [F, E_0..E_{D-1}, Q_0.., R_0..]
+template <typename T, int D>
+__global__ __launch_bounds__(kBlock) void k_sweep_bwd(SweepGeom S, const T* a, const T* u, const T* dom,
+                                                     const double* glx, const double* glw, double* partials) {
+    __shared__ Params<T> s_p;
+    __shared__ Axis s_axis[TQ_MAX_DIM];
+    __shared__ T s_tab[2 * kLoopTable];
+    __shared__ double s_red[kWavesPerBlock];
+    const GridGeom& g = S.g;
+    const long long b = blockIdx.y;
+    load_sweep_params(g.d, g.n, g.rule, b, a, u, dom, &s_p, s_axis);
+    __syncthreads();
+    const bool table = g.m > 0 && g.L >= 0 && g.n <= kLoopTable;
+    if (table) {
+        for (int j = threadIdx.x; j < g.n; j += kBlock) {
+            s_tab[2 * j] = (T)node_of(s_axis[g.L], j, g.n, g.rule, glx);
+            s_tab[2 * j + 1] = (T)weight_of(g.rule, j, g.n, glw);
+        }
+    }
+    __syncthreads();
+    double F[1] = {0.0}, E[D], Q[D], R[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) E[k] = Q[k] = R[k] = 0.0;
+    const long long wave = (long long)blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wave < g.num_waves) {
+        const int lw = (int)(wave % g.lane_waves);
+        const long long chunk = wave / g.lane_waves;
+        const long long t0 = chunk * g.ipw;
+        const long long t1 = (t0 + g.ipw < g.items) ? (t0 + g.ipw) : g.items;
+        sweep_bwd_tile<T, D>(g, s_p, s_axis, table ? s_tab : nullptr, glx, glw, lw, t0, t1, F, E, Q, R);
+    }
+    const int nv = 1 + 3 * D;
+    double* out = partials + (b * gridDim.x + blockIdx.x) * nv;
+    double v = block_sum(F[0], s_red);
+    if (threadIdx.x == 0) out[0] = v;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        v = block_sum(E[k], s_red);
+        if (threadIdx.x == 0) out[1 + k] = v;
+        v = block_sum(Q[k], s_red);
+        if (threadIdx.x == 0) out[1 + D + k] = v;
+        v = block_sum(R[k], s_red);
+        if (threadIdx.x == 0) out[1 + 2 * D + k] = v;
+    }
+}
+
+template <typename T>
+__device__ inline double sweep_scale(const T* dom, long long b, int d, int n, int rule) {
+    double s = 1.0;
+    for (int k = 0; k < d; ++k) s *= axis_scale((double)dom[(b * d + k) * 2], (double)dom[(b * d + k) * 2 + 1], n, rule);
+    return s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sweep_fwd_finalize(const double* partials, long long nbx, long long B,
+                                                              const T* dom, int d, int n, int rule, T* out,
+                                                              double* out64) {
+    const long long b = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (b >= B) return;
+    double v = 0.0;
+    for (long long j = lane; j < nbx; j += kWave) v += partials[b * nbx + j];
+    v = wave_sum(v);
+    if (lane == 0) {
+        const double r = v * sweep_scale(dom, b, d, n, rule);
+        if (out) out[b] = (T)r;
+        if (out64) out64[b] = r;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sweep_bwd_finalize(const double* partials, long long nbx, long long B,
+                                                              int D, const T* a, const T* dom, int n, int rule,
+                                                              const T* grad_out, T* ga, T* gu, T* gdom) {
+    const long long b = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (b >= B) return;
+    const int nv = 1 + 3 * D;
+    // lanes reduce one sum each (nv <= 25 < 64), blocks in order
+    double v = 0.0;
+    if (lane < nv)
+        for (long long j = 0; j < nbx; ++j) v += partials[(b * nbx + j) * nv + lane];
+    const double S = sweep_scale(dom, b, D, n, rule);
+    const double F = __shfl(v, 0, kWave);
+    const double I = S * F;
+    const double g = (double)grad_out[b];
+    for (int k = 0; k < D; ++k) {
+        const double E = __shfl(v, 1 + k, kWave);
+        const double Q = __shfl(v, 1 + D + k, kWave);
+        const double R = __shfl(v, 1 + 2 * D + k, kWave);
+        if (lane == 0) {
+            const double ak = (double)a[b * D + k];
+            const double lo = (double)dom[(b * D + k) * 2], hi = (double)dom[(b * D + k) * 2 + 1];
+            const double alpha = 0.5 * (hi - lo);
+            const double dbeta = -2.0 * ak * ak * S * E;
+            const double dalpha = I / alpha - 2.0 * ak * ak * S * R;
+            if (gu) gu[b * D + k] = (T)(g * 2.0 * ak * ak * S * E);
+            if (ga) ga[b * D + k] = (T)(g * -2.0 * ak * S * Q);
+            if (gdom) {
+                gdom[(b * D + k) * 2] = (T)(g * 0.5 * (dbeta - dalpha));
+                gdom[(b * D + k) * 2 + 1] = (T)(g * 0.5 * (dbeta + dalpha));
+            }
+        }
+    }
+}
+
+static int check_sweep(const tq_grid_desc* g, int64_t B) {
+    if (!g) return fail(TQ_E_NULL, "grid descriptor is NULL");
+    if (g->dtype != TQ_F32 && g->dtype != TQ_F64) return fail(TQ_E_ARG, "dtype must be TQ_F32 or TQ_F64");
+    if (g->dim < 1 || g->dim > kSweepMaxD) return fail(TQ_E_ARG, "sweep dim out of range (1..8)");
+    if (B < 1 || B > 65535) return fail(TQ_E_ARG, "sweep count out of range (1..65535)");
+    if (g->n < 1 || (g->rule != TQ_RULE_GAUSS_LEGENDRE && g->n < 2)) return fail(TQ_E_ARG, "n out of range");
+    if (g->rule == TQ_RULE_SIMPSON && g->n % 2 != 1) return fail(TQ_E_ARG, "Simpson needs odd n");
+    if (g->rule == TQ_RULE_BOOLE && (g->n - 1) % 4 != 0) return fail(TQ_E_ARG, "Boole needs n = 1 + 4k");
+    if (g->rule == TQ_RULE_GAUSS_LEGENDRE && (!g->gl_nodes || !g->gl_weights))
+        return fail(TQ_E_NULL, "Gauss-Legendre needs node/weight tables");
+    long long P;
+    if (ipow_checked(g->n, g->dim, 1LL << 62, &P) != 0) return fail(TQ_E_ARG, "n^dim too large");
+    return TQ_OK;
+}
+
+template <typename T>
+static int launch_sweep_fwd(const tq_grid_desc* gd, int64_t B, const void* a, const void* u, const void* dom,
+                            void* ws, size_t wsb, void* out, double* out64, hipStream_t s) {
+    SweepGeom S;
+    int rc = sweep_geom(gd, S);
+    if (rc) return rc;
+    if (!ws || wsb < (size_t)(S.blocks_x * B) * sizeof(double)) return fail(TQ_E_WORKSPACE, "sweep workspace too small");
+    if (S.blocks_x > 0x7fffffff) return fail(TQ_E_ARG, "too many blocks");
+    dim3 grid((unsigned)S.blocks_x, (unsigned)B);
+    if (S.g.use_table)
+        hipLaunchKernelGGL((k_sweep_fwd<T, true>), grid, dim3(kBlock), 0, s, S, (const T*)a, (const T*)u,
+                           (const T*)dom, gd->gl_nodes, gd->gl_weights, (double*)ws);
+    else
+        hipLaunchKernelGGL((k_sweep_fwd<T, false>), grid, dim3(kBlock), 0, s, S, (const T*)a, (const T*)u,
+                           (const T*)dom, gd->gl_nodes, gd->gl_weights, (double*)ws);
+    rc = check_launch("k_sweep_fwd");
+    if (rc) return rc;
+    const long long fb = (B + kWavesPerBlock - 1) / kWavesPerBlock;
+    hipLaunchKernelGGL((k_sweep_fwd_finalize<T>), dim3((unsigned)fb), dim3(kBlock), 0, s, (const double*)ws,
+                       S.blocks_x, (long long)B, (const T*)dom, gd->dim, gd->n, gd->rule, (T*)out, out64);
+    return check_launch("k_sweep_fwd_finalize");
+}
+
+template <typename T, int D>
+static int launch_bwd_d(const SweepGeom& S, int64_t B, const void* a, const void* u, const void* dom,
+                        const tq_grid_desc* gd, void* ws, hipStream_t s) {
+    dim3 grid((unsigned)S.blocks_x, (unsigned)B);
+    hipLaunchKernelGGL((k_sweep_bwd<T, D>), grid, dim3(kBlock), 0, s, S, (const T*)a, (const T*)u, (const T*)dom,
+                       gd->gl_nodes, gd->gl_weights, (double*)ws);
+    return check_launch("k_sweep_bwd");
+}
+
+template <typename T>
+static int launch_sweep_bwd(const tq_grid_desc* gd, int64_t B, const void* a, const void* u, const void* dom,
+                            const void* grad_out, void* ws, size_t wsb, void* ga, void* gu, void* gdom,
+                            hipStream_t s) {
+    SweepGeom S;
+    int rc = sweep_geom(gd, S);
+    if (rc) return rc;
+    const int D = gd->dim;
+    if (!ws || wsb < (size_t)(S.blocks_x * B * (1 + 3 * D)) * sizeof(double))
+        return fail(TQ_E_WORKSPACE, "sweep workspace too small");
+    switch (D) {
+        case 1: rc = launch_bwd_d<T, 1>(S, B, a, u, dom, gd, ws, s); break;
+        case 2: rc = launch_bwd_d<T, 2>(S, B, a, u, dom, gd, ws, s); break;
+        case 3: rc = launch_bwd_d<T, 3>(S, B, a, u, dom, gd, ws, s); break;
+        case 4: rc = launch_bwd_d<T, 4>(S, B, a, u, dom, gd, ws, s); break;
+        case 5: rc = launch_bwd_d<T, 5>(S, B, a, u, dom, gd, ws, s); break;
+        case 6: rc = launch_bwd_d<T, 6>(S, B, a, u, dom, gd, ws, s); break;
+        case 7: rc = launch_bwd_d<T, 7>(S, B, a, u, dom, gd, ws, s); break;
+        default: rc = launch_bwd_d<T, 8>(S, B, a, u, dom, gd, ws, s); break;
+    }
+    if (rc) return rc;
+    const long long fb = (B + kWavesPerBlock - 1) / kWavesPerBlock;
+    hipLaunchKernelGGL((k_sweep_bwd_finalize<T>), dim3((unsigned)fb), dim3(kBlock), 0, s, (const double*)ws,
+                       S.blocks_x, (long long)B, D, (const T*)a, (const T*)dom, gd->n, gd->rule, (const T*)grad_out,
+                       (T*)ga, (T*)gu, (T*)gdom);
+    return check_launch("k_sweep_bwd_finalize");
+}
+
+}  // namespace tq
+
+using namespace tq;
+
+extern "C" {
+
+size_t tq_sweep_workspace_size(const tq_grid_desc* g, int64_t B) {
+    if (check_sweep(g, B)) return 0;
+    SweepGeom S;
+    if (sweep_geom(g, S)) return 0;
+    return (size_t)(S.blocks_x * B * (1 + 3 * g->dim)) * sizeof(double);
+}
+
+int tq_sweep_gaussian_forward(const tq_grid_desc* g, int64_t B, const void* a, const void* u, const void* domains,
+                              void* workspace, size_t workspace_bytes, void* out, double* out64, void* stream) {
+    int rc = check_sweep(g, B);
+    if (rc) return rc;
+    if (!a || !u || !domains) return fail(TQ_E_NULL, "a/u/domains is NULL");
+    if (!out && !out64) return fail(TQ_E_NULL, "no output pointer");
+    if (g->dtype == TQ_F32)
+        return launch_sweep_fwd<float>(g, B, a, u, domains, workspace, workspace_bytes, out, out64, as_stream(stream));
+    return launch_sweep_fwd<double>(g, B, a, u, domains, workspace, workspace_bytes, out, out64, as_stream(stream));
+}
+
+int tq_sweep_gaussian_backward(const tq_grid_desc* g, int64_t B, const void* a, const void* u, const void* domains,
+                               const void* grad_out, void* workspace, size_t workspace_bytes, void* grad_a,
+                               void* grad_u, void* grad_domains, void* stream) {
+    int rc = check_sweep(g, B);
+    if (rc) return rc;
+    if (!a || !u || !domains || !grad_out) return fail(TQ_E_NULL, "a/u/domains/grad_out is NULL");
+    if (g->dtype == TQ_F32)
+        return launch_sweep_bwd<float>(g, B, a, u, domains, grad_out, workspace, workspace_bytes, grad_a, grad_u,
+                                       grad_domains, as_stream(stream));
+    return launch_sweep_bwd<double>(g, B, a, u, domains, grad_out, workspace, workspace_bytes, grad_a, grad_u,
+                                    grad_domains, as_stream(stream));
+}
+
+}  // extern "C"

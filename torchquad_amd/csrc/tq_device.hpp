@@ -128,6 +128,9 @@ struct Uniform<double> {
 // ---------------------------------------------------------------------------
 __device__ inline float tq_exp(float x) { return __expf(x); }
 __device__ inline double tq_exp(double x) { return exp(x); }
+// 2^x: one v_exp_f32 in FP32 (callers fold log2(e) into their constants)
+__device__ inline float tq_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ inline double tq_exp2(double x) { return exp2(x); }
 __device__ inline float tq_cos(float x) { return cosf(x); }
 __device__ inline double tq_cos(double x) { return cos(x); }
 __device__ inline float tq_sin(float x) { return sinf(x); }
@@ -172,7 +175,8 @@ __device__ inline AxisC<T> axis_c(const Params<T>& p, int k) {
 template <int FAM, typename T>
 struct Family;
 
-// exp(-sum a_k^2 (x_k-u_k)^2);  b_k = -a_k*u_k so that t_k = fma(x, a, b).
+// exp(-sum a_k^2 (x_k-u_k)^2) = 2^(-sum t_k^2) with t_k = fma(x, a', b'),
+// a' = a sqrt(log2 e), b' = -a u sqrt(log2 e) (Params prepared on the host).
 template <typename T>
 struct Family<TQ_FAM_GAUSSIAN_PEAK, T> {
     __device__ static inline State<T> start(const Params<T>&) { return {T(0), T(0)}; }
@@ -183,7 +187,7 @@ struct Family<TQ_FAM_GAUSSIAN_PEAK, T> {
         return s;
     }
     __device__ static inline State<T> merge(State<T> a, State<T> b) { return {a.s0 + b.s0, T(0)}; }
-    __device__ static inline T finish(State<T> s, const Params<T>&) { return tq_exp(s.s0); }
+    __device__ static inline T finish(State<T> s, const Params<T>&) { return tq_exp2(s.s0); }
 };
 
 // cos(2*pi*c1 + sum a_k x_k);  start carries 2*pi*c1 (in c2).
@@ -214,7 +218,7 @@ struct Family<TQ_FAM_PRODUCT_PEAK, T> {
     __device__ static inline T finish(State<T> s, const Params<T>&) { return tq_rcp(s.s0); }
 };
 
-// exp(c1 + sum a_k x_k)
+// exp(c1 + sum a_k x_k) = 2^(c1' + sum a'_k x_k), a' = a log2 e, c1' = c1 log2 e
 template <typename T>
 struct Family<TQ_FAM_EXP_LINEAR, T> {
     __device__ static inline State<T> start(const Params<T>& p) { return {p.c1, T(0)}; }
@@ -224,7 +228,7 @@ struct Family<TQ_FAM_EXP_LINEAR, T> {
         return s;
     }
     __device__ static inline State<T> merge(State<T> a, State<T> b) { return {a.s0 + b.s0, T(0)}; }
-    __device__ static inline T finish(State<T> s, const Params<T>&) { return tq_exp(s.s0); }
+    __device__ static inline T finish(State<T> s, const Params<T>&) { return tq_exp2(s.s0); }
 };
 
 // sum sin(a_k x_k + u_k)
@@ -240,7 +244,7 @@ struct Family<TQ_FAM_SIN_SUM, T> {
     __device__ static inline T finish(State<T> s, const Params<T>&) { return s.s0; }
 };
 
-// exp(-c3 sum (x_k-c1)^2) + exp(-c3 sum (x_k-c2)^2)
+// exp(-c3 sum (x_k-c1)^2) + exp(-c3 sum (x_k-c2)^2); Params c3 holds c3 log2 e
 template <typename T>
 struct Family<TQ_FAM_DOUBLE_GAUSSIAN, T> {
     __device__ static inline State<T> start(const Params<T>&) { return {T(0), T(0)}; }
@@ -255,7 +259,7 @@ struct Family<TQ_FAM_DOUBLE_GAUSSIAN, T> {
         return {a.s0 + b.s0, a.s1 + b.s1};
     }
     __device__ static inline T finish(State<T> s, const Params<T>& p) {
-        return tq_exp(-p.c3 * s.s0) + tq_exp(-p.c3 * s.s1);
+        return tq_exp2(-p.c3 * s.s0) + tq_exp2(-p.c3 * s.s1);
     }
 };
 

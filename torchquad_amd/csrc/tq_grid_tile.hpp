@@ -276,7 +276,7 @@ int plan_geom(int d, int n, int rule, int lane_axes, long long items_per_wave, l
     if (ipw <= 0) {
         const long long work = items * (long long)G.lane_waves;
         ipw = (work + target_waves - 1) / target_waves;
-        const long long min_ipw = items < 64 ? items : 64;
+        const long long min_ipw = items < 16 ? items : 16;
         if (ipw < min_ipw) ipw = min_ipw;
     }
     if (ipw < 1) ipw = 1;

@@ -30,8 +30,11 @@ inline int check_launch(const char* what) {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
-// Family-specific derived per-axis constants (Params::b, c2) computed in
-// double precision before rounding to T.
+constexpr double kLog2e = 1.4426950408889634074;
+
+// Family-specific derived constants computed in double precision before
+// rounding to T: Params::b, 2 pi c1 for the oscillatory family, and log2(e)
+// folded into the exponential families so that their finish is one 2^x.
 template <typename T>
 inline Params<T> make_params(const tq_integrand& f) {
     Params<T> p{};
@@ -39,18 +42,25 @@ inline Params<T> make_params(const tq_integrand& f) {
     p.c1 = (T)f.c1;
     p.c2 = (T)f.c2;
     p.c3 = (T)f.c3;
+    const double sl2e = 1.2011224087864498;  // sqrt(log2 e)
     for (int k = 0; k < TQ_MAX_DIM; ++k) {
-        p.a[k] = (T)f.a[k];
-        p.u[k] = (T)f.u[k];
-        double b = 0.0;
+        double a = f.a[k], b = 0.0;
         switch (f.family) {
-            case TQ_FAM_GAUSSIAN_PEAK: b = -f.a[k] * f.u[k]; break;
+            case TQ_FAM_GAUSSIAN_PEAK:
+                b = -f.a[k] * f.u[k] * sl2e;
+                a = f.a[k] * sl2e;
+                break;
             case TQ_FAM_PRODUCT_PEAK: b = (f.a[k] != 0.0) ? 1.0 / (f.a[k] * f.a[k]) : 0.0; break;
+            case TQ_FAM_EXP_LINEAR: a = f.a[k] * kLog2e; break;
             default: b = 0.0;
         }
+        p.a[k] = (T)a;
+        p.u[k] = (T)f.u[k];
         p.b[k] = (T)b;
     }
     if (f.family == TQ_FAM_OSCILLATORY) p.c2 = (T)(2.0 * M_PI * f.c1);
+    if (f.family == TQ_FAM_EXP_LINEAR) p.c1 = (T)(f.c1 * kLog2e);
+    if (f.family == TQ_FAM_DOUBLE_GAUSSIAN) p.c3 = (T)(f.c3 * kLog2e);
     return p;
 }
 
