@@ -1,12 +1,16 @@
 // Gauss-Legendre nodes and weights on [-1, 1] for any n, on the device.
 //
-// Near the ends (n sin(theta) < 50) and for n < 200: Newton iteration on
-// P_n from the three-term (Bonnet) recurrence
+// Near the ends (n sin(theta) < 50) and for n < 200: Newton iteration in
+// theta (x = cos theta) on P_n from the three-term (Bonnet) recurrence
 //     j P_j = (2j-1) x P_{j-1} - (j-1) P_{j-2},
-// started from Tricomi's estimate
+// run on z = 1 - x = 2 sin^2(theta/2) and the differences D_j = P_j - P_{j-1},
+//     D_j = ((j-1) D_{j-1} - (2j-1) z P_{j-1}) / j,
+// so that nodes next to x = 1, where x itself rounds to a few ulp, still get
+// their angle and weight to full relative precision.  Started from
+// Tricomi's estimate
 //     x_k ~ (1 - 1/(8 n^2) + 1/(8 n^3)) cos(pi (k - 1/4) / (n + 1/2)),
-// with P_n'(x) = n (x P_n - P_{n-1}) / (x^2 - 1) and weights
-//     w_k = 2 / ((1 - x_k^2) P_n'(x_k)^2).
+// with dP_n/dtheta = n (x P_n - P_{n-1}) / sin(theta) = n (D_n - z P_n) / sin(theta)
+// and weights w_k = 2 / (1 - x_k^2) / P_n'(x_k)^2 = 2 / (dP_n/dtheta)^2.
 // Interior nodes of large n: Newton in theta on the Stieltjes asymptotic
 // expansion used by Hale & Townsend (SIAM J. Sci. Comput. 35(2), 2013)
 //     P_n(cos t) = C_n sum_m h_m cos(a_m) / (2 sin t)^(m+1/2),
@@ -21,15 +25,17 @@
 
 namespace tq {
 
-__device__ inline void legendre_pair(int n, double x, double* pn, double* pn1) {
-    double p0 = 1.0, p1 = x;
+// P_n(cos t) and dP_n/dt from the difference form of the recurrence.
+__device__ inline void legendre_theta(int n, double t, double* pn, double* dpdt) {
+    const double sh = sin(0.5 * t);
+    const double z = 2.0 * sh * sh;
+    double p = 1.0 - z, d = -z;  // P_1, D_1 = P_1 - P_0
     for (int j = 2; j <= n; ++j) {
-        const double p2 = ((2.0 * j - 1.0) * x * p1 - (j - 1.0) * p0) / (double)j;
-        p0 = p1;
-        p1 = p2;
+        d = ((j - 1.0) * d - (2.0 * j - 1.0) * z * p) / (double)j;
+        p += d;
     }
-    *pn = p1;
-    *pn1 = p0;
+    *pn = p;
+    *dpdt = (double)n * (d - z * p) / sin(t);
 }
 
 constexpr int kAsymTerms = 14;
@@ -89,18 +95,19 @@ __global__ __launch_bounds__(kBlock) void k_gl_nodes(int n, double cn, double* x
         ws[k] = w;
         return;
     }
-    double x = (1.0 - 1.0 / (8.0 * nn * nn) + 1.0 / (8.0 * nn * nn * nn)) * cos(t0);
-    double pn, pn1, dp;
+    // Tricomi's x estimate moved to theta to first order
+    const double eps = 1.0 / (8.0 * nn * nn) - 1.0 / (8.0 * nn * nn * nn);
+    double t = (n == 1) ? 0.5 * M_PI : t0 + eps * cos(t0) / sin(t0);
+    double pn, dp;
     for (int it = 0; it < 16; ++it) {
-        legendre_pair(n, x, &pn, &pn1);
-        dp = nn * (x * pn - pn1) / (x * x - 1.0);
-        const double dx = pn / dp;
-        x -= dx;
-        if (fabs(dx) <= 1e-16) break;
+        legendre_theta(n, t, &pn, &dp);
+        const double dt = pn / dp;
+        t -= dt;
+        if (fabs(dt) <= 2e-16 * t) break;
     }
-    legendre_pair(n, x, &pn, &pn1);
-    dp = nn * (x * pn - pn1) / (x * x - 1.0);
-    const double w = 2.0 / ((1.0 - x * x) * dp * dp);
+    legendre_theta(n, t, &pn, &dp);
+    double x = cos(t);
+    const double w = 2.0 / (dp * dp);
     if ((n & 1) && k == half - 1) x = 0.0;
     xs[n - 1 - k] = x;
     ws[n - 1 - k] = w;
