@@ -7,8 +7,15 @@ seed 0).  The metric is "integrand evals/s at 1e-6 relative error", so the
 timed step is one Boole().integrate call at n=65 points per axis (65^5 =
 1.16e9 evals), the smallest Boole-valid n whose result is within 1e-6 of the
 analytic value for these parameters; n=17 (the literal 17^5 grid) is
-reported as per-call latency p50/p99.  Monte Carlo dim=8 with 1e8 samples
-(oscillatory integrand) is reported beside it.
+reported as per-call latency p50/p99.  Reported beside it:
+  * Monte Carlo dim=8, 1e8 samples, oscillatory integrand;
+  * VEGAS dim=6, 50 bins/axis, 20 iterations x 1e6 evals, double Gaussian;
+  * batched form: 8192 seeded Genz integrals (dim 1..6, N 1e3..1e6,
+    trapezoid/simpson/gauss-legendre) in one graph-replayed persistent
+    kernel, against a Python loop of plain per-integral calls;
+  * differentiation: 1024 Gauss-Legendre dim=4 n=32 Gaussian-peak integrals
+    with the hand-written backward, against a loop of per-vector autograd
+    calls through the materialised-points path.
 
 Prints one JSON line.  --gpus N>1 exits: this build measures one GPU.
 """
@@ -28,9 +35,23 @@ sys.path.insert(0, str(ROOT))
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (packed counted)
 PEAK_HBM_GBS = 8000.0
-# FLOPs per evaluation executed by the fused grid kernel for the Gaussian peak
-# (loop-axis fold: 2 FMA; exp: 1 mul + 1 exp; weighted accumulate: 1 FMA).
-FLOPS_PER_EVAL_GRID_GAUSSIAN = 8
+# Algorithmic FLOPs per evaluation of the dim-5 Gaussian peak
+# exp(-sum_i a_i^2 (x_i - u_i)^2) with its quadrature weight: 4d+3 arithmetic
+# (+, -, x = 1, FMA = 2) plus 1 exp = 24.  The fused kernel executes fewer
+# per point because row and lane partial sums are shared across points.
+DIM_C2A = 5
+FLOPS_PER_EVAL_GRID_GAUSSIAN = 4 * DIM_C2A + 3 + 1
+COPY_BYTES = 256 << 20
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_grid_fused_boole_d5_n65.json"
+
+
+def load_traffic():
+    """HBM bytes per launch of the timed kernel from the committed counter summary, or None."""
+    try:
+        t = json.loads(TRAFFIC_FILE.read_text())
+        return t["traffic_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def cpu_model():
@@ -43,6 +64,165 @@ def cpu_model():
     return platform.processor()
 
 
+def _events():
+    import torch
+
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def bench_vegas(c3, dev):
+    """VEGAS dim 6, 50 bins/axis, 20 iterations of about 1e6 evals."""
+    import torch
+
+    import torchquad_amd as tq
+
+    f = tq.DoubleGaussian(6, width=c3["width"])
+    dom = torch.tensor(c3["domain"], dtype=torch.float32, device=dev)
+    N = c3["N_increment"] * (c3["iterations"] + 5)
+    v = tq.VEGAS()
+    v.integrate(f, 6, N=N // 10, integration_domain=dom, seed=1, N_intervals=c3["N_intervals"])
+    torch.cuda.synchronize()
+    times, vals = [], []
+    for seed in range(3):
+        t0 = time.perf_counter()
+        val = float(v.integrate(f, 6, N=N, integration_domain=dom, seed=seed, N_intervals=c3["N_intervals"],
+                                max_iterations=c3["iterations"]))
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        vals.append(val)
+    t = statistics.median(times)
+    return {
+        "evals_per_s": v._nr_of_fevals / t,
+        "ms_per_integrate": t * 1e3,
+        "evals": int(v._nr_of_fevals),
+        "iterations": int(v.it),
+        "value": vals[-1],
+        "analytic": c3["analytic"],
+        "std_error": float(v._get_error()),
+    }
+
+
+def bench_batched(dev, steps):
+    """8192 Genz integrals: graph-replayed batch vs a loop of plain calls."""
+    import torch
+
+    import torchquad_amd as tq
+    from torchquad_amd.autotune import autotune
+    from torchquad_amd.batch import Batch
+    from torchquad_amd.workloads import genz_batch
+
+    items = genz_batch(8192, seed=0)
+    b = Batch(items, dtype=torch.float32, device=dev)
+    (tile, blocks), _ = autotune(b, reps=3, use_cache=False)
+    g = b.graph()
+    for _ in range(3):
+        g.replay()
+    torch.cuda.synchronize()
+    e0, e1 = _events()
+    reps = max(10, steps)
+    e0.record()
+    for _ in range(reps):
+        g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_batch = e0.elapsed_time(e1) / reps
+
+    integrators = {"trapezoid": tq.Trapezoid(), "simpson": tq.Simpson(), "gauss_legendre": tq.GaussLegendre()}
+    doms = {d: torch.tensor([[0.0, 1.0]] * d, dtype=torch.float32, device=dev) for d in range(1, 7)}
+
+    def plain_loop(sub):
+        for it in sub:
+            integrators[it["rule"]].integrate(it["fn"], it["dim"], N=it["N"], integration_domain=doms[it["dim"]])
+
+    plain_loop(items)  # warm pass: per-n Gauss-Legendre tables cached, as the batch plan holds them
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_loop(items)
+    torch.cuda.synchronize()
+    ms_plain = (time.perf_counter() - t0) * 1e3
+    return {
+        "integrals": len(items),
+        "evals": int(b.evals),
+        "evals_per_s": b.evals / (ms_batch / 1e3),
+        "ms_per_batch": ms_batch,
+        "tile_points": tile,
+        "blocks": blocks,
+        "plain_loop_ms": ms_plain,
+        "plain_loop_evals_per_s": b.evals / (ms_plain / 1e3),
+        "speedup_vs_plain_loop": ms_plain / ms_batch,
+    }
+
+
+def bench_grad(c5, dev, steps):
+    """1024 Gaussian-peak integrals (GL dim 4, n=32) forward + backward."""
+    import numpy as np
+    import torch
+
+    from torchquad_amd.autograd import gaussian_peak_integrals
+    import torchquad_amd as tq
+
+    rng = np.random.default_rng(c5["seed"])
+    B = c5["count"]
+    a = rng.uniform(1.0, 5.0, (B, 4))
+    u = rng.uniform(-0.5, 0.5, (B, 4))
+    lo = rng.uniform(-1.0, 0.0, (B, 4))
+    dom = np.stack([lo, lo + 1.0], axis=2)
+    ta = torch.tensor(a, dtype=torch.float32, device=dev, requires_grad=True)
+    tu = torch.tensor(u, dtype=torch.float32, device=dev, requires_grad=True)
+    td = torch.tensor(dom, dtype=torch.float32, device=dev, requires_grad=True)
+
+    def step():
+        out = gaussian_peak_integrals(ta, tu, td, n=c5["n"])
+        out.sum().backward()
+        return out
+
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = _events()
+    reps = max(10, steps)
+    e0.record()
+    for _ in range(reps):
+        out = step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    evals = B * c5["n"] ** 4
+
+    # a reference user's loop: one GaussLegendre call per vector, autograd
+    # through the points and the torch integrand
+    gl = tq.GaussLegendre()
+    sub = 16
+
+    def one(i):
+        ai = ta[i].detach().clone().requires_grad_(True)
+        ui = tu[i].detach().clone().requires_grad_(True)
+        di = td[i].detach().clone().requires_grad_(True)
+        v = gl.integrate(lambda x: torch.exp(-torch.sum((ai * (x - ui)) ** 2, dim=1)), 4, N=c5["n"] ** 4,
+                         integration_domain=di)
+        v.backward()
+
+    for i in range(2):
+        one(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(sub):
+        one(i)
+    torch.cuda.synchronize()
+    ms_loop = (time.perf_counter() - t0) * 1e3 * B / sub
+    return {
+        "vectors": B,
+        "evals_per_pass": evals,
+        "ms_forward_backward": ms,
+        "evals_per_s_forward_backward": 2 * evals / (ms / 1e3),
+        "first_value": float(out[0]),
+        "first_analytic": c5["first_analytic"],
+        "per_vector_autograd_loop_ms": ms_loop,
+        "per_vector_loop_sample": f"{sub} of {B} vectors timed, scaled to {B}",
+        "speedup_vs_autograd_loop": ms_loop / ms,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +232,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mc", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the VEGAS, batched and gradient lines")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run only a 256 MiB elementwise copy (counter calibration) and the timed step, "
+                    "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -79,6 +263,14 @@ def main():
     N = n**5
     stream = torch.cuda.current_stream(dev)
 
+    if args.traffic_probe:
+        # plain copy of known size: reads and writes COPY_BYTES each
+        src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+
     for _ in range(args.warmup):
         boole.integrate(f, 5, N=N, integration_domain=dom)
     torch.cuda.synchronize()
@@ -93,6 +285,9 @@ def main():
     ms_per_step = ms_total / args.steps
     evals_per_s = N / (ms_per_step / 1e3)
     val = float(val)
+    if args.traffic_probe:
+        print(json.dumps({"traffic_probe": True, "ms_per_step": ms_per_step, "copy_bytes": COPY_BYTES}))
+        return
     rel_err = abs(val - c2a["analytic"]) / abs(c2a["analytic"])
     oracle_same_rule = og.separable_gaussian_peak("boole", c2a["a"], c2a["u"], c2a["domain"], n)
     rel_vs_oracle = abs(val - oracle_same_rule) / abs(oracle_same_rule)
@@ -138,6 +333,12 @@ def main():
             "std_error": se,
         }
 
+    extra = {}
+    if not args.no_extra:
+        extra["vegas_d6"] = bench_vegas(cfg["C3"], dev)
+        extra["batched_c4"] = bench_batched(dev, args.steps)
+        extra["grad_c5"] = bench_grad(cfg["C5"], dev, args.steps)
+
     achieved_tflops = FLOPS_PER_EVAL_GRID_GAUSSIAN * evals_per_s / 1e12
     roofline = {
         "bound": "valu",
@@ -147,7 +348,8 @@ def main():
         "unit": "TFLOP/s",
         "frac": achieved_tflops / PEAK_FP32_TFLOPS,
         "flops_per_eval": FLOPS_PER_EVAL_GRID_GAUSSIAN,
-        "traffic": None,
+        "flops_per_eval_note": "4d+3 arithmetic + 1 exp at d=5 (algorithmic count)",
+        "traffic": load_traffic(),
     }
 
     cpu = None
@@ -196,6 +398,7 @@ def main():
             "n17_latency_ms_p99": p99,
             "n17_value": float(r17),
             "monte_carlo_d8_1e8": mc_info,
+            **extra,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -4,6 +4,7 @@ import re
 import warnings
 from pathlib import Path
 
+import numpy as np
 import pytest
 import torch
 
@@ -144,3 +145,72 @@ def test_builtin_torch_definitions_match_oracle():
     ]
     for f, ref in pairs:
         np.testing.assert_allclose(f(xt).numpy(), ref, rtol=1e-12, atol=1e-14)
+
+
+def test_forms_without_gpu_fail_loudly():
+    from torchquad_amd.autograd import gaussian_peak_integrals
+    from torchquad_amd.batch import Batch
+    from torchquad_amd.serve import IntegrationServer
+
+    if torch.cuda.is_available():
+        pytest.skip("checks the no-GPU behaviour")
+    with pytest.raises(_lib.NativeUnavailableError):
+        IntegrationServer()
+    with pytest.raises(_lib.NativeUnavailableError):
+        Batch([])
+    a = torch.ones(2, 2, dtype=torch.float64)
+    dom = torch.tensor([[[0.0, 1.0]] * 2] * 2, dtype=torch.float64)
+    with pytest.raises(_lib.NativeUnavailableError):
+        gaussian_peak_integrals(a, a, dom, n=4)
+
+
+def test_batch_plan_host_side():
+    """tq_batch_plan is host code: tiles cover every integral exactly once, longest first."""
+    from torchquad_amd.batch import n_per_axis
+
+    items = (_lib.BatchItem * 3)()
+    spec = [(0, 2, 33), (1, 3, 21), (3, 1, 100000)]
+    for it, (rule, dim, n) in zip(items, spec):
+        it.rule, it.family, it.dim, it.n = rule, _lib.FAM_GAUSSIAN_PEAK, dim, n
+        for k in range(dim):
+            it.domain[2 * k], it.domain[2 * k + 1] = 0.0, 1.0
+            it.a[k], it.u[k] = 1.0, 0.5
+        it.c0 = 1.0
+    L = _lib.load_library()
+    db, nt = ctypes.c_int64(), ctypes.c_int64()
+    assert L.tq_batch_plan_sizes(ctypes.cast(items, ctypes.c_void_p), 3, _lib.TQ_F32, 4096, ctypes.byref(db),
+                                 ctypes.byref(nt)) == 0
+    import numpy as np
+
+    desc = np.zeros(db.value, dtype=np.uint8)
+    tiles = (_lib.BatchTile * nt.value)()
+    tb = np.zeros(3, dtype=np.int64)
+    tc = np.zeros(3, dtype=np.int64)
+    assert L.tq_batch_plan(ctypes.cast(items, ctypes.c_void_p), 3, _lib.TQ_F32, 4096,
+                           desc.ctypes.data_as(ctypes.c_void_p), tiles, tb.ctypes.data_as(ctypes.c_void_p),
+                           tc.ctypes.data_as(ctypes.c_void_p)) == 0
+    assert tc.sum() == nt.value
+    # LPT: the 1e5-point GL integral first, then 21^3 = 9261, then 33^2 = 1089
+    assert list(np.argsort(tb)) == [2, 1, 0]
+    for i in range(3):
+        ts = [tiles[t] for t in range(tb[i], tb[i] + tc[i])]
+        assert all(t.item == i for t in ts)
+    assert n_per_axis(_lib.RULE_SIMPSON, 3, 10_000) == 21
+    assert n_per_axis(_lib.RULE_BOOLE, 5, 17**5) == 17
+
+
+def test_serving_helpers():
+    from torchquad_amd.serve import Ticket, latency_stats, poisson_arrivals
+
+    offs = poisson_arrivals(2000.0, 20000, seed=0)
+    assert np.all(np.diff(offs) > 0)
+    assert abs(len(offs) / offs[-1] - 2000.0) < 60.0  # mean rate within 3 %
+    ts = []
+    for k in range(100):
+        t = Ticket("grid", 0, 1)
+        t.t_arrival = 0.0
+        t._finish(value=1.0)
+        t.t_done = (k + 1) * 1e-3
+        ts.append(t)
+    st = latency_stats(ts)
+    assert st["count"] == 100 and st["p50_ms"] == pytest.approx(51.0) and st["p99_ms"] == pytest.approx(100.0)

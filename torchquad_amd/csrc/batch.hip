@@ -36,13 +36,32 @@ __global__ void k_batch_reset(unsigned int* counter) {
     if (threadIdx.x == 0) *counter = 0u;
 }
 
-template <typename T, bool TABLE>
-__device__ inline double batch_tile(const BatchDev<T>& D, const Axis* ax, const T* tab, const double* glx,
+// Stages the integral's loop-axis table in the wave's LDS slice (when used)
+// and runs the tile.
+template <typename T, int FAM>
+__device__ inline double batch_tile_fam(const BatchDev<T>& D, const Axis* ax, T* tab, bool table, const double* glx,
+                                        const double* glw, int lw, long long t0, long long t1) {
+    if (table) {
+        const int lane = threadIdx.x & (kWave - 1);
+        const AxisC<T> cL = axis_c(D.p, D.L);
+        const Axis axL = D.ax[D.L];
+        for (int j = lane; j < D.n; j += kWave) table_entry<T, FAM>(tab, j, axL, cL, D.n, D.rule, glx, glw, D.p);
+    }
+    // make this wave's LDS writes visible to all of its lanes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    return table ? grid_tile<T, FAM, true>(D.d, D.n, D.m, D.L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw, t0, t1)
+                 : grid_tile<T, FAM, false>(D.d, D.n, D.m, D.L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw, t0,
+                                            t1);
+}
+
+template <typename T>
+__device__ inline double batch_tile(const BatchDev<T>& D, const Axis* ax, T* tab, bool table, const double* glx,
                                     const double* glw, int lw, long long t0, long long t1) {
     switch (D.fam) {
-#define TQ_BT(F)                                                                                                   \
-    case F:                                                                                                        \
-        return grid_tile<T, F, TABLE>(D.d, D.n, D.m, D.L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw, t0, t1);
+#define TQ_BT(F) \
+    case F: return batch_tile_fam<T, F>(D, ax, tab, table, glx, glw, lw, t0, t1);
         TQ_BT(TQ_FAM_GAUSSIAN_PEAK)
         TQ_BT(TQ_FAM_OSCILLATORY)
         TQ_BT(TQ_FAM_PRODUCT_PEAK)
@@ -74,20 +93,8 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchDev<T>* descs, cons
         const double* gx = glx ? glx + D.gl_offset : nullptr;
         const double* gw = glw ? glw + D.gl_offset : nullptr;
         for (int k = lane; k < D.d; k += kWave) s_ax[wave][k] = D.ax[k];
-        const bool table = D.m > 0 && D.L >= 0 && D.n <= NT;
-        if (table) {
-            const Axis axL = D.ax[D.L];
-            for (int j = lane; j < D.n; j += kWave) {
-                s_tab[wave][2 * j] = (T)node_of(axL, j, D.n, D.rule, gx);
-                s_tab[wave][2 * j + 1] = (T)weight_of(D.rule, j, D.n, gw);
-            }
-        }
-        // make this wave's LDS writes visible to all of its lanes
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        double acc = table ? batch_tile<T, true>(D, s_ax[wave], s_tab[wave], gx, gw, tile.lane_wave, tile.t0, tile.t1)
-                           : batch_tile<T, false>(D, s_ax[wave], s_tab[wave], gx, gw, tile.lane_wave, tile.t0, tile.t1);
+        const bool table = D.m > 0 && D.L >= 0 && D.n * tab_stride(D.fam) <= 2 * NT;
+        double acc = batch_tile<T>(D, s_ax[wave], s_tab[wave], table, gx, gw, tile.lane_wave, tile.t0, tile.t1);
         acc = wave_sum(acc);
         if (lane == 0) tile_part[t] = acc;
         // all lanes are done with this tile's LDS before the next one writes it

@@ -99,11 +99,14 @@ __global__ __launch_bounds__(kBlock) void k_gl_nodes(int n, double cn, double* x
     const double eps = 1.0 / (8.0 * nn * nn) - 1.0 / (8.0 * nn * nn * nn);
     double t = (n == 1) ? 0.5 * M_PI : t0 + eps * cos(t0) / sin(t0);
     double pn, dp;
+    // quadratic convergence: once a step is below 1e-11 t the next error is
+    // below rounding, and the recurrence's own rounding keeps steps from
+    // shrinking to an ulp, so stop there
     for (int it = 0; it < 16; ++it) {
         legendre_theta(n, t, &pn, &dp);
         const double dt = pn / dp;
         t -= dt;
-        if (fabs(dt) <= 2e-16 * t) break;
+        if (fabs(dt) <= 1e-11 * t) break;
     }
     legendre_theta(n, t, &pn, &dp);
     double x = cos(t);

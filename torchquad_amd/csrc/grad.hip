@@ -68,10 +68,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_fwd(SweepGeom S, const T* a, c
     load_sweep_params(g.d, g.n, g.rule, b, a, u, dom, &s_p, s_axis);
     __syncthreads();
     if (TABLE) {
-        for (int j = threadIdx.x; j < g.n; j += kBlock) {
-            s_tab[2 * j] = (T)node_of(s_axis[g.L], j, g.n, g.rule, glx);
-            s_tab[2 * j + 1] = (T)weight_of(g.rule, j, g.n, glw);
-        }
+        const AxisC<T> cL = axis_c(s_p, g.L);
+        for (int j = threadIdx.x; j < g.n; j += kBlock)
+            table_entry<T, TQ_FAM_GAUSSIAN_PEAK>(s_tab, j, s_axis[g.L], cL, g.n, g.rule, glx, glw, s_p);
         __syncthreads();
     }
     const long long wave = (long long)blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);

@@ -81,10 +81,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_fused(GridArgs<T> A) {
     __syncthreads();
     T* s_tab = reinterpret_cast<T*>(smem_raw);
     if (TABLE) {
-        for (int j = tid; j < n; j += kBlock) {
-            s_tab[2 * j] = (T)node_of(s_axis[L], j, n, rule, A.glx);
-            s_tab[2 * j + 1] = (T)weight_of(rule, j, n, A.glw);
-        }
+        const AxisC<T> cL = axis_c(A.p, L);
+        for (int j = tid; j < n; j += kBlock) table_entry<T, FAM>(s_tab, j, s_axis[L], cL, n, rule, A.glx, A.glw, A.p);
         __syncthreads();
     }
     // readfirstlane: the wave index is uniform, so the tile loop runs on SGPRs
@@ -135,8 +133,8 @@ static int launch_grid(const tq_grid_desc* gd, const tq_integrand* f, const void
     A.glw = gd->gl_weights;
     A.partials = (double*)ws;
     A.p = make_params<T>(*f);
-    const size_t shmem = G.use_table ? (size_t)2 * G.n * sizeof(T) : 0;
     TQ_DISPATCH_FAMILY(f->family, {
+        const size_t shmem = (size_t)TabStride<FAM>::v * G.n * sizeof(T);
         if (G.use_table)
             hipLaunchKernelGGL((k_grid_fused<T, FAM, true>), dim3((unsigned)G.blocks), dim3(kBlock), shmem, st, A);
         else

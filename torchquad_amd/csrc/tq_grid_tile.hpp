@@ -73,12 +73,34 @@ __device__ __host__ inline double axis_scale(double a, double b, int n, int rule
     }
 }
 
+// Loop-axis LDS table: entry j holds the loop axis's contribution to the
+// integrand state at node j, fold_c(zero, c_L, x_j), then the node's weight;
+// per point the tile loop only merges it into the row/lane state.  Families
+// with a two-component state take three slots per entry.
+template <int FAM>
+struct TabStride {
+    static constexpr int v = (FAM == TQ_FAM_DOUBLE_GAUSSIAN) ? 3 : 2;
+};
+
+__host__ __device__ inline int tab_stride(int fam) { return fam == TQ_FAM_DOUBLE_GAUSSIAN ? 3 : 2; }
+
+template <typename T, int FAM>
+__device__ inline void table_entry(T* tab, int j, const Axis& axL, const AxisC<T>& c, int n, int rule,
+                                   const double* glx, const double* glw, const Params<T>& p) {
+    using F = Family<FAM, T>;
+    constexpr int S = TabStride<FAM>::v;
+    const State<T> s = F::fold_c(F::zero(), c, (T)node_of(axL, j, n, rule, glx), p);
+    tab[S * j] = s.s0;
+    if (S == 3) tab[S * j + 1] = s.s1;
+    tab[S * j + S - 1] = (T)weight_of(rule, j, n, glw);
+}
+
 // One tile of the tensor-product index space.  m == 0 (one axis): node
 // 64 t + lane for loop items t in [t_begin, t_end).  Otherwise the lanes of the calling wave
 // take lane combinations lw*64 .. lw*64+63 of the last m axes, the wave walks
 // loop items [t_begin, t_end) (row axes [0, L) x loop axis L).  Returns the
 // lane's weighted sum.  `ax` holds the per-axis maps, `tab` the loop-axis
-// (node, weight) pairs when TABLE.
+// table (table_entry) when TABLE.
 template <typename T, int FAM, bool TABLE>
 __device__ inline double grid_tile(int d, int n, int m, int L, int rule, int lane_count, const Params<T>& p,
                                    const Axis* ax, const T* tab, const double* glx, const double* glw, int lw,
@@ -165,15 +187,19 @@ __device__ inline double grid_tile(int d, int n, int m, int L, int rule, int lan
             T fa = T(0);
 #pragma unroll 4
             for (int jj = jb; jj < je; ++jj) {
-                T xL, wL;
+                T f, wL;
                 if (TABLE) {
-                    xL = tab[2 * jj];
-                    wL = tab[2 * jj + 1];
+                    constexpr int S = TabStride<FAM>::v;
+                    State<T> term;
+                    term.s0 = tab[S * jj];
+                    term.s1 = (S == 3) ? tab[S * jj + 1] : T(0);
+                    wL = tab[S * jj + S - 1];
+                    f = F::finish(F::merge(s_rl, term), p);
                 } else {
-                    xL = (T)node_of(ax[L], jj, n, rule, glx);
+                    const T xL = (T)node_of(ax[L], jj, n, rule, glx);
                     wL = (T)weight_of(rule, jj, n, glw);
+                    f = F::finish(F::fold_c(s_rl, acL, xL, p), p);
                 }
-                const T f = F::finish(F::fold_c(s_rl, acL, xL, p), p);
                 fa = fma(wL, f, fa);
             }
             racc += (double)fa;

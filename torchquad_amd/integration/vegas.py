@@ -62,6 +62,15 @@ class VEGAS(BaseIntegrator):
         Returns:
             tensor: integral value
         """
+        self._setup(fn, dim, N, integration_domain, seed, rng, use_grid_improve, eps_rel, eps_abs, max_iterations,
+                    backend, N_intervals)
+        for _ in self._steps(use_warmup):
+            pass
+        return self._finish()
+
+    def _setup(self, fn, dim, N, integration_domain, seed, rng, use_grid_improve, eps_rel, eps_abs, max_iterations,
+               backend, N_intervals):
+        """Checks, budget split, RNG and resident map/stratification state (vegas.py:30-128)."""
         self._check_inputs(dim=dim, N=N, integration_domain=integration_domain)
         logger.debug("VEGAS integrating a %d-dimensional fn with %d points", dim, N)
         self._dim = dim
@@ -104,21 +113,32 @@ class VEGAS(BaseIntegrator):
         self._slot_counter = 0
         self.it = 0
 
+    def _steps(self, use_warmup):
+        """Generator over the iterations (vegas.py:130-146).
+
+        Yields after the warm-up and after each iteration's launches; the
+        value is True when the next step will read the device (the abort
+        check every fifth iteration), so a scheduler can wait for the
+        device without blocking before resuming.
+        """
         if use_warmup:
             self._warmup_grid(5, self._starting_N // 5)
-
+            yield False
         while True:
             self.it = self.it + 1
             self._run_iteration()
+            yield self.it % 5 == 0
             if self._check_abort_conditions():
                 break
 
+    def _finish(self):
+        """Final statistics and dh normalisation; returns the result tensor (vegas.py:147-159)."""
         self._sync_results()
-        call("tq_vegas_normalize_dh", dev.struct, self._prev_slot, dev.stream())
+        call("tq_vegas_normalize_dh", self._dev.struct, self._prev_slot, self._dev.stream())
         result = self._get_result()
         if isinstance(result, torch.Tensor):
             return result.to(self.dtype)
-        return torch.tensor(result, dtype=self.dtype, device=domain.device)
+        return torch.tensor(result, dtype=self.dtype, device=self._domain.device)
 
     # -- device plumbing ---------------------------------------------------
     def _next_slot(self):
