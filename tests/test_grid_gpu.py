@@ -280,3 +280,28 @@ def test_max_dim(dev):
     val = float(tq.Trapezoid().integrate(tq.ExpLinear(d, a=[0.1] * d), d, N=2**d, integration_domain=dom))
     ref = og.separable_product("trapezoid", [lambda x: np.exp(0.1 * x)] * d, [[0, 1]] * d, 2)
     assert abs(val - ref) <= 1e-12 * ref
+
+
+@pytest.mark.parametrize("cls", [tq.Boole, tq.GaussLegendre])
+def test_compiled_integrate_graph_replays(cls, dev):
+    """get_jit_compiled_integrate: graph replays with new domains equal the eager results."""
+    f = tq.GaussianPeak([2.0, 3.0, 1.5], [0.1, -0.2, 0.3])
+    integ = cls()
+    comp = integ.get_jit_compiled_integrate(3, N=21**3, integration_domain=[[-1.0, 1.0]] * 3)
+    rng = np.random.default_rng(3)
+    for rep in range(6):
+        lo = rng.uniform(-1.0, 0.0, 3)
+        dom = torch.tensor(np.stack([lo, lo + rng.uniform(0.5, 2.0, 3)], 1), dtype=torch.float32, device=dev)
+        got = comp(f, dom)
+        # eager work and a host synchronisation between replays
+        ref = cls().integrate(f, 3, N=21**3, integration_domain=dom)
+        junk = torch.randn(4096, device=dev).sum()
+        torch.cuda.synchronize()
+        assert got.shape == ref.shape and float(got) == float(ref), (rep, float(got), float(ref))
+        assert float(junk) == float(junk)
+    # another built-in integrand gets its own graph; a user callable runs eagerly
+    g = tq.Oscillatory([1.0, 2.0, 0.5], 0.25)
+    dom = torch.tensor([[0.0, 1.0]] * 3, dtype=torch.float32, device=dev)
+    assert float(comp(g, dom)) == float(cls().integrate(g, 3, N=21**3, integration_domain=dom))
+    h = lambda x: torch.sum(x * x, dim=1)  # noqa: E731
+    assert float(comp(h, dom)) == pytest.approx(1.0, rel=1e-5)

@@ -216,6 +216,7 @@ class IntegrationServer:
         self._next_slot = 0
         self._gl_cache = OrderedDict()
         self._gl_cache_bytes = 0
+        self._gl_stream = None
         self.gl_cache_limit = 256 << 20
         self._thread = None
         self._stop = threading.Event()
@@ -361,18 +362,33 @@ class IntegrationServer:
                 else:
                     self._vegas.append(r)
 
-    def _gl_table(self, n):
-        t = self._gl_cache.get(n)
-        if t is not None:
-            self._gl_cache.move_to_end(n)
-            return t
-        t = gauss_legendre(n, device=self.device)
-        self._gl_cache[n] = t
-        self._gl_cache_bytes += 16 * n
-        while self._gl_cache_bytes > self.gl_cache_limit and len(self._gl_cache) > 1:
-            old_n, _ = self._gl_cache.popitem(last=False)
-            self._gl_cache_bytes -= 16 * old_n
-        return t
+    def _gl_ready(self, n):
+        """True once the n-point Gauss-Legendre table is on the device.
+
+        A new n starts its table on a side stream, so a request whose table
+        is still being generated waits for a later batch instead of
+        delaying the requests packed with it.
+        """
+        e = self._gl_cache.get(n)
+        if e is None:
+            if self._gl_stream is None:
+                self._gl_stream = torch.cuda.Stream(device=self.device)
+            with torch.cuda.stream(self._gl_stream):
+                x, w = gauss_legendre(n, device=self.device)
+                ev = torch.cuda.Event()
+                ev.record(self._gl_stream)
+            e = (x, w, ev)
+            self._gl_cache[n] = e
+            self._gl_cache_bytes += 16 * n
+            while self._gl_cache_bytes > self.gl_cache_limit and len(self._gl_cache) > 1:
+                old_n, old = self._gl_cache.popitem(last=False)
+                if old_n == n:  # never evict the table just started
+                    self._gl_cache[old_n] = old
+                    break
+                self._gl_cache_bytes -= 16 * old_n
+            return False
+        self._gl_cache.move_to_end(n)
+        return e[2].query()
 
     def _launch_grid(self):
         if not self._admitted_grid:
@@ -386,23 +402,25 @@ class IntegrationServer:
                 break
         if slot is None:
             return
-        reqs = self._admitted_grid
-        self._admitted_grid = []
+        reqs, later = [], []
+        for r in self._admitted_grid:
+            (later if r.gl_n and not self._gl_ready(r.gl_n) else reqs).append(r)
+        self._admitted_grid = later
+        if not reqs:
+            return
         count = len(reqs)
-        # Gauss-Legendre tables of the batch: generated (once per n, cached)
-        # on the caller's stream, concatenated on the slot's stream
+        # Gauss-Legendre tables of the batch (ready), concatenated on the slot's stream
         gl_x = gl_w = None
         ns = sorted({r.gl_n for r in reqs if r.gl_n})
-        cur = torch.cuda.current_stream(self.device)
         offsets, tabs, off = {}, [], 0
         for n in ns:
             offsets[n] = off
-            tabs.append(self._gl_table(n))
+            tabs.append(self._gl_cache[n])
             off += n
         with torch.cuda.stream(slot.stream):
             if ns:
-                slot.stream.wait_stream(cur)
-                for x, w in tabs:
+                for x, w, ev in tabs:
+                    slot.stream.wait_event(ev)
                     x.record_stream(slot.stream)
                     w.record_stream(slot.stream)
                 gl_x = torch.cat([t[0] for t in tabs])
