@@ -111,7 +111,7 @@ def test_gl_generator_large_n(dev):
     for n in (1000, 5000):
         x, w = gauss_legendre(n, dev)
         xs, ws = x.cpu().numpy(), w.cpu().numpy()
-        for k in (0, 31, 40, 200, n // 2 - 1):
+        for k in (0, 1, 15, 16, 31, 40, 200, n // 2 - 1):  # 15 | 16: boundary / interior switch
             mx, mw = ogl.node_weight(n, k)
             assert abs(xs[k] - mx) <= 2e-16 * max(1.0, n / 1000), (n, k, xs[k] - mx)
             assert abs(ws[k] - mw) <= 2e-12 * mw, (n, k, (ws[k] - mw) / mw)

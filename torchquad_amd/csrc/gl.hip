@@ -11,6 +11,12 @@
 //     x_k ~ (1 - 1/(8 n^2) + 1/(8 n^3)) cos(pi (k - 1/4) / (n + 1/2)),
 // with dP_n/dtheta = n (x P_n - P_{n-1}) / sin(theta) = n (D_n - z P_n) / sin(theta)
 // and weights w_k = 2 / (1 - x_k^2) / P_n'(x_k)^2 = 2 / (dP_n/dtheta)^2.
+// For n >= 200 the boundary nodes (n sin(theta) < 50, about 16 per end) use
+// the same recurrence, spread over a workgroup: the step (D_{j-1}, P_{j-1})
+// -> (D_j, P_j) is linear, so each of the 256 lanes forms the 2x2 product of
+// its contiguous range of j, and an ordered tree (wave shuffles, then LDS
+// across waves) multiplies the 256 products -- O(n/256 + log) steps instead
+// of O(n) on one lane.
 // Interior nodes of large n: Newton in theta on the Stieltjes asymptotic
 // expansion used by Hale & Townsend (SIAM J. Sci. Comput. 35(2), 2013)
 //     P_n(cos t) = C_n sum_m h_m cos(a_m) / (2 sin t)^(m+1/2),
@@ -34,6 +40,71 @@ __device__ inline void legendre_theta(int n, double t, double* pn, double* dpdt)
         d = ((j - 1.0) * d - (2.0 * j - 1.0) * z * p) / (double)j;
         p += d;
     }
+    *pn = p;
+    *dpdt = (double)n * (d - z * p) / sin(t);
+}
+
+// P_n(cos t) and dP_n/dt by the workgroup-parallel product of the
+// recurrence's transfer matrices.  Uniform in t across the workgroup;
+// s_mat holds 4 doubles per wave.
+__device__ inline void legendre_theta_block(int n, double t, double* s_mat, double* pn, double* dpdt) {
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
+    const double sh = sin(0.5 * t);
+    const double z = 2.0 * sh * sh;
+    const long long steps = (long long)n - 1;  // j = 2..n
+    const long long c = (steps + kBlock - 1) / kBlock;
+    const long long jlo = 2 + (long long)tid * c;
+    const long long jhi = (jlo + c - 1 < (long long)n) ? (jlo + c - 1) : (long long)n;
+    double p00 = 1.0, p01 = 0.0, p10 = 0.0, p11 = 1.0;
+    for (long long j = jlo; j <= jhi; ++j) {
+        const double jd = (double)j;
+        // 1/j: hardware reciprocal refined by two Newton steps
+        double r = __builtin_amdgcn_rcp(jd);
+        r = fma(r, fma(-jd, r, 1.0), r);
+        r = fma(r, fma(-jd, r, 1.0), r);
+        const double a = 1.0 - r, bz = (2.0 - r) * z;
+        // [d; p] <- [[a, -bz], [a, 1 - bz]] [d; p]:  d' = a d - bz p,  p' = p + d'
+        const double n00 = fma(a, p00, -bz * p10);
+        const double n01 = fma(a, p01, -bz * p11);
+        p10 += n00;
+        p11 += n01;
+        p00 = n00;
+        p01 = n01;
+    }
+    // ordered product within the wave: lane l <- P(l + s) * P(l)
+    for (int s = 1; s < kWave; s <<= 1) {
+        const double q00 = __shfl_down(p00, s, kWave), q01 = __shfl_down(p01, s, kWave);
+        const double q10 = __shfl_down(p10, s, kWave), q11 = __shfl_down(p11, s, kWave);
+        if ((lane & (2 * s - 1)) == 0) {
+            const double m00 = fma(q00, p00, q01 * p10), m01 = fma(q00, p01, q01 * p11);
+            const double m10 = fma(q10, p00, q11 * p10), m11 = fma(q10, p01, q11 * p11);
+            p00 = m00;
+            p01 = m01;
+            p10 = m10;
+            p11 = m11;
+        }
+    }
+    if (lane == 0) {
+        s_mat[4 * wave + 0] = p00;
+        s_mat[4 * wave + 1] = p01;
+        s_mat[4 * wave + 2] = p10;
+        s_mat[4 * wave + 3] = p11;
+    }
+    __syncthreads();
+    double t00 = s_mat[0], t01 = s_mat[1], t10 = s_mat[2], t11 = s_mat[3];
+    for (int w = 1; w < kWavesPerBlock; ++w) {
+        const double q00 = s_mat[4 * w], q01 = s_mat[4 * w + 1], q10 = s_mat[4 * w + 2], q11 = s_mat[4 * w + 3];
+        const double m00 = fma(q00, t00, q01 * t10), m01 = fma(q00, t01, q01 * t11);
+        const double m10 = fma(q10, t00, q11 * t10), m11 = fma(q10, t01, q11 * t11);
+        t00 = m00;
+        t01 = m01;
+        t10 = m10;
+        t11 = m11;
+    }
+    __syncthreads();  // s_mat is rewritten by the next call
+    // [d_1; p_1] = [-z; 1 - z]
+    const double d = fma(t00, -z, t01 * (1.0 - z));
+    const double p = fma(t10, -z, t11 * (1.0 - z));
     *pn = p;
     *dpdt = (double)n * (d - z * p) / sin(t);
 }
@@ -70,13 +141,46 @@ __device__ inline void stieltjes(int n, double t, double* f, double* fp) {
     *fp = FP;
 }
 
-__global__ __launch_bounds__(kBlock) void k_gl_nodes(int n, double cn, double* xs, double* ws) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+// Tricomi's x estimate moved to theta to first order
+__device__ inline double tricomi_theta(int n, int k) {
+    const double nn = (double)n;
+    const double t0 = M_PI * ((double)k + 0.75) / (nn + 0.5);
+    const double eps = 1.0 / (8.0 * nn * nn) - 1.0 / (8.0 * nn * nn * nn);
+    return (n == 1) ? 0.5 * M_PI : t0 + eps * cos(t0) / sin(t0);
+}
+
+// Boundary nodes k < nb of n >= kAsymMinN: one workgroup per node.
+__global__ __launch_bounds__(kBlock) void k_gl_boundary(int n, double* xs, double* ws) {
+    __shared__ double s_mat[4 * kWavesPerBlock];
+    const int k = blockIdx.x;
+    double t = tricomi_theta(n, k);
+    double pn, dp;
+    for (int it = 0; it < 16; ++it) {
+        legendre_theta_block(n, t, s_mat, &pn, &dp);
+        const double dt = pn / dp;
+        t -= dt;
+        if (fabs(dt) <= 1e-11 * t) break;
+    }
+    legendre_theta_block(n, t, s_mat, &pn, &dp);
+    if (threadIdx.x == 0) {
+        const double x = cos(t);
+        const double w = 2.0 / (dp * dp);
+        xs[n - 1 - k] = x;
+        ws[n - 1 - k] = w;
+        xs[k] = -x;
+        ws[k] = w;
+    }
+}
+
+// Nodes k in [k_first, half): asymptotic expansion for n >= kAsymMinN (the
+// boundary kernel covers k < k_first), the per-lane recurrence for small n.
+__global__ __launch_bounds__(kBlock) void k_gl_nodes(int n, int k_first, double cn, double* xs, double* ws) {
+    const int k = k_first + blockIdx.x * blockDim.x + threadIdx.x;
     const int half = (n + 1) / 2;
     if (k >= half) return;
     const double nn = (double)n;
     const double t0 = M_PI * ((double)k + 0.75) / (nn + 0.5);
-    if (n >= kAsymMinN && nn * sin(t0) >= kAsymMinNSin) {
+    if (n >= kAsymMinN) {
         double t = t0, f, fp;
         for (int it = 0; it < 6; ++it) {
             stieltjes(n, t, &f, &fp);
@@ -95,9 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_gl_nodes(int n, double cn, double* x
         ws[k] = w;
         return;
     }
-    // Tricomi's x estimate moved to theta to first order
-    const double eps = 1.0 / (8.0 * nn * nn) - 1.0 / (8.0 * nn * nn * nn);
-    double t = (n == 1) ? 0.5 * M_PI : t0 + eps * cos(t0) / sin(t0);
+    double t = tricomi_theta(n, k);
     double pn, dp;
     // quadratic convergence: once a step is below 1e-11 t the next error is
     // below rounding, and the recurrence's own rounding keeps steps from
@@ -126,12 +228,22 @@ extern "C" int tq_gauss_legendre(int32_t n, double* nodes, double* weights, void
     if (n < 1) return fail(TQ_E_ARG, "n must be >= 1");
     if (!nodes || !weights) return fail(TQ_E_NULL, "nodes/weights is NULL");
     const int half = (n + 1) / 2;
-    const int blocks = (half + kBlock - 1) / kBlock;
     // C_n = (4/pi) prod_{j=1..n} j/(j+1/2), in extended precision
     long double cn = 4.0L / 3.14159265358979323846264338327950288L;
-    if (n >= kAsymMinN)
+    int nb = 0;  // boundary nodes (n sin(theta_k) < 50) of the workgroup kernel
+    if (n >= kAsymMinN) {
         for (int j = 1; j <= n; ++j) cn *= (long double)j / ((long double)j + 0.5L);
-    hipLaunchKernelGGL(k_gl_nodes, dim3(blocks), dim3(kBlock), 0, as_stream(stream), (int)n, (double)cn, nodes,
-                       weights);
-    return check_launch("k_gl_nodes");
+        while (nb < half && (double)n * sin(M_PI * (nb + 0.75) / (n + 0.5)) < kAsymMinNSin) ++nb;
+        hipLaunchKernelGGL(k_gl_boundary, dim3(nb), dim3(kBlock), 0, as_stream(stream), (int)n, nodes, weights);
+        int rc = check_launch("k_gl_boundary");
+        if (rc) return rc;
+    }
+    const int rest = half - nb;
+    if (rest > 0) {
+        const int blocks = (rest + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(k_gl_nodes, dim3(blocks), dim3(kBlock), 0, as_stream(stream), (int)n, nb, (double)cn,
+                           nodes, weights);
+        return check_launch("k_gl_nodes");
+    }
+    return TQ_OK;
 }

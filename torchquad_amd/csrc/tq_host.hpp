@@ -33,8 +33,9 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 constexpr double kLog2e = 1.4426950408889634074;
 
 // Family-specific derived constants computed in double precision before
-// rounding to T: Params::b, 2 pi c1 for the oscillatory family, and log2(e)
-// folded into the exponential families so that their finish is one 2^x.
+// rounding to T: Params::b, the oscillatory phase (radians in FP64,
+// revolutions in FP32), and log2(e) folded into the exponential families so
+// that their finish is one 2^x.
 template <typename T>
 inline Params<T> make_params(const tq_integrand& f) {
     Params<T> p{};
@@ -52,13 +53,16 @@ inline Params<T> make_params(const tq_integrand& f) {
                 break;
             case TQ_FAM_PRODUCT_PEAK: b = (f.a[k] != 0.0) ? 1.0 / (f.a[k] * f.a[k]) : 0.0; break;
             case TQ_FAM_EXP_LINEAR: a = f.a[k] * kLog2e; break;
+            case TQ_FAM_OSCILLATORY:
+                if (sizeof(T) == 4) a = f.a[k] / (2.0 * M_PI);  // FP32 phase in revolutions
+                break;
             default: b = 0.0;
         }
         p.a[k] = (T)a;
         p.u[k] = (T)f.u[k];
         p.b[k] = (T)b;
     }
-    if (f.family == TQ_FAM_OSCILLATORY) p.c2 = (T)(2.0 * M_PI * f.c1);
+    if (f.family == TQ_FAM_OSCILLATORY) p.c2 = (T)(sizeof(T) == 4 ? f.c1 : 2.0 * M_PI * f.c1);
     if (f.family == TQ_FAM_EXP_LINEAR) p.c1 = (T)(f.c1 * kLog2e);
     if (f.family == TQ_FAM_DOUBLE_GAUSSIAN) p.c3 = (T)(f.c3 * kLog2e);
     return p;

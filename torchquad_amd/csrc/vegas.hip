@@ -28,6 +28,7 @@ constexpr int kScanBlockCubes = kBlock * kScanItems;
 constexpr int kHistLds = 768;         // dim*n_intervals handled with LDS histograms
 constexpr int kMapLds = 1024;         // doubles of map tables staged in LDS
 constexpr int kOffLds = kSB / 2 + 8;  // cube offsets of one sample block
+constexpr size_t kMapUpdateLds = 48 * 1024;  // largest update_map scratch kept in LDS
 
 static int check_state(const tq_vegas_state* st) {
     if (!st) return fail(TQ_E_NULL, "vegas state is NULL");
@@ -172,41 +173,77 @@ __global__ __launch_bounds__(kBlock) void k_vegas_offsets(tq_vegas_state st, int
 // ---------------------------------------------------------------------------
 // histograms
 // ---------------------------------------------------------------------------
-// Adds v into bin `id` of axis k for every lane with id >= 0, one distinct
-// bin per loop trip; the leader lane writes the wave-private slot.
-__device__ inline void wave_hist_add(double* hw, uint32_t* hc, int k, int ni, int id, double v) {
+// Per-bin sums of the wave's (bin, value) pairs.  The pairs are sorted by
+// (bin, lane) with a bitonic network over the 64 lanes, then a segmented
+// inclusive scan sums each run of equal bins; the last lane of a run gets
+// head = true with the run's bin, value sum and lane count.  Lanes with
+// id < 0 take no part.  The summation order depends only on the lanes' bins,
+// so the sums are reproducible (no floating-point atomics).
+struct BinSum {
+    long long bin;
+    double sum;
+    int count;
+    bool head;
+};
+
+__device__ inline BinSum wave_bin_sums(int id, double v) {
     const int lane = threadIdx.x & (kWave - 1);
-    unsigned long long pending = __ballot(id >= 0);
-    while (pending) {
-        const int leader = __ffsll((long long)pending) - 1;
-        const int b = __shfl(id, leader, kWave);
-        const bool mine = (id == b);
-        const unsigned long long mm = __ballot(mine);
-        const double s = wave_sum(mine ? v : 0.0);
-        if (lane == leader) {
-            hw[k * ni + b] += s;
-            hc[k * ni + b] += (uint32_t)__popcll(mm);
+    constexpr unsigned long long kNone = 0x7fffffffULL;
+    unsigned long long key = ((id >= 0 ? (unsigned long long)id : kNone) << 6) | (unsigned long long)lane;
+    double val = id >= 0 ? v : 0.0;
+    for (int size = 2; size <= kWave; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const unsigned long long ok = __shfl_xor(key, stride, kWave);
+            const double ov = __shfl_xor(val, stride, kWave);
+            const bool up = (lane & size) == 0;
+            const bool lower = (lane & stride) == 0;
+            const bool take = (lower == up) ? (ok < key) : (ok > key);
+            if (take) {
+                key = ok;
+                val = ov;
+            }
         }
-        pending &= ~mm;
+    }
+    const long long bin = (long long)(key >> 6);
+    double sum = val;
+    int cnt = 1;
+    for (int off = 1; off < kWave; off <<= 1) {
+        const double ps = __shfl_up(sum, off, kWave);
+        const int pc = __shfl_up(cnt, off, kWave);
+        const long long pb = __shfl_up(bin, off, kWave);
+        if (lane >= off && pb == bin) {
+            sum += ps;
+            cnt += pc;
+        }
+    }
+    const long long nb = __shfl_down(bin, 1, kWave);
+    BinSum r;
+    r.bin = bin;
+    r.sum = sum;
+    r.count = cnt;
+    r.head = (lane == kWave - 1 || nb != bin) && bin != (long long)kNone;
+    return r;
+}
+
+// Adds v into bin `id` of axis k of the wave-private LDS slot for every lane
+// with id >= 0.
+__device__ inline void wave_hist_add(double* hw, uint32_t* hc, int k, int ni, int id, double v) {
+    if (!__ballot(id >= 0)) return;
+    const BinSum r = wave_bin_sums(id, v);
+    if (r.head) {
+        hw[k * ni + r.bin] += r.sum;
+        hc[k * ni + r.bin] += (uint32_t)r.count;
     }
 }
 
 // Same aggregation, written with FP64 atomics to the global accumulators
-// (maps too large for LDS; the summation order is then not fixed).
+// (maps too large for LDS; the summation order across waves is then not fixed).
 __device__ inline void wave_hist_add_global(double* w, double* c, int k, int ni, int id, double v) {
-    const int lane = threadIdx.x & (kWave - 1);
-    unsigned long long pending = __ballot(id >= 0);
-    while (pending) {
-        const int leader = __ffsll((long long)pending) - 1;
-        const int b = __shfl(id, leader, kWave);
-        const bool mine = (id == b);
-        const unsigned long long mm = __ballot(mine);
-        const double s = wave_sum(mine ? v : 0.0);
-        if (lane == leader) {
-            atomicAdd(&w[k * ni + b], s);
-            atomicAdd(&c[k * ni + b], (double)__popcll(mm));
-        }
-        pending &= ~mm;
+    if (!__ballot(id >= 0)) return;
+    const BinSum r = wave_bin_sums(id, v);
+    if (r.head) {
+        atomicAdd(&w[(long long)k * ni + r.bin], r.sum);
+        atomicAdd(&c[(long long)k * ni + r.bin], (double)r.count);
     }
 }
 
@@ -533,7 +570,20 @@ __global__ __launch_bounds__(kBlock) void k_vegas_strat(tq_vegas_state st, const
         if (b > st.max_samples) b = st.max_samples;
         if (a > b) a = b;
         double JF = 0.0, JF2 = 0.0;
-        for (long long s = a; s < b; ++s) {
+        // sequential sums (the oracle's order); loads issued 8 at a time so
+        // that cubes holding many samples are not latency-bound
+        long long s = a;
+        for (; s + 8 <= b; s += 8) {
+            double v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = w[s + i];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                JF += v[i];
+                JF2 += v[i] * v[i];
+            }
+        }
+        for (; s < b; ++s) {
             const double v = w[s];
             JF += v;
             JF2 += v * v;
@@ -741,8 +791,10 @@ __device__ inline int update_axis(double* x_edges, double* dx_edges, const doubl
 }
 
 // Whole update_map in one block, thread k handles axis k sequentially.
-// Scratch (hist_w): per axis 5 rows of n+1 doubles.
-__global__ __launch_bounds__(64) void k_vegas_map_update(tq_vegas_state st, int reset) {
+// Scratch: per axis 5 rows of n+1 doubles, in LDS when lds_scratch (dynamic
+// shared memory of dim*5*(n+1) doubles), else in hist_w.
+__global__ __launch_bounds__(64) void k_vegas_map_update(tq_vegas_state st, int reset, int lds_scratch) {
+    extern __shared__ double s_scr[];
     __shared__ double s_sum[TQ_MAX_DIM];
     __shared__ int s_anyzero;
     const int k = threadIdx.x;
@@ -754,7 +806,7 @@ __global__ __launch_bounds__(64) void k_vegas_map_update(tq_vegas_state st, int 
             if (st.counts[k * n + j] == 0.0) atomicOr(&s_anyzero, 1);
     __syncthreads();
     const bool any_zero = s_anyzero != 0;
-    double* scr = st.hist_w + (long long)k * 5 * (n + 1);
+    double* scr = (lds_scratch ? s_scr : st.hist_w) + (long long)k * 5 * (n + 1);
     double* w = scr;
     double* z = scr + (n + 1);
     if (k < d) s_sum[k] = smooth_fill_axis(st.weights + k * n, st.counts + k * n, n, w, z, any_zero);
@@ -868,7 +920,9 @@ static int launch_map_update(const tq_vegas_state* st, long long nsamp, int it, 
         int rc = check_launch("k_vegas_map_reduce");
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_vegas_map_update, dim3(1), dim3(64), 0, s, *st, W <= kHistLds ? 0 : 1);
+    const size_t scr = (size_t)st->dim * 5 * (st->n_intervals + 1) * sizeof(double);
+    const int lds = scr <= kMapUpdateLds ? 1 : 0;
+    hipLaunchKernelGGL(k_vegas_map_update, dim3(1), dim3(64), lds ? scr : 0, s, *st, W <= kHistLds ? 0 : 1, lds);
     return check_launch("k_vegas_map_update");
 }
 
@@ -1090,7 +1144,9 @@ int tq_vegas_map_accumulate(const tq_vegas_state* st, const void* y, const void*
 int tq_vegas_map_update_accumulated(const tq_vegas_state* st, void* stream) {
     int rc = check_state(st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_vegas_map_update, dim3(1), dim3(64), 0, as_stream(stream), *st, 1);
+    const size_t scr = (size_t)st->dim * 5 * (st->n_intervals + 1) * sizeof(double);
+    const int lds = scr <= kMapUpdateLds ? 1 : 0;
+    hipLaunchKernelGGL(k_vegas_map_update, dim3(1), dim3(64), lds ? scr : 0, as_stream(stream), *st, 1, lds);
     return check_launch("k_vegas_map_update");
 }
 
