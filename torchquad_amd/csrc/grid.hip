@@ -69,7 +69,18 @@ static int make_geom(const tq_grid_desc* gd, GridGeom& G) {
 // ---------------------------------------------------------------------------
 // Fused kernel
 // ---------------------------------------------------------------------------
-template <typename T, int FAM, bool TABLE>
+// Dynamic LDS of the fused kernel: [row weights: L*n FP64][loop table:
+// S*n T][row contributions: (S-1)*L*n T] (the row parts only when ROWTAB).
+template <typename T, int FAM>
+inline size_t fused_lds_bytes(const GridGeom& G, bool rowtab) {
+    constexpr int S = TabStride<FAM>::v;
+    size_t b = (size_t)S * G.n * sizeof(T);
+    if (rowtab) b += (size_t)G.L * G.n * (sizeof(double) + (S - 1) * sizeof(T));
+    return b;
+}
+constexpr size_t kFusedLdsMax = 48 * 1024;
+
+template <typename T, int FAM, bool TABLE, bool ROWTAB>
 __global__ __launch_bounds__(kBlock) void k_grid_fused(GridArgs<T> A) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ Axis s_axis[TQ_MAX_DIM];
@@ -79,10 +90,18 @@ __global__ __launch_bounds__(kBlock) void k_grid_fused(GridArgs<T> A) {
     const int n = g.n, d = g.d, L = g.L, rule = g.rule;
     if (tid < d) s_axis[tid] = make_axis(A.domain, tid, n, rule);
     __syncthreads();
-    T* s_tab = reinterpret_cast<T*>(smem_raw);
+    double* s_rw = reinterpret_cast<double*>(smem_raw);
+    T* s_tab = reinterpret_cast<T*>(smem_raw + (ROWTAB ? (size_t)L * n * sizeof(double) : 0));
+    T* s_rt = s_tab + (size_t)TabStride<FAM>::v * n;
     if (TABLE) {
         const AxisC<T> cL = axis_c(A.p, L);
         for (int j = tid; j < n; j += kBlock) table_entry<T, FAM>(s_tab, j, s_axis[L], cL, n, rule, A.glx, A.glw, A.p);
+        if (ROWTAB)
+            for (int e = tid; e < L * n; e += kBlock) {
+                const int k = e / n;
+                row_table_entry<T, FAM>(s_rt, s_rw, k, e - k * n, s_axis[k], axis_c(A.p, k), n, rule, A.glx, A.glw,
+                                        A.p);
+            }
         __syncthreads();
     }
     // readfirstlane: the wave index is uniform, so the tile loop runs on SGPRs
@@ -93,7 +112,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_fused(GridArgs<T> A) {
         const long long chunk = wave / g.lane_waves;
         const long long t0 = chunk * g.ipw;
         const long long t1 = (t0 + g.ipw < g.items) ? (t0 + g.ipw) : g.items;
-        acc = grid_tile<T, FAM, TABLE>(d, n, g.m, L, rule, g.lane_count, A.p, s_axis, s_tab, A.glx, A.glw, lw, t0, t1);
+        acc = grid_tile<T, FAM, TABLE, ROWTAB>(d, n, g.m, L, rule, g.lane_count, A.p, s_axis, s_tab, A.glx, A.glw, lw,
+                                               t0, t1, s_rt, s_rw);
     }
     const double bs = block_sum(acc, s_red);
     if (tid == 0) A.partials[blockIdx.x] = bs;
@@ -134,11 +154,16 @@ static int launch_grid(const tq_grid_desc* gd, const tq_integrand* f, const void
     A.partials = (double*)ws;
     A.p = make_params<T>(*f);
     TQ_DISPATCH_FAMILY(f->family, {
-        const size_t shmem = (size_t)TabStride<FAM>::v * G.n * sizeof(T);
-        if (G.use_table)
-            hipLaunchKernelGGL((k_grid_fused<T, FAM, true>), dim3((unsigned)G.blocks), dim3(kBlock), shmem, st, A);
+        const bool rowtab = G.use_table && G.L >= 1 && (fused_lds_bytes<T, FAM>(G, true)) <= kFusedLdsMax;
+        const size_t shmem = (fused_lds_bytes<T, FAM>(G, rowtab));
+        if (rowtab)
+            hipLaunchKernelGGL((k_grid_fused<T, FAM, true, true>), dim3((unsigned)G.blocks), dim3(kBlock), shmem, st,
+                               A);
+        else if (G.use_table)
+            hipLaunchKernelGGL((k_grid_fused<T, FAM, true, false>), dim3((unsigned)G.blocks), dim3(kBlock), shmem, st,
+                               A);
         else
-            hipLaunchKernelGGL((k_grid_fused<T, FAM, false>), dim3((unsigned)G.blocks), dim3(kBlock), 0, st, A);
+            hipLaunchKernelGGL((k_grid_fused<T, FAM, false, false>), dim3((unsigned)G.blocks), dim3(kBlock), 0, st, A);
     });
     rc = check_launch("k_grid_fused");
     if (rc) return rc;

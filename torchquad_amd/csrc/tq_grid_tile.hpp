@@ -95,16 +95,33 @@ __device__ inline void table_entry(T* tab, int j, const Axis& axL, const AxisC<T
     tab[S * j + S - 1] = (T)weight_of(rule, j, n, glw);
 }
 
+// Row-axis tables (ROWTAB): for the row axes k < L, entry k*n + j holds the
+// axis's contribution at node j (the state slots of table_entry) in `rt` and
+// its weight in FP64 in `rw`, so a row's state and weight are merged from
+// the tables instead of being recomputed from the axis maps.
+template <typename T, int FAM>
+__device__ inline void row_table_entry(T* rt, double* rw, int k, int j, const Axis& ax, const AxisC<T>& c, int n,
+                                       int rule, const double* glx, const double* glw, const Params<T>& p) {
+    using F = Family<FAM, T>;
+    constexpr int S1 = TabStride<FAM>::v - 1;
+    const State<T> s = F::fold_c(F::zero(), c, (T)node_of(ax, j, n, rule, glx), p);
+    const int e = k * n + j;
+    rt[S1 * e] = s.s0;
+    if (S1 == 2) rt[S1 * e + 1] = s.s1;
+    rw[e] = weight_of(rule, j, n, glw);
+}
+
 // One tile of the tensor-product index space.  m == 0 (one axis): node
 // 64 t + lane for loop items t in [t_begin, t_end).  Otherwise the lanes of the calling wave
 // take lane combinations lw*64 .. lw*64+63 of the last m axes, the wave walks
 // loop items [t_begin, t_end) (row axes [0, L) x loop axis L).  Returns the
 // lane's weighted sum.  `ax` holds the per-axis maps, `tab` the loop-axis
 // table (table_entry) when TABLE.
-template <typename T, int FAM, bool TABLE>
+template <typename T, int FAM, bool TABLE, bool ROWTAB = false>
 __device__ inline double grid_tile(int d, int n, int m, int L, int rule, int lane_count, const Params<T>& p,
                                    const Axis* ax, const T* tab, const double* glx, const double* glw, int lw,
-                                   long long t_begin, long long t_end) {
+                                   long long t_begin, long long t_end, const T* rt = nullptr,
+                                   const double* rw = nullptr) {
     using F = Family<FAM, T>;
     const int lane = threadIdx.x & (kWave - 1);
     double acc = 0.0;
@@ -171,11 +188,26 @@ __device__ inline double grid_tile(int d, int n, int m, int L, int rule, int lan
     while (t < t_end) {
         State<T> s_row = F::start(p);
         double w_row = w_lane;
+        if (ROWTAB) {
+            constexpr int S1 = TabStride<FAM>::v - 1;
 #pragma unroll
-        for (int k = 0; k < TQ_MAX_DIM; ++k) {
-            if (k < L) {
-                s_row = F::fold_c(s_row, axis_c(p, k), (T)node_of(ax[k], rdig[k], n, rule, glx), p);
-                w_row *= weight_of(rule, rdig[k], n, glw);
+            for (int k = 0; k < TQ_MAX_DIM; ++k) {
+                if (k < L) {
+                    const int e = k * n + rdig[k];
+                    State<T> term;
+                    term.s0 = rt[S1 * e];
+                    term.s1 = (S1 == 2) ? rt[S1 * e + 1] : T(0);
+                    s_row = F::merge(s_row, term);
+                    w_row *= rw[e];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < TQ_MAX_DIM; ++k) {
+                if (k < L) {
+                    s_row = F::fold_c(s_row, axis_c(p, k), (T)node_of(ax[k], rdig[k], n, rule, glx), p);
+                    w_row *= weight_of(rule, rdig[k], n, glw);
+                }
             }
         }
         const State<T> s_rl = F::merge(s_row, s_lane);
