@@ -15,7 +15,10 @@ reported as per-call latency p50/p99.  Reported beside it:
     kernel, against a Python loop of plain per-integral calls;
   * differentiation: 1024 Gauss-Legendre dim=4 n=32 Gaussian-peak integrals
     with the hand-written backward, against a loop of per-vector autograd
-    calls through the materialised-points path.
+    calls through the materialised-points path;
+  * serving: Genz integral requests arriving as a Poisson stream at
+    2000/s into the continuous-batching server (p50/p99 latency from
+    arrival to result), and the same requests submitted at once.
 
 Prints one JSON line.  --gpus N>1 exits: this build measures one GPU.
 """
@@ -150,6 +153,49 @@ def bench_batched(dev, steps):
         "plain_loop_ms": ms_plain,
         "plain_loop_evals_per_s": b.evals / (ms_plain / 1e3),
         "speedup_vs_plain_loop": ms_plain / ms_batch,
+    }
+
+
+def bench_serving(dev, rate=2000.0, count=4000):
+    """Serving loop: Poisson arrivals of Genz integrals at `rate`/s; latency and throughput."""
+    import torch
+
+    from torchquad_amd.serve import IntegrationServer, latency_stats, serve_open_loop
+    from torchquad_amd.workloads import genz_batch
+
+    items = genz_batch(count, seed=1)
+
+    def req(it):
+        return lambda s, arrival: s.submit_grid(it["rule"], it["fn"], it["dim"], N=it["N"], domain=it["domain"],
+                                                arrival=arrival)
+
+    server = IntegrationServer(dev, dtype=torch.float32)
+    # warm-up: Gauss-Legendre tables, pinned buffers, kernels
+    serve_open_loop(server, [req(it) for it in items[:200]], rate=1e6, seed=2)
+    server.completed.clear()
+    t0 = time.perf_counter()
+    tickets = serve_open_loop(server, [req(it) for it in items], rate=rate, seed=3)
+    wall = time.perf_counter() - t0
+    st = latency_stats(tickets)
+    evals = sum(t.evals for t in tickets)
+    # saturation: every request submitted at once
+    server.completed.clear()
+    t0 = time.perf_counter()
+    tickets2 = [req(it)(server, None) for it in items]
+    server.run_until_idle(timeout=120)
+    wall2 = time.perf_counter() - t0
+    return {
+        "arrival_rate_per_s": rate,
+        "requests": count,
+        "p50_ms": st.get("p50_ms"),
+        "p99_ms": st.get("p99_ms"),
+        "mean_ms": st.get("mean_ms"),
+        "max_ms": st.get("max_ms"),
+        "evals_per_s_at_rate": evals / wall,
+        "batches": server.batches_launched,
+        "saturated_evals_per_s": sum(t.evals for t in tickets2) / wall2,
+        "saturated_requests_per_s": count / wall2,
+        "workload": "Genz integrals (dim 1..6, N 1e3..1e6, trapezoid/simpson/gauss-legendre), numpy PCG64 seed 1",
     }
 
 
@@ -338,6 +384,7 @@ def main():
         extra["vegas_d6"] = bench_vegas(cfg["C3"], dev)
         extra["batched_c4"] = bench_batched(dev, args.steps)
         extra["grad_c5"] = bench_grad(cfg["C5"], dev, args.steps)
+        extra["serving_c4"] = bench_serving(dev)
 
     achieved_tflops = FLOPS_PER_EVAL_GRID_GAUSSIAN * evals_per_s / 1e12
     roofline = {

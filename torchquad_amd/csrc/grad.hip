@@ -56,13 +56,26 @@ __device__ inline void load_sweep_params(int d, int n, int rule, long long b, co
     }
 }
 
-template <typename T, bool TABLE>
+constexpr int kSweepRowTab = 1024;  // row-axis table entries (L*n) kept in LDS
+
+// Dynamic LDS: [row weights: L*n FP64][loop table: 2n T][row terms: L*n T]
+template <typename T>
+inline size_t sweep_fwd_lds(const GridGeom& g, bool table, bool rowtab) {
+    size_t b = table ? (size_t)2 * g.n * sizeof(T) : 0;
+    if (rowtab) b += (size_t)g.L * g.n * (sizeof(double) + sizeof(T));
+    return b;
+}
+
+template <typename T, bool TABLE, bool ROWTAB>
 __global__ __launch_bounds__(kBlock) void k_sweep_fwd(SweepGeom S, const T* a, const T* u, const T* dom,
                                                      const double* glx, const double* glw, double* partials) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ Params<T> s_p;
     __shared__ Axis s_axis[TQ_MAX_DIM];
-    __shared__ T s_tab[2 * kLoopTable];
     __shared__ double s_red[kWavesPerBlock];
+    double* s_rw = reinterpret_cast<double*>(smem_raw);
+    T* s_tab = reinterpret_cast<T*>(smem_raw + (ROWTAB ? (size_t)S.g.L * S.g.n * sizeof(double) : 0));
+    T* s_rt = s_tab + 2 * (size_t)S.g.n;
     const GridGeom& g = S.g;
     const long long b = blockIdx.y;
     load_sweep_params(g.d, g.n, g.rule, b, a, u, dom, &s_p, s_axis);
@@ -71,6 +84,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep_fwd(SweepGeom S, const T* a, c
         const AxisC<T> cL = axis_c(s_p, g.L);
         for (int j = threadIdx.x; j < g.n; j += kBlock)
             table_entry<T, TQ_FAM_GAUSSIAN_PEAK>(s_tab, j, s_axis[g.L], cL, g.n, g.rule, glx, glw, s_p);
+        if (ROWTAB)
+            for (int e = threadIdx.x; e < g.L * g.n; e += kBlock) {
+                const int k = e / g.n;
+                row_table_entry<T, TQ_FAM_GAUSSIAN_PEAK>(s_rt, s_rw, k, e - k * g.n, s_axis[k], axis_c(s_p, k), g.n,
+                                                         g.rule, glx, glw, s_p);
+            }
         __syncthreads();
     }
     const long long wave = (long long)blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -80,8 +99,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_fwd(SweepGeom S, const T* a, c
         const long long chunk = wave / g.lane_waves;
         const long long t0 = chunk * g.ipw;
         const long long t1 = (t0 + g.ipw < g.items) ? (t0 + g.ipw) : g.items;
-        acc = grid_tile<T, TQ_FAM_GAUSSIAN_PEAK, TABLE>(g.d, g.n, g.m, g.L, g.rule, g.lane_count, s_p, s_axis, s_tab,
-                                                        glx, glw, lw, t0, t1);
+        acc = grid_tile<T, TQ_FAM_GAUSSIAN_PEAK, TABLE, ROWTAB>(g.d, g.n, g.m, g.L, g.rule, g.lane_count, s_p, s_axis,
+                                                                s_tab, glx, glw, lw, t0, t1, s_rt, s_rw);
     }
     const double bs = block_sum(acc, s_red);
     if (threadIdx.x == 0) partials[b * gridDim.x + blockIdx.x] = bs;
@@ -394,11 +413,14 @@ static int launch_sweep_fwd(const tq_grid_desc* gd, int64_t B, const void* a, co
     if (!ws || wsb < (size_t)(S.blocks_x * B) * sizeof(double)) return fail(TQ_E_WORKSPACE, "sweep workspace too small");
     if (S.blocks_x > 0x7fffffff) return fail(TQ_E_ARG, "too many blocks");
     dim3 grid((unsigned)S.blocks_x, (unsigned)B);
-    if (S.g.use_table)
-        hipLaunchKernelGGL((k_sweep_fwd<T, true>), grid, dim3(kBlock), 0, s, S, (const T*)a, (const T*)u,
-                           (const T*)dom, gd->gl_nodes, gd->gl_weights, (double*)ws);
+    if (S.g.use_table && S.g.L >= 1 && (long long)S.g.L * S.g.n <= kSweepRowTab)
+        hipLaunchKernelGGL((k_sweep_fwd<T, true, true>), grid, dim3(kBlock), sweep_fwd_lds<T>(S.g, true, true), s, S,
+                           (const T*)a, (const T*)u, (const T*)dom, gd->gl_nodes, gd->gl_weights, (double*)ws);
+    else if (S.g.use_table)
+        hipLaunchKernelGGL((k_sweep_fwd<T, true, false>), grid, dim3(kBlock), sweep_fwd_lds<T>(S.g, true, false), s,
+                           S, (const T*)a, (const T*)u, (const T*)dom, gd->gl_nodes, gd->gl_weights, (double*)ws);
     else
-        hipLaunchKernelGGL((k_sweep_fwd<T, false>), grid, dim3(kBlock), 0, s, S, (const T*)a, (const T*)u,
+        hipLaunchKernelGGL((k_sweep_fwd<T, false, false>), grid, dim3(kBlock), 0, s, S, (const T*)a, (const T*)u,
                            (const T*)dom, gd->gl_nodes, gd->gl_weights, (double*)ws);
     rc = check_launch("k_sweep_fwd");
     if (rc) return rc;
