@@ -138,3 +138,18 @@ def test_threaded_clients(dev):
     assert results == ref
     st = latency_stats(server.completed)
     assert st["count"] == len(items) and st["p99_ms"] >= st["p50_ms"] > 0
+
+
+def test_gl_table_cache_eviction(dev):
+    """Tables evicted while a batch is packed (tiny cache) still serve every request."""
+    ns = [33, 2000, 33, 2500, 1500, 33, 2000, 700, 33]
+    f = tq.GaussianPeak([3.0], [0.4])
+    items = [{"rule": "gauss_legendre", "fn": f, "dim": 1, "n": n, "domain": [[0.0, 1.0]]} for n in ns]
+    server = IntegrationServer(dev, tile_points=TILE)
+    server.gl_cache_limit = 16 * 3000  # holds about one large table
+    for rounds in range(3):
+        tickets = [server.submit_grid(it["rule"], it["fn"], 1, n=it["n"], domain=it["domain"]) for it in items]
+        server.run_until_idle(timeout=60)
+        ref = integrate_batch(items, dtype=torch.float32, device=dev, tile_points=TILE).tolist()
+        assert [t.result() for t in tickets] == ref
+    assert server._gl_cache_bytes <= max(server.gl_cache_limit, 16 * 2500)

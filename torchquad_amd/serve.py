@@ -363,7 +363,7 @@ class IntegrationServer:
                     self._vegas.append(r)
 
     def _gl_ready(self, n):
-        """True once the n-point Gauss-Legendre table is on the device.
+        """The n-point Gauss-Legendre table (x, w, event) once it is on the device, else None.
 
         A new n starts its table on a side stream, so a request whose table
         is still being generated waits for a later batch instead of
@@ -380,15 +380,17 @@ class IntegrationServer:
             e = (x, w, ev)
             self._gl_cache[n] = e
             self._gl_cache_bytes += 16 * n
+            # least recently used tables go first; a batch being packed holds
+            # its own references, so eviction never pulls a table from under it
             while self._gl_cache_bytes > self.gl_cache_limit and len(self._gl_cache) > 1:
                 old_n, old = self._gl_cache.popitem(last=False)
                 if old_n == n:  # never evict the table just started
                     self._gl_cache[old_n] = old
                     break
                 self._gl_cache_bytes -= 16 * old_n
-            return False
+            return None
         self._gl_cache.move_to_end(n)
-        return e[2].query()
+        return e if e[2].query() else None
 
     def _launch_grid(self):
         if not self._admitted_grid:
@@ -402,9 +404,15 @@ class IntegrationServer:
                 break
         if slot is None:
             return
-        reqs, later = [], []
+        reqs, later, ready = [], [], {}
         for r in self._admitted_grid:
-            (later if r.gl_n and not self._gl_ready(r.gl_n) else reqs).append(r)
+            if r.gl_n:
+                e = ready.get(r.gl_n) or self._gl_ready(r.gl_n)
+                if e is None:
+                    later.append(r)
+                    continue
+                ready[r.gl_n] = e
+            reqs.append(r)
         self._admitted_grid = later
         if not reqs:
             return
@@ -415,7 +423,7 @@ class IntegrationServer:
         offsets, tabs, off = {}, [], 0
         for n in ns:
             offsets[n] = off
-            tabs.append(self._gl_cache[n])
+            tabs.append(ready[n])
             off += n
         with torch.cuda.stream(slot.stream):
             if ns:
