@@ -269,17 +269,21 @@ struct SampleArgs {
     const T* u_in;  // optional caller uniforms [samples][dim] (MODE_POINTS)
 };
 
+// Coordinate k of sample s from the Philox stream: FP32 uses word k%4 of
+// block k/4, FP64 words 2(k%2), 2(k%2)+1 of block k/2.  `blk` carries the
+// block between the coordinates that share it; callers walk k upwards from 0
+// with a compile-time k, so the block is generated once per 4 (2) coordinates.
 template <typename T>
-__device__ inline T uniform_coord(uint64_t seed, uint64_t s, int k);
+__device__ inline T uniform_coord(uint64_t seed, uint64_t s, int k, u32x4& blk);
 template <>
-__device__ inline float uniform_coord<float>(uint64_t seed, uint64_t s, int k) {
-    const u32x4 r = philox_block(seed, s, (uint32_t)(k >> 2));
-    return u01f(r.v[k & 3]);
+__device__ inline float uniform_coord<float>(uint64_t seed, uint64_t s, int k, u32x4& blk) {
+    if ((k & 3) == 0) blk = philox_block(seed, s, (uint32_t)(k >> 2));
+    return u01f(blk.v[k & 3]);
 }
 template <>
-__device__ inline double uniform_coord<double>(uint64_t seed, uint64_t s, int k) {
-    const u32x4 r = philox_block(seed, s, (uint32_t)(k >> 1));
-    return (k & 1) ? u01d(r.v[2], r.v[3]) : u01d(r.v[0], r.v[1]);
+__device__ inline double uniform_coord<double>(uint64_t seed, uint64_t s, int k, u32x4& blk) {
+    if ((k & 1) == 0) blk = philox_block(seed, s, (uint32_t)(k >> 1));
+    return (k & 1) ? u01d(blk.v[2], blk.v[3]) : u01d(blk.v[0], blk.v[1]);
 }
 
 template <typename T, int FAM, int MODE, bool LDS_HIST>
@@ -385,6 +389,7 @@ __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
         int ids[TQ_MAX_DIM];
         double jac = 1.0;
         long long rem = h;
+        u32x4 blk;
 #pragma unroll
         for (int k = 0; k < TQ_MAX_DIM; ++k) {
             ids[k] = -1;
@@ -392,7 +397,7 @@ __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
             if (k < d) {
                 const T u = (MODE != MODE_FUSED && A.u_in)
                                 ? (active ? A.u_in[s * d + k] : T(0))
-                                : uniform_coord<T>(A.seed, A.offset + (uint64_t)(active ? s : 0), k);
+                                : uniform_coord<T>(A.seed, A.offset + (uint64_t)(active ? s : 0), k, blk);
                 T y;
                 if (A.warmup) {
                     y = u * ymax;
