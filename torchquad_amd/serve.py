@@ -104,12 +104,13 @@ class Ticket:
 
 
 class _GridRequest:
-    __slots__ = ("ticket", "item", "gl_n")
+    __slots__ = ("ticket", "item", "gl_n", "gl_entry")
 
     def __init__(self, ticket, item, gl_n):
         self.ticket = ticket
         self.item = item
         self.gl_n = gl_n
+        self.gl_entry = None  # (nodes, weights, event) once its table is started
 
 
 class _VegasRequest:
@@ -362,39 +363,40 @@ class IntegrationServer:
                 else:
                     self._vegas.append(r)
 
-    def _gl_ready(self, n):
-        """The n-point Gauss-Legendre table (x, w, event) once it is on the device, else None.
+    def _gl_table(self, n):
+        """The n-point Gauss-Legendre table entry (nodes, weights, ready event).
 
         A new n starts its table on a side stream, so a request whose table
-        is still being generated waits for a later batch instead of
-        delaying the requests packed with it.
+        is still being generated waits for a later batch instead of delaying
+        the requests packed with it.  Requests keep a reference to their
+        entry, so evicting it from the cache (least recently used first,
+        beyond gl_cache_limit bytes) never takes a table from a request that
+        still needs it.
         """
         e = self._gl_cache.get(n)
-        if e is None:
-            if self._gl_stream is None:
-                self._gl_stream = torch.cuda.Stream(device=self.device)
-            with torch.cuda.stream(self._gl_stream):
-                x, w = gauss_legendre(n, device=self.device)
-                ev = torch.cuda.Event()
-                ev.record(self._gl_stream)
-            e = (x, w, ev)
-            self._gl_cache[n] = e
-            self._gl_cache_bytes += 16 * n
-            # least recently used tables go first; a batch being packed holds
-            # its own references, so eviction never pulls a table from under it
-            while self._gl_cache_bytes > self.gl_cache_limit and len(self._gl_cache) > 1:
-                old_n, old = self._gl_cache.popitem(last=False)
-                if old_n == n:  # never evict the table just started
-                    self._gl_cache[old_n] = old
-                    break
-                self._gl_cache_bytes -= 16 * old_n
-            return None
-        self._gl_cache.move_to_end(n)
-        return e if e[2].query() else None
+        if e is not None:
+            self._gl_cache.move_to_end(n)
+            return e
+        if self._gl_stream is None:
+            self._gl_stream = torch.cuda.Stream(device=self.device)
+        with torch.cuda.stream(self._gl_stream):
+            x, w = gauss_legendre(n, device=self.device)
+            ev = torch.cuda.Event()
+            ev.record(self._gl_stream)
+        e = (x, w, ev)
+        self._gl_cache[n] = e
+        self._gl_cache_bytes += 16 * n
+        while self._gl_cache_bytes > self.gl_cache_limit and len(self._gl_cache) > 1:
+            old_n, _ = self._gl_cache.popitem(last=False)
+            self._gl_cache_bytes -= 16 * old_n
+        return e
 
     def _launch_grid(self):
         if not self._admitted_grid:
             return
+        for r in self._admitted_grid:  # start tables even while every slot is busy
+            if r.gl_n and r.gl_entry is None:
+                r.gl_entry = self._gl_table(r.gl_n)
         slot = None
         for k in range(len(self._slots)):
             s = self._slots[(self._next_slot + k) % len(self._slots)]
@@ -407,11 +409,10 @@ class IntegrationServer:
         reqs, later, ready = [], [], {}
         for r in self._admitted_grid:
             if r.gl_n:
-                e = ready.get(r.gl_n) or self._gl_ready(r.gl_n)
-                if e is None:
+                if not r.gl_entry[2].query():
                     later.append(r)
                     continue
-                ready[r.gl_n] = e
+                ready.setdefault(r.gl_n, r.gl_entry)
             reqs.append(r)
         self._admitted_grid = later
         if not reqs:
