@@ -108,9 +108,11 @@ __global__ __launch_bounds__(kBlock) void k_grid_fused(GridArgs<T> A) {
     const long long wave = (long long)blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(tid >> 6);
     double acc = 0.0;
     if (wave < g.num_waves) {
-        const int lw = (int)(wave % g.lane_waves);
-        const long long chunk = wave / g.lane_waves;
-        const long long t0 = chunk * g.ipw;
+        // wave < 2^31 (grid size limit): 32-bit division
+        const unsigned uw = (unsigned)wave, ulw = (unsigned)g.lane_waves;
+        const unsigned chunk = uw / ulw;
+        const int lw = (int)(uw - chunk * ulw);
+        const long long t0 = (long long)chunk * g.ipw;
         const long long t1 = (t0 + g.ipw < g.items) ? (t0 + g.ipw) : g.items;
         acc = grid_tile<T, FAM, TABLE, ROWTAB>(d, n, g.m, L, rule, g.lane_count, A.p, s_axis, s_tab, A.glx, A.glw, lw,
                                                t0, t1, s_rt, s_rw);

@@ -169,11 +169,29 @@ __device__ inline double grid_tile(int d, int n, int m, int L, int rule, int lan
     }
     const AxisC<T> acL = axis_c(p, L);
     long long t = t_begin;
-    long long row = t / n;
-    int j = (int)(t - row * n);
-    // row odometer digits for axes [0, L)
+    long long row;
+    int j;
+    // row odometer digits for axes [0, L); 32-bit divisions when the item
+    // index fits (64-bit division is a long instruction sequence)
     int rdig[TQ_MAX_DIM];
-    {
+    if (t < 0x7fffffffLL) {
+        const unsigned un = (unsigned)n;
+        unsigned r = (unsigned)t / un;
+        j = (int)((unsigned)t - r * un);
+        row = r;
+#pragma unroll
+        for (int k = TQ_MAX_DIM - 1; k >= 0; --k) {
+            if (k < L) {
+                const unsigned q = r / un;
+                rdig[k] = (int)(r - q * un);
+                r = q;
+            } else {
+                rdig[k] = 0;
+            }
+        }
+    } else {
+        row = t / n;
+        j = (int)(t - row * n);
         long long r = row;
 #pragma unroll
         for (int k = TQ_MAX_DIM - 1; k >= 0; --k) {
