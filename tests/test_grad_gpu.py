@@ -92,3 +92,25 @@ def test_n_from_N_and_validation(dev):
     assert torch.equal(v1, v2)
     with pytest.raises(ValueError):
         gaussian_peak_integrals(a, u, dom[:, :1], n=9)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_long_loop_axis_without_table(dtype, dev):
+    """n > 1024 on the loop axis: the backward computes its loop terms on the fly."""
+    a = np.array([[2.0, 1.5], [3.0, 2.5]])
+    u = np.array([[0.1, -0.2], [0.3, 0.0]])
+    lo = np.array([[-1.0, -0.5], [-0.8, -1.0]])
+    hi = lo + 1.3
+    t = lambda v: torch.tensor(v, dtype=dtype, device=dev, requires_grad=True)  # noqa: E731
+    ta, tu, td = t(a), t(u), t(np.stack([lo, hi], 2))
+    out = gaussian_peak_integrals(ta, tu, td, n=1500, rule="gauss_legendre")
+    out.sum().backward()
+    tol = 2e-5 if dtype == torch.float32 else 1e-10
+    for b in range(2):
+        I, da, du, dlo, dhi = ogr.closed_form(a[b], u[b], lo[b], hi[b])
+        assert abs(float(out[b]) - I) <= (1e-6 if dtype == torch.float32 else 1e-12) * I
+        scale = max(np.max(np.abs(da)), I)
+        np.testing.assert_allclose(ta.grad[b].cpu().numpy(), da, atol=tol * scale)
+        np.testing.assert_allclose(tu.grad[b].cpu().numpy(), du, atol=tol * scale)
+        np.testing.assert_allclose(td.grad[b, :, 0].cpu().numpy(), dlo, atol=tol * scale)
+        np.testing.assert_allclose(td.grad[b, :, 1].cpu().numpy(), dhi, atol=tol * scale)

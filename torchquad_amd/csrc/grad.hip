@@ -57,6 +57,7 @@ __device__ inline void load_sweep_params(int d, int n, int rule, long long b, co
 }
 
 constexpr int kSweepRowTab = 1024;  // row-axis table entries (L*n) kept in LDS
+constexpr int kBwdTable = 1024;     // largest loop axis tabulated by the backward kernel
 
 // Dynamic LDS: [row weights: L*n FP64][loop table: 2n T][row terms: L*n T]
 template <typename T>
@@ -208,14 +209,18 @@ __device__ inline void sweep_bwd_tile(const GridGeom& g, const Params<T>& p, con
             rk[k] = T(0);
         }
     }
+    // Along a row every axis but L is fixed, so for k != L the sums over the
+    // row factor: sum W f t_k = t_k F_row, sum W f t_k^2 = t_k^2 F_row,
+    // sum W f t_k r_k = t_k r_k F_row.  Only the loop axis is accumulated
+    // per point, from its table entry (exponent term, weight, t, r).
     while (t < t_end) {
-        T w_row = w_lane;
+        double w_row = (double)w_lane;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             if (k < L) {
                 xk[k] = (T)node_of(ax[k], rdig[k], n, rule, glx);
                 rk[k] = (T)node_r(rdig[k], n, rule, glx);
-                w_row *= (T)weight_of(rule, rdig[k], n, glw);
+                w_row *= weight_of(rule, rdig[k], n, glw);
             }
         }
         // exponent and t of the fixed axes
@@ -232,40 +237,44 @@ __device__ inline void sweep_bwd_tile(const GridGeom& g, const Params<T>& p, con
         const int jend = (left < (long long)(n - j)) ? (j + (int)left) : n;
         for (int jb = j; jb < jend; jb += kFlush) {
             const int je = (jb + kFlush < jend) ? (jb + kFlush) : jend;
-            T fF = T(0), fE[D], fQ[D], fR[D];
-#pragma unroll
-            for (int k = 0; k < D; ++k) fE[k] = fQ[k] = fR[k] = T(0);
+            T fF = T(0), fE = T(0), fQ = T(0), fR = T(0);
+#pragma unroll 4
             for (int jj = jb; jj < je; ++jj) {
-                T xL, wL;
+                T term, wL, tL, rL;
                 if (tab) {
-                    xL = tab[2 * jj];
-                    wL = tab[2 * jj + 1];
+                    term = tab[4 * jj];
+                    wL = tab[4 * jj + 1];
+                    tL = tab[4 * jj + 2];
+                    rL = tab[4 * jj + 3];
                 } else {
-                    xL = (T)node_of(ax[L], jj, n, rule, glx);
+                    const T xL = (T)node_of(ax[L], jj, n, rule, glx);
+                    const T e = fma(xL, aL, bL);
+                    term = -e * e;
                     wL = (T)weight_of(rule, jj, n, glw);
+                    tL = xL - uL;
+                    rL = (T)node_r(jj, n, rule, glx);
                 }
-                const T rL = (T)node_r(jj, n, rule, glx);
-                const T tL = xL - uL;
-                const T e = fma(xL, aL, bL);
-                const T wf = wL * w_row * tq_exp2(fma(-e, e, s_fix));
+                const T wf = wL * tq_exp2(s_fix + term);
                 fF += wf;
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    const bool isL = (k == L);
-                    const T tt = isL ? tL : tk[k];
-                    const T rr = isL ? rL : rk[k];
-                    const T wt = wf * tt;
-                    fE[k] += wt;
-                    fQ[k] = fma(wt, tt, fQ[k]);
-                    fR[k] = fma(wt, rr, fR[k]);
-                }
+                const T wt = wf * tL;
+                fE += wt;
+                fQ = fma(wt, tL, fQ);
+                fR = fma(wt, rL, fR);
             }
-            accF[0] += (double)fF;
+            const double Fr = w_row * (double)fF;
+            accF[0] += Fr;
 #pragma unroll
             for (int k = 0; k < D; ++k) {
-                accE[k] += (double)fE[k];
-                accQ[k] += (double)fQ[k];
-                accR[k] += (double)fR[k];
+                if (k == L) {
+                    accE[k] += w_row * (double)fE;
+                    accQ[k] += w_row * (double)fQ;
+                    accR[k] += w_row * (double)fR;
+                } else {
+                    const double tt = (double)tk[k];
+                    accE[k] += Fr * tt;
+                    accQ[k] += Fr * tt * tt;
+                    accR[k] += Fr * tt * (double)rk[k];
+                }
             }
         }
         t += jend - j;
@@ -287,19 +296,27 @@ __device__ inline void sweep_bwd_tile(const GridGeom& g, const Params<T>& p, con
 template <typename T, int D>
 __global__ __launch_bounds__(kBlock) void k_sweep_bwd(SweepGeom S, const T* a, const T* u, const T* dom,
                                                      const double* glx, const double* glw, double* partials) {
+    // loop-axis table (dynamic LDS, 4 T per node): exponent term
+    // -(a x + b)^2 in log2 units, weight, t = x - u, normalised node r
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* s_tab = reinterpret_cast<T*>(smem_raw);
     __shared__ Params<T> s_p;
     __shared__ Axis s_axis[TQ_MAX_DIM];
-    __shared__ T s_tab[2 * kLoopTable];
     __shared__ double s_red[kWavesPerBlock];
     const GridGeom& g = S.g;
     const long long b = blockIdx.y;
     load_sweep_params(g.d, g.n, g.rule, b, a, u, dom, &s_p, s_axis);
     __syncthreads();
-    const bool table = g.m > 0 && g.L >= 0 && g.n <= kLoopTable;
+    const bool table = g.m > 0 && g.L >= 0 && g.n <= kBwdTable;
     if (table) {
+        const int L = g.L;
         for (int j = threadIdx.x; j < g.n; j += kBlock) {
-            s_tab[2 * j] = (T)node_of(s_axis[g.L], j, g.n, g.rule, glx);
-            s_tab[2 * j + 1] = (T)weight_of(g.rule, j, g.n, glw);
+            const T x = (T)node_of(s_axis[L], j, g.n, g.rule, glx);
+            const T e = fma(x, s_p.a[L], s_p.b[L]);
+            s_tab[4 * j] = -e * e;
+            s_tab[4 * j + 1] = (T)weight_of(g.rule, j, g.n, glw);
+            s_tab[4 * j + 2] = x - s_p.u[L];
+            s_tab[4 * j + 3] = (T)node_r(j, g.n, g.rule, glx);
         }
     }
     __syncthreads();
@@ -434,7 +451,8 @@ template <typename T, int D>
 static int launch_bwd_d(const SweepGeom& S, int64_t B, const void* a, const void* u, const void* dom,
                         const tq_grid_desc* gd, void* ws, hipStream_t s) {
     dim3 grid((unsigned)S.blocks_x, (unsigned)B);
-    hipLaunchKernelGGL((k_sweep_bwd<T, D>), grid, dim3(kBlock), 0, s, S, (const T*)a, (const T*)u, (const T*)dom,
+    const size_t lds = (S.g.m > 0 && S.g.L >= 0 && S.g.n <= kBwdTable) ? (size_t)4 * S.g.n * sizeof(T) : 0;
+    hipLaunchKernelGGL((k_sweep_bwd<T, D>), grid, dim3(kBlock), lds, s, S, (const T*)a, (const T*)u, (const T*)dom,
                        gd->gl_nodes, gd->gl_weights, (double*)ws);
     return check_launch("k_sweep_bwd");
 }
