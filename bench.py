@@ -317,6 +317,14 @@ def main():
             torch.neg(src, out=dst)
         torch.cuda.synchronize()
 
+    # project autotuner: items per wave of the fused kernel for this geometry
+    # (and the 17^5 latency case), chosen before the timed region
+    from torchquad_amd import _lib
+    from torchquad_amd.autotune import tune_grid
+
+    ipw, _ = tune_grid(_lib.RULE_BOOLE, f, 5, n, dom)
+    ipw17, _ = tune_grid(_lib.RULE_BOOLE, f, 5, 17, dom)
+
     for _ in range(args.warmup):
         boole.integrate(f, 5, N=N, integration_domain=dom)
     torch.cuda.synchronize()
@@ -439,6 +447,8 @@ def main():
         "data": "synthetic (seeded Genz Gaussian-peak parameters, numpy PCG64 seed 0)",
         "config": {
             "workload": f"plain Boole dim=5 [-1,1]^5 fp32 Gaussian peak, n={n}/axis ({N} evals per step)",
+            "items_per_wave": ipw,
+            "items_per_wave_n17": ipw17,
             "rel_err_vs_analytic": rel_err,
             "rel_err_vs_fp64_oracle_same_rule": rel_vs_oracle,
             "n17_latency_ms_p50": p50,

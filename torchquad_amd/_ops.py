@@ -35,8 +35,12 @@ def grid_desc(rule, dim, n, dtype, gl=None, lane_axes=0, items_per_wave=0):
 def grid_integrate(rule, integrand, dim, n, domain, gl=None, lane_axes=0, items_per_wave=0, out_f64=None):
     """Fused tensor-product rule for a built-in integrand; 0-dim result tensor."""
     require_device(domain)
-    g = grid_desc(rule, dim, n, domain.dtype, gl, lane_axes, items_per_wave)
     desc = integrand.descriptor(dim)
+    if items_per_wave == 0 and lane_axes == 0:
+        from .autotune import grid_ipw
+
+        items_per_wave = grid_ipw(rule, desc.family, dim, n, domain.dtype)
+    g = grid_desc(rule, dim, n, domain.dtype, gl, lane_axes, items_per_wave)
     nbytes = lib().tq_grid_workspace_size(ctypes.byref(g))
     ws = workspace.get(max(nbytes, 8), domain.device)
     out = torch.empty((), dtype=domain.dtype, device=domain.device)

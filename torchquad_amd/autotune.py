@@ -95,3 +95,64 @@ def autotune(batch, tiles=TILE_CANDIDATES, blocks=BLOCK_CANDIDATES, reps=3, use_
         _cache[key] = {"tile_points": best[0], "blocks": best[1], "ms": best[2], "table": table}
         _store()
     return (best[0], best[1]), table
+
+
+# ---- plain fused grid kernel: items per wave -------------------------------
+IPW_CANDIDATES = (0, 256, 512, 1024, 2048, 4096, 8192)
+
+
+def _grid_key(rule, family, dim, n, dtype):
+    return f"grid|{int(rule)}|{int(family)}|{int(dim)}|{int(n)}|{dtype}"
+
+
+def grid_ipw(rule, family, dim, n, dtype):
+    """Tuned items-per-wave of the fused grid kernel for this geometry, or 0 (automatic)."""
+    key = _grid_key(rule, family, dim, n, dtype)
+    hit = _cache.get(key)
+    if hit is None and not _cache and _cache_file():
+        with _lock:
+            _load()
+        hit = _cache.get(key)
+    return int(hit["items_per_wave"]) if hit else 0
+
+
+def tune_grid(rule, fn, dim, n, domain, candidates=IPW_CANDIDATES, reps=5, use_cache=True):
+    """Time the fused grid kernel of (rule, fn, dim, n) on `domain` for each
+    items-per-wave candidate (HIP events, median of `reps` after a warm-up),
+    cache the fastest for later calls with the same geometry and return
+    (items_per_wave, table)."""
+    from . import _ops
+
+    family = fn.descriptor(dim).family
+    key = _grid_key(rule, family, dim, n, domain.dtype)
+    with _lock:
+        if not _cache:
+            _load()
+        hit = _cache.get(key) if use_cache else None
+    if hit is not None:
+        return int(hit["items_per_wave"]), hit.get("table", [])
+    gl = None
+    if int(rule) == 3:
+        from .gl_nodes import gauss_legendre
+
+        gl = gauss_legendre(int(n), device=domain.device)
+    table = []
+    for ipw in candidates:
+        _ops.grid_integrate(rule, fn, dim, n, domain, gl, 0, ipw)
+        torch.cuda.synchronize(domain.device)
+        times = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            _ops.grid_integrate(rule, fn, dim, n, domain, gl, 0, ipw)
+            e1.record()
+            e1.synchronize()
+            times.append(e0.elapsed_time(e1))
+        times.sort()
+        table.append((ipw, times[len(times) // 2]))
+    best = min(table, key=lambda r: r[1])
+    with _lock:
+        _cache[key] = {"items_per_wave": best[0], "ms": best[1], "table": table}
+        _store()
+    return best[0], table

@@ -795,44 +795,173 @@ __device__ inline int update_axis(double* x_edges, double* dx_edges, const doubl
     return flags;
 }
 
-// Whole update_map in one block, thread k handles axis k sequentially.
-// Scratch: per axis 5 rows of n+1 doubles, in LDS when lds_scratch (dynamic
-// shared memory of dim*5*(n+1) doubles), else in hist_w.
-__global__ __launch_bounds__(64) void k_vegas_map_update(tq_vegas_state st, int reset, int lds_scratch) {
+// LDS writes of this wave visible to all of its lanes
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Whole update_map in one block, wave k handles axis k (vegas_map.py:113-261).
+// Per-bin arithmetic (weight/count, smoothing with its log and sqrt, the new
+// edges) runs across the lanes; the zero-count fill, the sums and the
+// cumulative scans run on lane 0 in bin order, so the arithmetic is that of
+// smooth_fill_axis / smooth_finish_axis / update_axis.  Scratch per axis:
+// 5 rows of n+1 doubles, in LDS when lds_scratch (dynamic shared memory of
+// dim*5*(n+1) doubles), else in hist_w.
+__global__ __launch_bounds__(1024) void k_vegas_map_update(tq_vegas_state st, int reset, int lds_scratch) {
     extern __shared__ double s_scr[];
     __shared__ double s_sum[TQ_MAX_DIM];
+    __shared__ int s_status[TQ_MAX_DIM];
     __shared__ int s_anyzero;
-    const int k = threadIdx.x;
+    const int k = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
     const int d = st.dim, n = st.n_intervals;
-    if (k == 0) s_anyzero = 0;
+    if (threadIdx.x == 0) s_anyzero = 0;
     __syncthreads();
     if (k < d)
-        for (int j = 0; j < n; ++j)
+        for (int j = lane; j < n; j += kWave)
             if (st.counts[k * n + j] == 0.0) atomicOr(&s_anyzero, 1);
     __syncthreads();
     const bool any_zero = s_anyzero != 0;
     double* scr = (lds_scratch ? s_scr : st.hist_w) + (long long)k * 5 * (n + 1);
     double* w = scr;
     double* z = scr + (n + 1);
-    if (k < d) s_sum[k] = smooth_fill_axis(st.weights + k * n, st.counts + k * n, n, w, z, any_zero);
+    double* sw = scr + 2 * (n + 1);
+    double* newx = scr + 3 * (n + 1);
+    if (k < d) {
+        const double* wt = st.weights + k * n;
+        const double* ct = st.counts + k * n;
+        for (int j = lane; j < n; j += kWave) {
+            z[j] = (ct[j] == 0.0) ? 1.0 : 0.0;
+            w[j] = (!any_zero || ct[j] != 0.0) ? wt[j] / ct[j] : wt[j];
+        }
+        wave_lds_sync();
+        if (lane == 0) {
+            if (any_zero) {
+                for (int pass = 0; pass < 10; ++pass) {
+                    bool left = false;
+                    for (int j = 0; j < n - 1; ++j)
+                        if (z[j] != 0.0) w[j] = w[j + 1];
+                    for (int j = 0; j < n - 1; ++j) z[j] = (z[j] != 0.0 && z[j + 1] != 0.0) ? 1.0 : 0.0;
+                    for (int j = n - 1; j >= 1; --j)
+                        if (z[j] != 0.0) w[j] = w[j - 1];
+                    for (int j = n - 1; j >= 1; --j) z[j] = (z[j] != 0.0 && z[j - 1] != 0.0) ? 1.0 : 0.0;
+                    for (int j = 0; j < n; ++j) left = left || (z[j] != 0.0);
+                    if (!left) break;
+                }
+            }
+            double sum = 0.0;
+            for (int j = 0; j < n; ++j) sum += w[j];
+            s_sum[k] = sum;
+        }
+    }
     __syncthreads();
     bool cannot = false;
     for (int i = 0; i < d; ++i) cannot = cannot || (s_sum[i] == 0.0);
     if (cannot) {
-        if (k == 0) st.flags[0] += 1;
+        if (threadIdx.x == 0) st.flags[0] += 1;
     } else if (k < d) {
-        double* sw = scr + 2 * (n + 1);
-        smooth_finish_axis(w, n, s_sum[k], st.alpha, sw);
-        // num/val/newx reuse rows 0,1 and row 3, 4
-        const int fl = update_axis(st.x_edges + k * (n + 1), st.dx_edges + k * n, sw, n, scr, scr + (n + 1),
-                                   scr + 3 * (n + 1));
-        if (fl & 1) atomicAdd(&st.flags[1], 1);
-        if (fl & 2) atomicAdd(&st.flags[2], 1);
+        const double sum = s_sum[k];
+        const double alpha = st.alpha;
+        // [1,6,1]/8 smoothing (edges [7,1]/8) and range compression (EQ 18-19)
+        for (int j = lane; j < n; j += kWave) {
+            double dd;
+            if (j == 0) dd = 7.0 * w[0] + w[1];
+            else if (j == n - 1) dd = w[n - 2] + 7.0 * w[n - 1];
+            else dd = w[j - 1] + 6.0 * w[j] + w[j + 1];
+            dd = dd / (8.0 * sum);
+            if (dd != 0.0) {
+                const double r = (dd - 1.0) / log(dd);
+                dd = (alpha == 0.5) ? sqrt(r) : pow(r, alpha);
+            }
+            sw[j] = dd;
+        }
+        wave_lds_sync();
+        // rows 0 and 1 now hold num / val, then per edge the accumulated
+        // remainder and the source bin of the new edge
+        double* num = scr;
+        double* val = scr + (n + 1);
+        double* xe = st.x_edges + k * (n + 1);
+        double* dxe = st.dx_edges + k * n;
+        if (lane == 0) {
+            int status = 0;
+            double total = 0.0;
+            for (int j = 0; j < n; ++j) total += sw[j];
+            const double delta_d = total / (double)n;
+            for (int m = 0; m <= n; ++m) {
+                num[m] = 0.0;
+                val[m] = 0.0;
+            }
+            double cum = 0.0;
+            for (int j = 0; j < n - 1 && !status; ++j) {
+                cum += sw[j];
+                const double q = cum / delta_d;
+                const long long mlt = (q >= 0.0 && q <= (double)n) ? (long long)q : -1;
+                if (mlt < 0) {
+                    status = 3;  // NaN / out of range: edges cannot be formed
+                } else {
+                    num[mlt] += 1.0;
+                    val[mlt] += sw[j];
+                }
+            }
+            if (!status) {
+                long long idx = 0;
+                double accu = 0.0;
+                for (int i = 0; i < n - 1; ++i) {
+                    idx += (long long)num[i];
+                    accu += delta_d - val[i];
+                    num[i] = accu;
+                    val[i] = (double)(idx > n - 1 ? n - 1 : idx);
+                }
+            }
+            s_status[k] = status;
+        }
+        wave_lds_sync();
+        if (s_status[k] == 0) {
+            for (int i = lane; i < n - 1; i += kWave) {
+                const int ix = (int)val[i];
+                newx[i + 1] = xe[ix] + num[i] / sw[ix] * dxe[ix];
+            }
+            if (lane == 0) {
+                newx[0] = xe[0];
+                newx[n] = xe[n];
+            }
+            wave_lds_sync();
+            if (lane == 0) {
+                int fl = 0;
+                bool all_finite = true;
+                for (int j = 0; j <= n; ++j) all_finite = all_finite && isfinite(newx[j]);
+                if (!all_finite) {
+                    fl |= 1;
+                    // replace by the mean of the neighbours (vegas_map.py:252-255)
+                    for (int j = 1; j < n; ++j) val[j] = isfinite(newx[j]) ? 1.0 : 0.0;
+                    for (int j = 1; j < n; ++j) num[j] = 0.5 * (newx[j - 1] + newx[j + 1]);
+                    for (int j = 1; j < n; ++j)
+                        if (val[j] == 0.0) newx[j] = num[j];
+                    for (int j = 0; j <= n; ++j)
+                        if (!isfinite(newx[j])) fl |= 2;
+                }
+                s_status[k] = fl;
+            }
+            wave_lds_sync();
+            const int fl = s_status[k];
+            if (!(fl & 2)) {
+                for (int j = lane; j <= n; j += kWave) xe[j] = newx[j];
+                for (int j = lane; j < n; j += kWave) dxe[j] = newx[j + 1] - newx[j];
+            }
+            if (lane == 0) {
+                if (fl & 1) atomicAdd(&st.flags[1], 1);
+                if (fl & 2) atomicAdd(&st.flags[2], 1);
+            }
+        } else if (lane == 0) {
+            atomicAdd(&st.flags[1], 1);
+            atomicAdd(&st.flags[2], 1);
+        }
     }
     if (reset) {
         __syncthreads();
         if (k < d)
-            for (int j = 0; j < n; ++j) {
+            for (int j = lane; j < n; j += kWave) {
                 st.weights[k * n + j] = 0.0;
                 st.counts[k * n + j] = 0.0;
             }
@@ -927,7 +1056,8 @@ static int launch_map_update(const tq_vegas_state* st, long long nsamp, int it, 
     }
     const size_t scr = (size_t)st->dim * 5 * (st->n_intervals + 1) * sizeof(double);
     const int lds = scr <= kMapUpdateLds ? 1 : 0;
-    hipLaunchKernelGGL(k_vegas_map_update, dim3(1), dim3(64), lds ? scr : 0, s, *st, W <= kHistLds ? 0 : 1, lds);
+    hipLaunchKernelGGL(k_vegas_map_update, dim3(1), dim3(kWave * st->dim), lds ? scr : 0, s, *st, W <= kHistLds ? 0 : 1,
+                       lds);
     return check_launch("k_vegas_map_update");
 }
 
@@ -1151,7 +1281,8 @@ int tq_vegas_map_update_accumulated(const tq_vegas_state* st, void* stream) {
     if (rc) return rc;
     const size_t scr = (size_t)st->dim * 5 * (st->n_intervals + 1) * sizeof(double);
     const int lds = scr <= kMapUpdateLds ? 1 : 0;
-    hipLaunchKernelGGL(k_vegas_map_update, dim3(1), dim3(64), lds ? scr : 0, as_stream(stream), *st, 1, lds);
+    hipLaunchKernelGGL(k_vegas_map_update, dim3(1), dim3(kWave * st->dim), lds ? scr : 0, as_stream(stream), *st, 1,
+                       lds);
     return check_launch("k_vegas_map_update");
 }
 
