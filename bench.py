@@ -18,7 +18,10 @@ reported as per-call latency p50/p99.  Reported beside it:
     calls through the materialised-points path;
   * serving: Genz integral requests arriving as a Poisson stream at
     2000/s into the continuous-batching server (p50/p99 latency from
-    arrival to result), and the same requests submitted at once.
+    arrival to result), and the same requests submitted at once;
+  * the reference's CPU-runnable case (Trapezoid / Simpson, dim=3,
+    101^3 points, fp64, exp(x1+x2+x3)) on the GPU, and the oracle's CPU
+    rates for it and for bounded samples of the MC and VEGAS configs.
 
 Prints one JSON line.  --gpus N>1 exits: this build measures one GPU.
 """
@@ -154,6 +157,66 @@ def bench_batched(dev, steps):
         "plain_loop_evals_per_s": b.evals / (ms_plain / 1e3),
         "speedup_vs_plain_loop": ms_plain / ms_batch,
     }
+
+
+def bench_c1(c1, dev):
+    """Reference's CPU-runnable case on the GPU: Trapezoid and Simpson, d=3, N=101^3, fp64,
+    f = exp(x1+x2+x3) on [0,1]^3."""
+    import torch
+
+    import torchquad_amd as tq
+
+    f = tq.ExpLinear(3, [1.0, 1.0, 1.0])
+    dom = torch.tensor(c1["domain"], dtype=torch.float64, device=dev)
+    out = {}
+    for name, integ in (("trapezoid", tq.Trapezoid()), ("simpson", tq.Simpson())):
+        for _ in range(3):
+            integ.integrate(f, 3, N=c1["N"], integration_domain=dom)
+        torch.cuda.synchronize()
+        e0, e1 = _events()
+        reps = 20
+        e0.record()
+        for _ in range(reps):
+            v = integ.integrate(f, 3, N=c1["N"], integration_domain=dom)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"ms_per_call": ms, "evals_per_s": c1["N"] / (ms / 1e3), "value": float(v),
+                     "rel_err_vs_analytic": abs(float(v) - c1["analytic"]) / c1["analytic"]}
+    return out
+
+
+def cpu_ports(cfg, c2b):
+    """Oracle restatements timed on the host for bounded samples of the other configs."""
+    import numpy as np
+
+    from oracle import families as fam
+    from oracle import grid as og
+    from oracle import native
+    from oracle import vegas as ov
+
+    out = {}
+    c1 = cfg["C1"]
+    for rule in ("trapezoid", "simpson"):
+        t0 = time.perf_counter()
+        og.integrate(rule, lambda x: fam.exp_linear(x, np.ones(3)), 3, c1["N"], c1["domain"])
+        dt = time.perf_counter() - t0
+        out[f"c1_{rule}"] = {"evals_per_s": c1["N"] / dt, "sample": "full C1 grid 101^3 fp64 (numpy)", "cores": 1}
+    n_mc = 2_000_000
+    t0 = time.perf_counter()
+    native.mc_sums("float32", "oscillatory", 8, n_mc, 0, 0, c2b["domain"], c1=c2b["w"], a=c2b["c"])
+    dt = time.perf_counter() - t0
+    out["mc_d8"] = {"evals_per_s": n_mc / dt, "sample": f"{n_mc} samples, C oracle (Philox + FP64 sums)",
+                    "cores": 1}
+    c3 = cfg["C3"]
+    n_v = 2_500_000
+    t0 = time.perf_counter()
+    r = ov.run(lambda x: fam.double_gaussian(x, c3["width"]), 6, n_v, c3["domain"], seed=0, dtype=np.float32,
+               N_intervals=c3["N_intervals"])
+    dt = time.perf_counter() - t0
+    out["vegas_d6"] = {"evals_per_s": r["fevals"] / dt, "sample": f"VEGAS N={n_v} (numpy restatement)",
+                       "cores": 1}
+    return out
 
 
 def bench_serving(dev, rate=2000.0, count=4000):
@@ -393,6 +456,9 @@ def main():
         extra["batched_c4"] = bench_batched(dev, args.steps)
         extra["grad_c5"] = bench_grad(cfg["C5"], dev, args.steps)
         extra["serving_c4"] = bench_serving(dev)
+        extra["c1_fp64_d3_n101"] = bench_c1(cfg["C1"], dev)
+        if not args.no_cpu:
+            extra["cpu_port_other_configs"] = cpu_ports(cfg, c2b)
 
     achieved_tflops = FLOPS_PER_EVAL_GRID_GAUSSIAN * evals_per_s / 1e12
     roofline = {
