@@ -225,3 +225,48 @@ def test_vegas_gradients(ref_vectors, dev):
                       integration_domain=[[0.0, 1.0], [0.0, 2.0]], seed=0)
     val.backward()
     np.testing.assert_array_less(np.abs(param.grad.cpu().numpy() - np.array([2.0, 1.0, 8.0 / 3.0])), 5e-2)
+
+
+def test_warm_start_and_state_dict(dev):
+    """state_dict() -> integrate(warm_start=...): the map and dh are carried over exactly."""
+    f = tq.DoubleGaussian(3, width=0.2)
+    dom = torch.tensor([[0.0, 1.0]] * 3, dtype=torch.float64, device=dev)
+    analytic = fam.double_gaussian_integral(3, [0.0] * 3, [1.0] * 3, width=0.2)
+    v1 = tq.VEGAS()
+    v1.integrate(f, 3, N=60_000, integration_domain=dom, seed=1, N_intervals=20)
+    st = v1.state_dict()
+    v2 = tq.VEGAS()
+    v2._setup(f, 3, 60_000, dom, 2, None, True, 0, 0, 20, None, 20, None, st)
+    assert torch.equal(v2._dev.x_edges, st["x_edges"]) and torch.equal(v2._dev.dh, st["dh"])
+    v3 = tq.VEGAS()
+    r3 = float(v3.integrate(f, 3, N=60_000, integration_domain=dom, seed=2, N_intervals=20, warm_start=st,
+                            use_warmup=False))
+    assert abs(r3 - analytic) <= max(5 * v3._get_error(), 1e-2 * analytic)
+    with pytest.raises(ValueError):
+        tq.VEGAS().integrate(f, 3, N=60_000, integration_domain=dom, seed=2, N_intervals=10, warm_start=st)
+
+
+def test_n_strat_override(dev):
+    f = tq.DoubleGaussian(3, width=0.2)
+    dom = torch.tensor([[0.0, 1.0]] * 3, dtype=torch.float64, device=dev)
+    v = tq.VEGAS()
+    r = float(v.integrate(f, 3, N=60_000, integration_domain=dom, seed=4, N_intervals=20, N_strat=4))
+    assert v.strat.N_strat == 4 and v._dev.n_cubes == 64
+    analytic = fam.double_gaussian_integral(3, [0.0] * 3, [1.0] * 3, width=0.2)
+    assert abs(r - analytic) <= max(5 * v._get_error(), 1e-2 * analytic)
+
+
+def test_served_vegas_warm_start(dev):
+    from torchquad_amd.serve import IntegrationServer
+
+    f = tq.DoubleGaussian(3, width=0.2)
+    server = IntegrationServer(dev, dtype=torch.float64)
+    t1 = server.submit_vegas(f, 3, N=60_000, domain=[[0.0, 1.0]] * 3, seed=1, N_intervals=20)
+    server.run_until_idle(timeout=60)
+    t2 = server.submit_vegas(f, 3, N=60_000, domain=[[0.0, 1.0]] * 3, seed=2, N_intervals=20, warm_start=t1.state,
+                             use_warmup=False)
+    server.run_until_idle(timeout=60)
+    dom = torch.tensor([[0.0, 1.0]] * 3, dtype=torch.float64, device=dev)
+    ref = float(tq.VEGAS().integrate(f, 3, N=60_000, integration_domain=dom, seed=2, N_intervals=20,
+                                     warm_start=t1.state, use_warmup=False))
+    assert t2.result() == ref

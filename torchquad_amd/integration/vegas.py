@@ -52,24 +52,33 @@ class VEGAS(BaseIntegrator):
         use_warmup=True,
         backend=None,
         N_intervals=None,
+        N_strat=None,
+        warm_start=None,
     ):
         """Integrate fn with VEGAS+ (vegas.py:30-159).
 
-        Args are the reference's; ``N_intervals`` (optional) sets the number
-        of map bins per axis, defaulting to the reference's
-        max(2, N_increment // 10) (vegas.py:117).
+        Args are the reference's, plus (all optional):
+            N_intervals: map bins per axis; default the reference's
+                max(2, N_increment // 10) (vegas.py:117).
+            N_strat: stratification steps per axis; default the reference's
+                min(1000, int((N_increment / 4)^(1/dim)))
+                (vegas_stratification.py:27-28).
+            warm_start: a ``state_dict()`` of an earlier run with the same
+                dim, N_intervals and N_strat; the map edges and the
+                stratification's dh start from it (typically with
+                use_warmup=False).
 
         Returns:
             tensor: integral value
         """
         self._setup(fn, dim, N, integration_domain, seed, rng, use_grid_improve, eps_rel, eps_abs, max_iterations,
-                    backend, N_intervals)
+                    backend, N_intervals, N_strat, warm_start)
         for _ in self._steps(use_warmup):
             pass
         return self._finish()
 
     def _setup(self, fn, dim, N, integration_domain, seed, rng, use_grid_improve, eps_rel, eps_abs, max_iterations,
-               backend, N_intervals):
+               backend, N_intervals, N_strat=None, warm_start=None):
         """Checks, budget split, RNG and resident map/stratification state (vegas.py:30-128)."""
         self._check_inputs(dim=dim, N=N, integration_domain=integration_domain)
         logger.debug("VEGAS integrating a %d-dimensional fn with %d points", dim, N)
@@ -98,13 +107,20 @@ class VEGAS(BaseIntegrator):
 
         if N_intervals is None:
             N_intervals = max(2, self._N_increment // 10)
-        n_strat = _vegas_dev.n_strat_for(self._N_increment, dim)
+        if N_strat is None:
+            n_strat = _vegas_dev.n_strat_for(self._N_increment, dim)
+        else:
+            n_strat = int(N_strat)
+            if n_strat < 1 or n_strat**dim > 1 << 31:
+                raise ValueError("N_strat must be >= 1 with N_strat**dim < 2**31")
         dev = _vegas_dev.VegasDevice(dim, N_intervals, max(n_strat, 1), self.dtype, domain.device)
         self._dev = dev
         self.map = VEGASMap(N_intervals, dim, self.backend, self.dtype, _device=dev)
         self.strat = VEGASStratification(self._N_increment, dim, rng, self.backend, self.dtype, _device=dev)
         self._desc = fn.descriptor(dim) if self._fused else None
         self._flags_seen = [0, 0, 0, 0]
+        if warm_start is not None:
+            self._load_state(warm_start)
 
         self.results = []
         self.sigma2 = []
@@ -139,6 +155,38 @@ class VEGAS(BaseIntegrator):
         if isinstance(result, torch.Tensor):
             return result.to(self.dtype)
         return torch.tensor(result, dtype=self.dtype, device=self._domain.device)
+
+    # -- checkpoint ----------------------------------------------------------
+    def state_dict(self):
+        """Map edges and stratification dh after integrate(), for a warm start.
+
+        The reference keeps this state in object attributes and re-creates it
+        on every call (vegas.py:118-128); here it can be carried to a later
+        integrate(..., warm_start=state) with the same dim, N_intervals and
+        N_strat.
+        """
+        dev = self._dev
+        return {
+            "dim": dev.dim,
+            "N_intervals": dev.n_intervals,
+            "N_strat": dev.n_strat,
+            "x_edges": dev.x_edges.detach().clone(),
+            "dx_edges": dev.dx_edges.detach().clone(),
+            "dh": dev.dh.detach().clone(),
+        }
+
+    def _load_state(self, state):
+        dev = self._dev
+        got = (int(state["dim"]), int(state["N_intervals"]), int(state["N_strat"]))
+        want = (dev.dim, dev.n_intervals, dev.n_strat)
+        if got != want:
+            raise ValueError(f"warm_start has (dim, N_intervals, N_strat) = {got}, this run needs {want}")
+        for name in ("x_edges", "dx_edges", "dh"):
+            src = torch.as_tensor(state[name], dtype=torch.float64, device=dev.device)
+            dst = getattr(dev, name)
+            if tuple(src.shape) != tuple(dst.shape):
+                raise ValueError(f"warm_start {name} has shape {tuple(src.shape)}, expected {tuple(dst.shape)}")
+            dst.copy_(src)
 
     # -- device plumbing ---------------------------------------------------
     def _next_slot(self):

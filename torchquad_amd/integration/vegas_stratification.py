@@ -26,7 +26,7 @@ class VEGASStratification:
     def __init__(self, N_increment, dim, rng, backend, dtype, beta=0.75, _device=None):
         self.rng = rng
         self.dim = dim
-        self.N_strat = _vegas_dev.n_strat_for(N_increment, dim)
+        self.N_strat = _device.n_strat if _device is not None else _vegas_dev.n_strat_for(N_increment, dim)
         self.beta = beta
         self.N_cubes = self.N_strat**self.dim
         self.V_cubes = (1.0 / self.N_strat) ** self.dim

@@ -76,6 +76,7 @@ class Ticket:
         self.value = None
         self.error = None
         self.cancelled = False
+        self.state = None  # VEGAS requests: the run's state_dict(), for a later warm start
         self._event = threading.Event()
 
     def done(self):
@@ -268,8 +269,14 @@ class IntegrationServer:
         return t
 
     def submit_vegas(self, fn, dim, N, domain=None, seed=0, N_intervals=None, max_iterations=20, eps_rel=0,
-                     eps_abs=0, use_grid_improve=True, use_warmup=True, arrival=None):
-        """Queue a VEGAS integral of a built-in integrand; returns a Ticket."""
+                     eps_abs=0, use_grid_improve=True, use_warmup=True, arrival=None, N_strat=None,
+                     warm_start=None):
+        """Queue a VEGAS integral of a built-in integrand; returns a Ticket.
+
+        N_strat and warm_start are those of VEGAS.integrate; a warm start is
+        the state_dict() of an earlier VEGAS run (e.g. a previous request's
+        integrator, kept by the caller).
+        """
         from ._vegas_dev import SB, n_strat_for
 
         if not isinstance(fn, BuiltinIntegrand):
@@ -277,7 +284,7 @@ class IntegrationServer:
         dim, N = int(dim), int(N)
         n_inc = N // (max_iterations + 5)
         ni = N_intervals if N_intervals is not None else max(2, n_inc // 10)
-        nc = max(n_strat_for(n_inc, dim), 1) ** dim
+        nc = max(int(N_strat) if N_strat is not None else n_strat_for(n_inc, dim), 1) ** dim
         cap = int(1.25 * (N // 5 + 3 * nc + SB)) + SB
         blocks = cap // SB + 4
         nbytes = (8 * (dim * (ni + 1) + 3 * dim * ni) + nc * 48 + 4096 * 32 + cap * 8
@@ -286,7 +293,8 @@ class IntegrationServer:
         if arrival is not None:
             t.t_arrival = arrival
         args = dict(fn=fn, dim=dim, N=N, domain=domain, seed=seed, N_intervals=ni, max_iterations=max_iterations,
-                    eps_rel=eps_rel, eps_abs=eps_abs, use_grid_improve=use_grid_improve, use_warmup=use_warmup)
+                    eps_rel=eps_rel, eps_abs=eps_abs, use_grid_improve=use_grid_improve, use_warmup=use_warmup,
+                    N_strat=N_strat, warm_start=warm_start)
         self._enqueue(_VegasRequest(t, args))
         return t
 
@@ -491,7 +499,8 @@ class IntegrationServer:
                         dom = torch.as_tensor(dom, dtype=self.dtype, device=self.device)
                         v = VEGAS()
                         v._setup(a["fn"], a["dim"], a["N"], dom, a["seed"], None, a["use_grid_improve"],
-                                 a["eps_rel"], a["eps_abs"], a["max_iterations"], None, a["N_intervals"])
+                                 a["eps_rel"], a["eps_abs"], a["max_iterations"], None, a["N_intervals"],
+                                 a["N_strat"], a["warm_start"])
                         r.integrator = v
                         r.steps = v._steps(a["use_warmup"])
                     try:
@@ -505,6 +514,7 @@ class IntegrationServer:
                         value = float(r.integrator._finish())
                 self._vegas.remove(r)
                 self._release(t)
+                t.state = r.integrator.state_dict()
                 t._finish(value=value)
                 self.completed.append(t)
                 finished += 1
