@@ -305,3 +305,37 @@ def test_compiled_integrate_graph_replays(cls, dev):
     assert float(comp(g, dom)) == float(cls().integrate(g, 3, N=21**3, integration_domain=dom))
     h = lambda x: torch.sum(x * x, dim=1)  # noqa: E731
     assert float(comp(h, dom)) == pytest.approx(1.0, rel=1e-5)
+
+
+def test_gauss_hermite_rule(dev):
+    """Gauss-Hermite on the fused and point/reduce kernels vs the tutorial's value, closed forms
+    and the reference-style subclass (tutorial.rst:1198-1236) on the reference layout."""
+    gh = tq.GaussHermite()
+    # the tutorial's example: int exp(-x^2) (1 - x) dx = sqrt(pi)
+    val = float(gh.integrate(lambda x: 1 - x, dim=1, N=200,
+                             integration_domain=torch.tensor([[-1.0, 1.0]], dtype=torch.float64, device=dev)))
+    assert abs(val - math.sqrt(math.pi)) <= 1e-12 * math.sqrt(math.pi)
+    dom2 = torch.tensor([[-1.0, 1.0]] * 2, dtype=torch.float64, device=dev)
+    # fused: int exp(-|x|^2) dx over R^2 = pi; int exp(-x^2 + x/2) dx = sqrt(pi) e^(1/16)
+    assert abs(float(gh.integrate(tq.Constant(2, 1.0), 2, N=30**2, integration_domain=dom2)) - math.pi) <= 1e-12 * math.pi
+    dom1 = torch.tensor([[-1.0, 1.0]], dtype=torch.float64, device=dev)
+    ref = math.sqrt(math.pi) * math.exp(1.0 / 16.0)
+    assert abs(float(gh.integrate(tq.ExpLinear(1, [0.5]), 1, N=40, integration_domain=dom1)) - ref) <= 1e-12 * ref
+
+    class UserHermite(tq.Gaussian):  # the tutorial's subclass, run on the reference layout
+        def __init__(self):
+            super().__init__()
+            import scipy.special
+
+            self._root_fn = scipy.special.roots_hermite
+
+        @staticmethod
+        def _apply_composite_rule(cur_dim_areas, dim, hs, domain):
+            for _ in range(dim):
+                cur_dim_areas = torch.sum(cur_dim_areas, dim=len(cur_dim_areas.shape) - 1)
+            return cur_dim_areas
+
+    f = tq.GaussianPeak([0.7, 1.3], [0.2, -0.1])
+    a = float(gh.integrate(f, 2, N=24**2, integration_domain=dom2))
+    b = float(UserHermite().integrate(f, 2, N=24**2, integration_domain=dom2))
+    assert abs(a - b) <= 1e-12 * abs(b)

@@ -16,7 +16,7 @@ from .integration.monte_carlo import MonteCarlo
 from .integration.trapezoid import Trapezoid
 from .integration.simpson import Simpson
 from .integration.boole import Boole
-from .integration.gaussian import GaussLegendre, Gaussian
+from .integration.gaussian import GaussHermite, GaussLegendre, Gaussian
 from .integration.vegas import VEGAS
 from .integration.vegas_map import VEGASMap
 from .integration.vegas_stratification import VEGASStratification
@@ -51,6 +51,7 @@ __all__ = [
     "VEGASMap",
     "VEGASStratification",
     "GaussLegendre",
+    "GaussHermite",
     "Gaussian",
     "RNG",
     "enable_cuda",

@@ -155,3 +155,75 @@ class GaussLegendre(Gaussian):
     def _apply_composite_rule(cur_dim_areas, dim, hs, domain):
         """Sum with (b-a)/2 per axis via the HIP reduction (gaussian.py:132-144)."""
         return _composite_by_kernel(_ops.RULES["gauss_legendre"], cur_dim_areas, dim, hs, domain)
+
+
+@_mark_owner
+class GaussHermite(Gaussian):
+    """Gauss-Hermite quadrature: integrals of exp(-|x|^2) f(x) over R^dim.
+
+    The reference shows this rule as a user subclass of Gaussian
+    (tutorial.rst:1198-1236): roots and weights from
+    ``scipy.special.roots_hermite``, identity ``_resize_roots`` and a plain
+    sum as composite rule, so the integration domain only sets dtype and
+    device.  Here it runs on the fused and point/reduce kernels of the
+    Gauss-Legendre path with the Hermite node/weight tables on the unit box
+    [-1, 1]^dim, where the affine map is the identity and the per-axis scale
+    (b - a)/2 is 1.  A subclass that overrides a hook runs its own code on
+    the reference's layout.
+    """
+
+    _tq_rule = _ops.RULES["gauss_legendre"]
+
+    def __init__(self):
+        super().__init__()
+        import scipy.special
+
+        self.name = "Gauss-Hermite"
+        self._root_fn = scipy.special.roots_hermite
+
+    def integrate(self, fn, dim, N=8, integration_domain=None, backend=None):
+        """Integrate exp(-|x|^2) fn(x) over R^dim with N total points."""
+        from .utils import _float_domain, _setup_integration_domain
+
+        dom = _float_domain(_setup_integration_domain(dim, integration_domain, backend))
+        unit = torch.tensor([[-1.0, 1.0]] * dim, dtype=dom.dtype, device=dom.device)
+        return super().integrate(fn, dim, N, unit, backend)
+
+    def _tq_standard(self):
+        import scipy.special
+
+        return (
+            super()._tq_standard()
+            and type(self)._resize_roots is Gaussian._resize_roots
+            and type(self)._roots is Gaussian._roots
+            and type(self)._cached_points_and_weights is Gaussian._cached_points_and_weights
+            and self._root_fn is scipy.special.roots_hermite
+            and self._root_args == ()
+        )
+
+    def _gl_tables(self, n, device):
+        """Device float64 (nodes, weights) of the n-point Hermite rule, cached per (n, device)."""
+        key = (int(n), str(device))
+        t = self._device_cache.get(key)
+        if t is None:
+            r, w = self._cached_points_and_weights(int(n))
+            t = (
+                torch.tensor(numpy.asarray(r, dtype=numpy.float64), device=device),
+                torch.tensor(numpy.asarray(w, dtype=numpy.float64), device=device),
+            )
+            self._device_cache[key] = t
+        return t
+
+    @property
+    def _grid_func(self):
+        f = Gaussian._grid_func.fget(self)
+        if self._tq_standard():
+            f._tq_spec = ("gauss_legendre", self._gl_tables)
+        return f
+
+    @staticmethod
+    def _apply_composite_rule(cur_dim_areas, dim, hs, domain):
+        """Plain sum over each axis (the tutorial's Gauss-Hermite composite rule)."""
+        for _ in range(dim):
+            cur_dim_areas = torch.sum(cur_dim_areas, dim=len(cur_dim_areas.shape) - 1)
+        return cur_dim_areas
