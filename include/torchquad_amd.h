@@ -201,17 +201,22 @@ typedef struct tq_batch_tile {
 int tq_batch_plan_sizes(const tq_batch_item* items, int64_t count, int32_t dtype, int64_t tile_points,
                         int64_t* desc_bytes, int64_t* n_tiles);
 /* Host-side plan: descriptors (desc_out, host memory of desc_bytes), tiles
- * ordered longest integral first with each integral's tiles contiguous,
- * tile_begin[i] / tile_count[i] for integral i. */
+ * ordered by kernel class and, within a class, longest integral first, each
+ * integral's tiles contiguous; tile_begin[i] / tile_count[i] for integral i.
+ * class_offsets (optional, [4]): tiles [class_offsets[c], class_offsets[c+1])
+ * belong to class c -- 0: loop and row-axis tables within 4 KB of LDS per
+ * wave (or no loop axis), 1: within 8 KB, 2: larger (general kernel). */
 int tq_batch_plan(const tq_batch_item* items, int64_t count, int32_t dtype, int64_t tile_points, void* desc_out,
-                  tq_batch_tile* tiles_out, int64_t* tile_begin, int64_t* tile_count);
-/* Runs a plan whose arrays were copied to the device: clears *counter,
- * persistent kernel of `blocks` workgroups (0 = automatic) pulling tiles,
- * then out[i] (dtype) / out64[i] = integral i.  tile_partials: [n_tiles]. */
+                  tq_batch_tile* tiles_out, int64_t* tile_begin, int64_t* tile_count, int64_t* class_offsets);
+/* Runs a plan whose arrays were copied to the device: clears counter[0..2],
+ * one persistent kernel per non-empty class of `blocks` workgroups (0 =
+ * automatic) pulling that class's tiles, then out[i] (dtype) / out64[i] =
+ * integral i.  tile_partials: [n_tiles]; counter: [3]; class_offsets: host
+ * [4] from tq_batch_plan, or NULL to run every tile on the general kernel. */
 int tq_batch_run(int32_t dtype, const void* desc, int64_t count, const tq_batch_tile* tiles, int64_t n_tiles,
                  const int64_t* tile_begin, const int64_t* tile_count, const double* gl_nodes,
                  const double* gl_weights, double* tile_partials, unsigned int* counter, void* out, double* out64,
-                 int32_t blocks, void* stream);
+                 int32_t blocks, const int64_t* class_offsets, void* stream);
 
 /* ---- differentiation ----------------------------------------------------
  * Parameter sweep of the Gaussian peak f = exp(-sum a_k^2 (x_k-u_k)^2) over B

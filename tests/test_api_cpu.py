@@ -186,12 +186,15 @@ def test_batch_plan_host_side():
     tiles = (_lib.BatchTile * nt.value)()
     tb = np.zeros(3, dtype=np.int64)
     tc = np.zeros(3, dtype=np.int64)
+    co = np.full(4, -1, dtype=np.int64)
     assert L.tq_batch_plan(ctypes.cast(items, ctypes.c_void_p), 3, _lib.TQ_F32, 4096,
                            desc.ctypes.data_as(ctypes.c_void_p), tiles, tb.ctypes.data_as(ctypes.c_void_p),
-                           tc.ctypes.data_as(ctypes.c_void_p)) == 0
+                           tc.ctypes.data_as(ctypes.c_void_p), co.ctypes.data_as(ctypes.c_void_p)) == 0
     assert tc.sum() == nt.value
-    # LPT: the 1e5-point GL integral first, then 21^3 = 9261, then 33^2 = 1089
+    # all three fit the small-table class: LPT order, the 1e5-point GL integral
+    # first, then 21^3 = 9261, then 33^2 = 1089
     assert list(np.argsort(tb)) == [2, 1, 0]
+    assert list(co) == [0, nt.value, nt.value, nt.value]
     for i in range(3):
         ts = [tiles[t] for t in range(tb[i], tb[i] + tc[i])]
         assert all(t.item == i for t in ts)

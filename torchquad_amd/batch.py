@@ -157,9 +157,11 @@ class Batch:
         tiles = np.zeros(max(n_tiles.value, 1) * ctypes.sizeof(_lib.BatchTile), dtype=np.uint8)
         tb = np.zeros(max(self.count, 1), dtype=np.int64)
         tc = np.zeros(max(self.count, 1), dtype=np.int64)
+        self.class_offsets = np.zeros(4, dtype=np.int64)
         call("tq_batch_plan", ctypes.cast(self._items, ctypes.c_void_p), self.count, self.dt, int(tile_points),
              desc.ctypes.data_as(ctypes.c_void_p), tiles.ctypes.data_as(ctypes.c_void_p),
-             tb.ctypes.data_as(ctypes.c_void_p), tc.ctypes.data_as(ctypes.c_void_p))
+             tb.ctypes.data_as(ctypes.c_void_p), tc.ctypes.data_as(ctypes.c_void_p),
+             self.class_offsets.ctypes.data_as(ctypes.c_void_p))
         dev = self.device
         self.n_tiles = int(n_tiles.value)
         self.desc = torch.from_numpy(desc).to(dev)
@@ -167,7 +169,7 @@ class Batch:
         self.tile_begin = torch.from_numpy(tb).to(dev)
         self.tile_count = torch.from_numpy(tc).to(dev)
         self.tile_part = torch.empty(max(self.n_tiles, 1), dtype=torch.float64, device=dev)
-        self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.counter = torch.zeros(4, dtype=torch.int32, device=dev)
         self.tile_points = int(tile_points)
 
     def retile(self, tile_points=None, blocks=None):
@@ -181,7 +183,8 @@ class Batch:
         """Launch the batch on torch's current stream; returns (out, out64)."""
         call("tq_batch_run", self.dt, ptr(self.desc), self.count, ptr(self.tiles), self.n_tiles,
              ptr(self.tile_begin), ptr(self.tile_count), ptr(self.gl_x), ptr(self.gl_w), ptr(self.tile_part),
-             ptr(self.counter), ptr(self.out), ptr(self.out64), self.blocks, stream_ptr(self.device))
+             ptr(self.counter), ptr(self.out), ptr(self.out64), self.blocks,
+             self.class_offsets.ctypes.data_as(ctypes.c_void_p), stream_ptr(self.device))
         return self.out, self.out64
 
     def graph(self):

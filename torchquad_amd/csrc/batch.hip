@@ -29,18 +29,32 @@ struct BatchDev {
 
 template <typename T>
 struct BatchTab {
-    static constexpr int N = (sizeof(T) == 4) ? 1024 : 512;  // per-wave loop table entries
+    static constexpr int N = (sizeof(T) == 4) ? 1024 : 512;  // general kernel: per-wave loop table entries
 };
 
-__global__ void k_batch_reset(unsigned int* counter) {
-    if (threadIdx.x == 0) *counter = 0u;
+// Per-wave LDS region of the row-table kernels, in bytes: class 0 (small
+// tables, high occupancy) and class 1 (loop tables up to 8 KB).
+constexpr int kRegionSmall = 4096;
+constexpr int kRegionBig = 8192;
+constexpr int kBatchClasses = 3;  // 0, 1: row-table kernels; 2: general kernel
+
+// LDS bytes of an integral's tables in the row-table kernels:
+// [row weights L*n FP64][loop table S*n T][row terms (S-1)*L*n T]
+inline long long rowtab_bytes(int fam, int n, int L, int tsize) {
+    const int S = tab_stride(fam);
+    if (L < 0) return 0;
+    return (long long)L * n * 8 + (long long)S * n * tsize + (long long)(S - 1) * L * n * tsize;
 }
 
-// Stages the integral's loop-axis table in the wave's LDS slice (when used)
-// and runs the tile.
+__global__ void k_batch_reset(unsigned int* counter) {
+    if (threadIdx.x < kBatchClasses) counter[threadIdx.x] = 0u;
+}
+
+// General kernel: loop-axis table when it fits, row axes recomputed per row.
 template <typename T, int FAM>
-__device__ inline double batch_tile_fam(const BatchDev<T>& D, const Axis* ax, T* tab, bool table, const double* glx,
-                                        const double* glw, int lw, long long t0, long long t1) {
+__device__ __forceinline__ double batch_tile_general(const BatchDev<T>& D, const Axis* ax, T* tab, bool table,
+                                            const double* glx, const double* glw, int lw, long long t0,
+                                            long long t1) {
     if (table) {
         const int lane = threadIdx.x & (kWave - 1);
         const AxisC<T> cL = axis_c(D.p, D.L);
@@ -56,12 +70,40 @@ __device__ inline double batch_tile_fam(const BatchDev<T>& D, const Axis* ax, T*
                                             t1);
 }
 
-template <typename T>
-__device__ inline double batch_tile(const BatchDev<T>& D, const Axis* ax, T* tab, bool table, const double* glx,
-                                    const double* glw, int lw, long long t0, long long t1) {
+// Row-table kernels: the planner guarantees that the loop and row tables of
+// every integral they see fit the region (or that it has no loop axis).
+template <typename T, int FAM>
+__device__ __forceinline__ double batch_tile_rowtab(const BatchDev<T>& D, const Axis* ax, unsigned char* region,
+                                           const double* glx, const double* glw, int lw, long long t0,
+                                           long long t1) {
+    const int n = D.n, L = D.L;
+    double* rw = reinterpret_cast<double*>(region);
+    T* tab = reinterpret_cast<T*>(region + (L > 0 ? (size_t)L * n * sizeof(double) : 0));
+    T* rt = tab + (size_t)TabStride<FAM>::v * n;
+    if (D.m > 0 && L >= 0) {
+        const int lane = threadIdx.x & (kWave - 1);
+        const AxisC<T> cL = axis_c(D.p, L);
+        for (int j = lane; j < n; j += kWave) table_entry<T, FAM>(tab, j, ax[L], cL, n, D.rule, glx, glw, D.p);
+        for (int e = lane; e < L * n; e += kWave) {
+            const int k = e / n;
+            row_table_entry<T, FAM>(rt, rw, k, e - k * n, ax[k], axis_c(D.p, k), n, D.rule, glx, glw, D.p);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    return grid_tile<T, FAM, true, true>(D.d, n, D.m, L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw, t0, t1, rt,
+                                         rw);
+}
+
+template <typename T, int REGION>
+__device__ __forceinline__ double batch_tile(const BatchDev<T>& D, const Axis* ax, unsigned char* region, bool table,
+                                    const double* glx, const double* glw, int lw, long long t0, long long t1) {
     switch (D.fam) {
-#define TQ_BT(F) \
-    case F: return batch_tile_fam<T, F>(D, ax, tab, table, glx, glw, lw, t0, t1);
+#define TQ_BT(F)                                                                                                 \
+    case F:                                                                                                      \
+        return REGION > 0 ? batch_tile_rowtab<T, F>(D, ax, region, glx, glw, lw, t0, t1)                         \
+                          : batch_tile_general<T, F>(D, ax, reinterpret_cast<T*>(region), table, glx, glw, lw, t0, t1);
         TQ_BT(TQ_FAM_GAUSSIAN_PEAK)
         TQ_BT(TQ_FAM_OSCILLATORY)
         TQ_BT(TQ_FAM_PRODUCT_PEAK)
@@ -74,15 +116,20 @@ __device__ inline double batch_tile(const BatchDev<T>& D, const Axis* ax, T* tab
     }
 }
 
-template <typename T>
+// Persistent kernel over tiles [0, n_tiles) of one class.  REGION > 0: row-
+// table kernel with a per-wave LDS region of REGION bytes; REGION == 0:
+// general kernel.
+template <typename T, int REGION>
 __global__ __launch_bounds__(kBlock) void k_batch(const BatchDev<T>* descs, const tq_batch_tile* tiles,
                                                   long long n_tiles, const double* glx, const double* glw,
                                                   double* tile_part, unsigned int* counter) {
     constexpr int NT = BatchTab<T>::N;
+    constexpr int RB = REGION > 0 ? REGION : (int)(2 * NT * sizeof(T));
     __shared__ Axis s_ax[kWavesPerBlock][TQ_MAX_DIM];
-    __shared__ T s_tab[kWavesPerBlock][2 * NT];
+    __shared__ double s_region[kWavesPerBlock][RB / 8];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & (kWave - 1);
+    unsigned char* region = reinterpret_cast<unsigned char*>(s_region[wave]);
     for (;;) {
         unsigned int t = 0;
         if (lane == 0) t = atomicAdd(counter, 1u);
@@ -93,8 +140,11 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchDev<T>* descs, cons
         const double* gx = glx ? glx + D.gl_offset : nullptr;
         const double* gw = glw ? glw + D.gl_offset : nullptr;
         for (int k = lane; k < D.d; k += kWave) s_ax[wave][k] = D.ax[k];
-        const bool table = D.m > 0 && D.L >= 0 && D.n * tab_stride(D.fam) <= 2 * NT;
-        double acc = batch_tile<T>(D, s_ax[wave], s_tab[wave], table, gx, gw, tile.lane_wave, tile.t0, tile.t1);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const bool table = REGION == 0 && D.m > 0 && D.L >= 0 && D.n * tab_stride(D.fam) <= 2 * NT;
+        double acc = batch_tile<T, REGION>(D, s_ax[wave], region, table, gx, gw, tile.lane_wave, tile.t0, tile.t1);
         acc = wave_sum(acc);
         if (lane == 0) tile_part[t] = acc;
         // all lanes are done with this tile's LDS before the next one writes it
@@ -154,8 +204,10 @@ static long long tile_ipw(const GridGeom& G, long long tile_points) {
 
 template <typename T>
 static int plan_batch(const tq_batch_item* items, long long count, long long tile_points, BatchDev<T>* desc,
-                      tq_batch_tile* tiles, int64_t* tile_begin, int64_t* tile_count, long long* n_tiles_out) {
+                      tq_batch_tile* tiles, int64_t* tile_begin, int64_t* tile_count, long long* n_tiles_out,
+                      int64_t* class_offsets = nullptr) {
     struct Entry {
+        int cls;
         long long cost;
         long long idx;
     };
@@ -177,7 +229,12 @@ static int plan_batch(const tq_batch_item* items, long long count, long long til
         geo[i].chunks = chunks;
         long long P;
         ipow_checked(it.n, it.dim, 1LL << 62, &P);
-        order[i] = {P, i};
+        int cls = 0;
+        if (geo[i].m > 0 && geo[i].L >= 0) {
+            const long long b = rowtab_bytes(it.family, it.n, geo[i].L, (int)sizeof(T));
+            cls = b <= kRegionSmall ? 0 : (b <= kRegionBig ? 1 : 2);
+        }
+        order[i] = {cls, P, i};
         total += chunks * geo[i].lane_waves;
         if (desc) {
             BatchDev<T>& D = desc[i];
@@ -216,13 +273,16 @@ static int plan_batch(const tq_batch_item* items, long long count, long long til
         }
     }
     if (tiles) {
-        // longest integrals first; stable on the original index
+        // by kernel class, then longest integrals first; stable on the index
         std::sort(order, order + count, [](const Entry& x, const Entry& y) {
+            if (x.cls != y.cls) return x.cls < y.cls;
             return x.cost != y.cost ? x.cost > y.cost : x.idx < y.idx;
         });
         long long t = 0;
+        int next_cls = 0;
         for (long long o = 0; o < count; ++o) {
             const long long i = order[o].idx;
+            while (class_offsets && next_cls <= order[o].cls) class_offsets[next_cls++] = t;
             const GridGeom& G = geo[i];
             tile_begin[i] = t;
             for (int lw = 0; lw < G.lane_waves; ++lw) {
@@ -236,6 +296,7 @@ static int plan_batch(const tq_batch_item* items, long long count, long long til
             }
             tile_count[i] = t - tile_begin[i];
         }
+        while (class_offsets && next_cls <= kBatchClasses) class_offsets[next_cls++] = t;
     }
     *n_tiles_out = total;
     delete[] order;
@@ -243,21 +304,41 @@ static int plan_batch(const tq_batch_item* items, long long count, long long til
     return TQ_OK;
 }
 
+template <typename T, int REGION>
+static int launch_class(const void* desc, const tq_batch_tile* tiles, long long n, const double* glx,
+                        const double* glw, double* tile_part, unsigned int* counter, int blocks, hipStream_t s) {
+    if (n <= 0) return TQ_OK;
+    long long need = (n + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (blocks > need) blocks = (int)need;
+    hipLaunchKernelGGL((k_batch<T, REGION>), dim3(blocks), dim3(kBlock), 0, s, (const BatchDev<T>*)desc, tiles, n,
+                       glx, glw, tile_part, counter);
+    return check_launch("k_batch");
+}
+
 template <typename T>
 static int run_batch(const void* desc, long long count, const tq_batch_tile* tiles, long long n_tiles,
                      const int64_t* tile_begin, const int64_t* tile_count, const double* glx, const double* glw,
                      double* tile_part, unsigned int* counter, void* out, double* out64, int blocks,
-                     hipStream_t s) {
+                     const int64_t* class_offsets, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counter);
     int rc = check_launch("k_batch_reset");
     if (rc) return rc;
     if (blocks <= 0) blocks = 2048;
-    long long need = (n_tiles + kWavesPerBlock - 1) / kWavesPerBlock;
-    if (need < 1) need = 1;
-    if (blocks > need) blocks = (int)need;
-    hipLaunchKernelGGL((k_batch<T>), dim3(blocks), dim3(kBlock), 0, s, (const BatchDev<T>*)desc, tiles, n_tiles,
-                       glx, glw, tile_part, counter);
-    rc = check_launch("k_batch");
+    long long off[kBatchClasses + 1] = {0, 0, 0, n_tiles};  // no classes: every tile on the general kernel
+    if (class_offsets) {
+        for (int c = 0; c <= kBatchClasses; ++c) off[c] = class_offsets[c];
+        for (int c = 0; c < kBatchClasses; ++c)
+            if (off[c] < 0 || off[c] > off[c + 1] || off[kBatchClasses] != n_tiles)
+                return fail(TQ_E_ARG, "class_offsets do not partition the tiles");
+    }
+    rc = launch_class<T, kRegionSmall>(desc, tiles + off[0], off[1] - off[0], glx, glw, tile_part + off[0],
+                                       counter + 0, blocks, s);
+    if (!rc)
+        rc = launch_class<T, kRegionBig>(desc, tiles + off[1], off[2] - off[1], glx, glw, tile_part + off[1],
+                                         counter + 1, blocks, s);
+    if (!rc)
+        rc = launch_class<T, 0>(desc, tiles + off[2], off[3] - off[2], glx, glw, tile_part + off[2], counter + 2,
+                                blocks, s);
     if (rc) return rc;
     const long long fb = (count + kWavesPerBlock - 1) / kWavesPerBlock;
     hipLaunchKernelGGL((k_batch_finalize<T>), dim3((unsigned)(fb > 0 ? fb : 1)), dim3(kBlock), 0, s,
@@ -286,33 +367,33 @@ int tq_batch_plan_sizes(const tq_batch_item* items, int64_t count, int32_t dtype
 }
 
 int tq_batch_plan(const tq_batch_item* items, int64_t count, int32_t dtype, int64_t tile_points, void* desc_out,
-                  tq_batch_tile* tiles_out, int64_t* tile_begin, int64_t* tile_count) {
+                  tq_batch_tile* tiles_out, int64_t* tile_begin, int64_t* tile_count, int64_t* class_offsets) {
     if (!items || !desc_out || !tiles_out || !tile_begin || !tile_count) return fail(TQ_E_NULL, "NULL argument");
     if (count < 0 || tile_points < 1) return fail(TQ_E_ARG, "count/tile_points out of range");
     long long nt = 0;
     if (dtype == TQ_F32)
         return plan_batch<float>(items, count, tile_points, (BatchDev<float>*)desc_out, tiles_out, tile_begin,
-                                 tile_count, &nt);
+                                 tile_count, &nt, class_offsets);
     if (dtype == TQ_F64)
         return plan_batch<double>(items, count, tile_points, (BatchDev<double>*)desc_out, tiles_out, tile_begin,
-                                  tile_count, &nt);
+                                  tile_count, &nt, class_offsets);
     return fail(TQ_E_ARG, "dtype must be TQ_F32 or TQ_F64");
 }
 
 int tq_batch_run(int32_t dtype, const void* desc, int64_t count, const tq_batch_tile* tiles, int64_t n_tiles,
                  const int64_t* tile_begin, const int64_t* tile_count, const double* gl_nodes,
                  const double* gl_weights, double* tile_partials, unsigned int* counter, void* out, double* out64,
-                 int32_t blocks, void* stream) {
+                 int32_t blocks, const int64_t* class_offsets, void* stream) {
     if (!desc || !tiles || !tile_begin || !tile_count || !tile_partials || !counter)
         return fail(TQ_E_NULL, "NULL argument");
     if (count < 0 || n_tiles < 0 || n_tiles > 0xffffffffLL) return fail(TQ_E_ARG, "count/n_tiles out of range");
     if (!out && !out64) return fail(TQ_E_NULL, "no output pointer");
     if (dtype == TQ_F32)
         return run_batch<float>(desc, count, tiles, n_tiles, tile_begin, tile_count, gl_nodes, gl_weights,
-                                tile_partials, counter, out, out64, blocks, as_stream(stream));
+                                tile_partials, counter, out, out64, blocks, class_offsets, as_stream(stream));
     if (dtype == TQ_F64)
         return run_batch<double>(desc, count, tiles, n_tiles, tile_begin, tile_count, gl_nodes, gl_weights,
-                                 tile_partials, counter, out, out64, blocks, as_stream(stream));
+                                 tile_partials, counter, out, out64, blocks, class_offsets, as_stream(stream));
     return fail(TQ_E_ARG, "dtype must be TQ_F32 or TQ_F64");
 }
 

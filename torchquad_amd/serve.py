@@ -157,7 +157,8 @@ class _BatchSlot:
         self.tb = _Pinned(device)
         self.tc = _Pinned(device)
         self.part = torch.empty(0, dtype=torch.float64, device=device)
-        self.counter = torch.zeros(1, dtype=torch.int32, device=device)
+        self.counter = torch.zeros(4, dtype=torch.int32, device=device)
+        self.class_offsets = np.zeros(4, dtype=np.int64)
         self.out = torch.empty(0, dtype=dtype, device=device)
         self.out64 = torch.empty(0, dtype=torch.float64, device=device)
         self.host_out = torch.empty(0, dtype=torch.float64, pin_memory=True)
@@ -459,7 +460,7 @@ class IntegrationServer:
             bh, bd = slot.tb.reserve(count * 8)
             ch, cd = slot.tc.reserve(count * 8)
             call("tq_batch_plan", ctypes.c_void_p(base), count, self.dt, self.tile_points, ptr(dh), ptr(th), ptr(bh),
-                 ptr(ch))
+                 ptr(ch), slot.class_offsets.ctypes.data_as(ctypes.c_void_p))
             dd[: desc_bytes.value].copy_(dh[: desc_bytes.value], non_blocking=True)
             td[: nt * TILE_BYTES].copy_(th[: nt * TILE_BYTES], non_blocking=True)
             bd[: count * 8].copy_(bh[: count * 8], non_blocking=True)
@@ -467,7 +468,7 @@ class IntegrationServer:
             slot.ensure_outputs(count, nt)
             call("tq_batch_run", self.dt, ptr(dd), count, ptr(td), int(n_tiles.value), ptr(bd), ptr(cd), ptr(gl_x),
                  ptr(gl_w), ptr(slot.part), ptr(slot.counter), ptr(slot.out), ptr(slot.out64), 0,
-                 slot.stream.cuda_stream)
+                 slot.class_offsets.ctypes.data_as(ctypes.c_void_p), slot.stream.cuda_stream)
             slot.host_out[:count].copy_(slot.out64[:count], non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(slot.stream)
