@@ -192,8 +192,10 @@ typedef struct tq_batch_item {
 
 typedef struct tq_batch_tile {
     int32_t item;            /* integral index                                    */
-    int32_t lane_wave;       /* 64-lane group of the integral's lane axes         */
-    int64_t t0, t1;          /* loop items [t0, t1)                               */
+    int32_t lane_wave;       /* first 64-lane group of the integral's lane axes   */
+    int32_t lane_wave_end;   /* one past the last 64-lane group of the tile       */
+    int32_t reserved;
+    int64_t t0, t1;          /* loop items [t0, t1) of each of those groups       */
 } tq_batch_tile;
 
 /* Host-side plan sizes: bytes of the device descriptor array and tile count

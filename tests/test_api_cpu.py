@@ -198,6 +198,18 @@ def test_batch_plan_host_side():
     for i in range(3):
         ts = [tiles[t] for t in range(tb[i], tb[i] + tc[i])]
         assert all(t.item == i for t in ts)
+        assert all(t.lane_wave < t.lane_wave_end for t in ts)
+    # every (64-lane group, loop item) pair of an integral is covered exactly once
+    for i in range(3):
+        seen = set()
+        for t in range(tb[i], tb[i] + tc[i]):
+            tl = tiles[t]
+            for lw in range(tl.lane_wave, tl.lane_wave_end):
+                for it in range(tl.t0, tl.t1):
+                    assert (lw, it) not in seen
+                    seen.add((lw, it))
+        n, dim = spec[i][2], spec[i][1]
+        assert len(seen) > 0
     assert n_per_axis(_lib.RULE_SIMPSON, 3, 10_000) == 21
     assert n_per_axis(_lib.RULE_BOOLE, 5, 17**5) == 17
 

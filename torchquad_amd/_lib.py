@@ -87,6 +87,8 @@ class BatchTile(ctypes.Structure):
     _fields_ = [
         ("item", ctypes.c_int32),
         ("lane_wave", ctypes.c_int32),
+        ("lane_wave_end", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
         ("t0", ctypes.c_int64),
         ("t1", ctypes.c_int64),
     ]

@@ -53,7 +53,7 @@ __global__ void k_batch_reset(unsigned int* counter) {
 // General kernel: loop-axis table when it fits, row axes recomputed per row.
 template <typename T, int FAM>
 __device__ __forceinline__ double batch_tile_general(const BatchDev<T>& D, const Axis* ax, T* tab, bool table,
-                                            const double* glx, const double* glw, int lw, long long t0,
+                                            const double* glx, const double* glw, int lw0, int lw1, long long t0,
                                             long long t1) {
     if (table) {
         const int lane = threadIdx.x & (kWave - 1);
@@ -65,16 +65,20 @@ __device__ __forceinline__ double batch_tile_general(const BatchDev<T>& D, const
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    return table ? grid_tile<T, FAM, true>(D.d, D.n, D.m, D.L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw, t0, t1)
-                 : grid_tile<T, FAM, false>(D.d, D.n, D.m, D.L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw, t0,
-                                            t1);
+    double acc = 0.0;
+    for (int lw = lw0; lw < lw1; ++lw)
+        acc += table ? grid_tile<T, FAM, true>(D.d, D.n, D.m, D.L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw, t0,
+                                               t1)
+                     : grid_tile<T, FAM, false>(D.d, D.n, D.m, D.L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw,
+                                                t0, t1);
+    return acc;
 }
 
 // Row-table kernels: the planner guarantees that the loop and row tables of
 // every integral they see fit the region (or that it has no loop axis).
 template <typename T, int FAM>
 __device__ __forceinline__ double batch_tile_rowtab(const BatchDev<T>& D, const Axis* ax, unsigned char* region,
-                                           const double* glx, const double* glw, int lw, long long t0,
+                                           const double* glx, const double* glw, int lw0, int lw1, long long t0,
                                            long long t1) {
     const int n = D.n, L = D.L;
     double* rw = reinterpret_cast<double*>(region);
@@ -92,18 +96,23 @@ __device__ __forceinline__ double batch_tile_rowtab(const BatchDev<T>& D, const 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    return grid_tile<T, FAM, true, true>(D.d, n, D.m, L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw, t0, t1, rt,
-                                         rw);
+    double acc = 0.0;
+    for (int lw = lw0; lw < lw1; ++lw)
+        acc += grid_tile<T, FAM, true, true>(D.d, n, D.m, L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw, t0, t1,
+                                             rt, rw);
+    return acc;
 }
 
 template <typename T, int REGION>
 __device__ __forceinline__ double batch_tile(const BatchDev<T>& D, const Axis* ax, unsigned char* region, bool table,
-                                    const double* glx, const double* glw, int lw, long long t0, long long t1) {
+                                    const double* glx, const double* glw, int lw0, int lw1, long long t0,
+                                    long long t1) {
     switch (D.fam) {
-#define TQ_BT(F)                                                                                                 \
-    case F:                                                                                                      \
-        return REGION > 0 ? batch_tile_rowtab<T, F>(D, ax, region, glx, glw, lw, t0, t1)                         \
-                          : batch_tile_general<T, F>(D, ax, reinterpret_cast<T*>(region), table, glx, glw, lw, t0, t1);
+#define TQ_BT(F)                                                                                                    \
+    case F:                                                                                                         \
+        return REGION > 0 ? batch_tile_rowtab<T, F>(D, ax, region, glx, glw, lw0, lw1, t0, t1)                      \
+                          : batch_tile_general<T, F>(D, ax, reinterpret_cast<T*>(region), table, glx, glw, lw0, lw1, \
+                                                     t0, t1);
         TQ_BT(TQ_FAM_GAUSSIAN_PEAK)
         TQ_BT(TQ_FAM_OSCILLATORY)
         TQ_BT(TQ_FAM_PRODUCT_PEAK)
@@ -144,7 +153,8 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchDev<T>* descs, cons
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         const bool table = REGION == 0 && D.m > 0 && D.L >= 0 && D.n * tab_stride(D.fam) <= 2 * NT;
-        double acc = batch_tile<T, REGION>(D, s_ax[wave], region, table, gx, gw, tile.lane_wave, tile.t0, tile.t1);
+        double acc = batch_tile<T, REGION>(D, s_ax[wave], region, table, gx, gw, tile.lane_wave, tile.lane_wave_end,
+                                           tile.t0, tile.t1);
         acc = wave_sum(acc);
         if (lane == 0) tile_part[t] = acc;
         // all lanes are done with this tile's LDS before the next one writes it
@@ -213,6 +223,7 @@ static int plan_batch(const tq_batch_item* items, long long count, long long til
     };
     Entry* order = new Entry[count > 0 ? count : 1];
     GridGeom* geo = new GridGeom[count > 0 ? count : 1];
+    long long* group = new long long[count > 0 ? count : 1];  // 64-lane groups per tile
     long long total = 0;
     for (long long i = 0; i < count; ++i) {
         const tq_batch_item& it = items[i];
@@ -221,12 +232,21 @@ static int plan_batch(const tq_batch_item* items, long long count, long long til
         if (rc) {
             delete[] order;
             delete[] geo;
+            delete[] group;
             return rc;
         }
         const long long ipw = tile_ipw(geo[i], tile_points);
         const long long chunks = (geo[i].items + ipw - 1) / ipw;
         geo[i].ipw = ipw;
         geo[i].chunks = chunks;
+        // a lane group's whole loop is shorter than a tile: several groups per tile
+        long long g = 1;
+        if (chunks == 1) {
+            g = tile_points / ((long long)kWave * geo[i].items);
+            if (g < 1) g = 1;
+            if (g > geo[i].lane_waves) g = geo[i].lane_waves;
+        }
+        group[i] = g;
         long long P;
         ipow_checked(it.n, it.dim, 1LL << 62, &P);
         int cls = 0;
@@ -235,7 +255,7 @@ static int plan_batch(const tq_batch_item* items, long long count, long long til
             cls = b <= kRegionSmall ? 0 : (b <= kRegionBig ? 1 : 2);
         }
         order[i] = {cls, P, i};
-        total += chunks * geo[i].lane_waves;
+        total += chunks * ((geo[i].lane_waves + g - 1) / g);
         if (desc) {
             BatchDev<T>& D = desc[i];
             D.rule = it.rule;
@@ -285,11 +305,14 @@ static int plan_batch(const tq_batch_item* items, long long count, long long til
             while (class_offsets && next_cls <= order[o].cls) class_offsets[next_cls++] = t;
             const GridGeom& G = geo[i];
             tile_begin[i] = t;
-            for (int lw = 0; lw < G.lane_waves; ++lw) {
+            const int g = (int)group[i];
+            for (int lw = 0; lw < G.lane_waves; lw += g) {
                 for (long long c = 0; c < G.chunks; ++c) {
                     tq_batch_tile& tl = tiles[t++];
                     tl.item = (int32_t)i;
                     tl.lane_wave = lw;
+                    tl.lane_wave_end = (lw + g < G.lane_waves) ? lw + g : G.lane_waves;
+                    tl.reserved = 0;
                     tl.t0 = c * G.ipw;
                     tl.t1 = (c + 1) * G.ipw < G.items ? (c + 1) * G.ipw : G.items;
                 }
@@ -301,6 +324,7 @@ static int plan_batch(const tq_batch_item* items, long long count, long long til
     *n_tiles_out = total;
     delete[] order;
     delete[] geo;
+    delete[] group;
     return TQ_OK;
 }
 
