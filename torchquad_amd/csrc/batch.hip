@@ -205,8 +205,14 @@ static int check_item(const tq_batch_item& it) {
     return TQ_OK;
 }
 
+// Split-mode integrals (one axis, m == 0) walk one 64-node row per item
+// with global loads for Gauss-Legendre tables; shorter tiles spread a long
+// axis over more waves.
+constexpr long long kSplitMaxItems = 128;
+
 static long long tile_ipw(const GridGeom& G, long long tile_points) {
     long long ipw = tile_points / kWave;
+    if (G.m == 0 && ipw > kSplitMaxItems) ipw = kSplitMaxItems;
     if (ipw < 1) ipw = 1;
     if (ipw > G.items) ipw = G.items;
     return ipw;

@@ -126,20 +126,29 @@ __device__ inline double grid_tile(int d, int n, int m, int L, int rule, int lan
     const int lane = threadIdx.x & (kWave - 1);
     double acc = 0.0;
     if (m == 0) {
-        // one axis split into rows of 64 consecutive nodes: node k = 64 t + lane
+        // one axis split into rows of 64 consecutive nodes: node k = 64 t + lane.
+        // Branch-free body (index clamped, zero weight past n) so that the
+        // loop unrolls and the node/weight loads of later steps issue early.
         const State<T> s0 = F::start(p);
         const AxisC<T> ac0 = axis_c(p, 0);
+        const bool gl = rule == TQ_RULE_GAUSS_LEGENDRE;
         for (long long tb = t_begin; tb < t_end; tb += kFlush) {
             const long long te = (tb + kFlush < t_end) ? (tb + kFlush) : t_end;
             T fa = T(0);
+#pragma unroll 4
             for (long long t = tb; t < te; ++t) {
                 const long long k = t * kWave + lane;
-                if (k < n) {
-                    const int kk = (int)k;
-                    const T x = (T)node_of(ax[0], kk, n, rule, glx);
-                    const T w = (T)weight_of(rule, kk, n, glw);
-                    fa = fma(w, F::finish(F::fold_c(s0, ac0, x, p), p), fa);
+                const int kk = (int)(k < n ? k : n - 1);
+                double xd, wd;
+                if (gl) {
+                    xd = fma(ax[0].a, glx[kk], ax[0].h);
+                    wd = glw[kk];
+                } else {
+                    xd = (kk == n - 1) ? ax[0].b : fma((double)kk, ax[0].h, ax[0].a);
+                    wd = weight_of(rule, kk, n, glw);
                 }
+                const T w = (k < n) ? (T)wd : T(0);
+                fa = fma(w, F::finish(F::fold_c(s0, ac0, (T)xd, p), p), fa);
             }
             acc += (double)fa;
         }
