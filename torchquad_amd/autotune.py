@@ -13,7 +13,7 @@ import threading
 
 import torch
 
-TILE_CANDIDATES = (8192, 16384, 32768, 65536, 131072)
+TILE_CANDIDATES = (16384, 32768, 65536, 131072)
 BLOCK_CANDIDATES = (1024, 2048, 4096)
 
 _cache = {}
