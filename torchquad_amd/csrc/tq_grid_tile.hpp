@@ -212,66 +212,7 @@ __device__ inline double grid_tile(int d, int n, int m, int L, int rule, int lan
             }
         }
     }
-    // row state and weight from the row tables, and the odometer step
-    auto rowtab_state = [&](State<T>& s_row, double& w_row) {
-        constexpr int S1 = TabStride<FAM>::v - 1;
-        s_row = F::start(p);
-        w_row = w_lane;
-#pragma unroll
-        for (int k = 0; k < TQ_MAX_DIM; ++k) {
-            if (k < L) {
-                const int e = k * n + rdig[k];
-                State<T> term;
-                term.s0 = rt[S1 * e];
-                term.s1 = (S1 == 2) ? rt[S1 * e + 1] : T(0);
-                s_row = F::merge(s_row, term);
-                w_row *= rw[e];
-            }
-        }
-    };
-    auto next_row = [&]() {
-#pragma unroll
-        for (int k = TQ_MAX_DIM - 1; k >= 0; --k) {
-            if (k < L) {
-                if (++rdig[k] < n) break;
-                rdig[k] = 0;
-            }
-        }
-    };
     while (t < t_end) {
-        if (ROWTAB && TABLE && j == 0 && t_end - t >= 2LL * n) {
-            // two whole rows at once: every loop-axis table entry feeds a point
-            // of each; each row's partial sums keep the single-row order
-            State<T> sA, sB;
-            double wA, wB;
-            rowtab_state(sA, wA);
-            next_row();
-            rowtab_state(sB, wB);
-            sA = F::merge(sA, s_lane);
-            sB = F::merge(sB, s_lane);
-            constexpr int S = TabStride<FAM>::v;
-            double raccA = 0.0, raccB = 0.0;
-            for (int jb = 0; jb < n; jb += kFlush) {
-                const int je = (jb + kFlush < n) ? (jb + kFlush) : n;
-                T faA = T(0), faB = T(0);
-#pragma unroll 4
-                for (int jj = jb; jj < je; ++jj) {
-                    State<T> term;
-                    term.s0 = tab[S * jj];
-                    term.s1 = (S == 3) ? tab[S * jj + 1] : T(0);
-                    const T wL = tab[S * jj + S - 1];
-                    faA = fma(wL, F::finish(F::merge(sA, term), p), faA);
-                    faB = fma(wL, F::finish(F::merge(sB, term), p), faB);
-                }
-                raccA += (double)faA;
-                raccB += (double)faB;
-            }
-            acc = fma(wA, raccA, acc);
-            acc = fma(wB, raccB, acc);
-            t += 2LL * n;
-            next_row();
-            continue;
-        }
         State<T> s_row = F::start(p);
         double w_row = w_lane;
         if (ROWTAB) {
@@ -327,7 +268,13 @@ __device__ inline double grid_tile(int d, int n, int m, int L, int rule, int lan
         j = jend;
         if (j == n) {
             j = 0;
-            next_row();
+#pragma unroll
+            for (int k = TQ_MAX_DIM - 1; k >= 0; --k) {
+                if (k < L) {
+                    if (++rdig[k] < n) break;
+                    rdig[k] = 0;
+                }
+            }
         }
     }
     return acc;
