@@ -13,7 +13,7 @@
 //   dI/dbeta_k = -2 a_k^2 S E_k,  dI/dalpha_k = I/alpha_k - 2 a_k^2 S R_k,
 //   dI/dlo_k = (dI/dbeta_k - dI/dalpha_k)/2,  dI/dhi_k = (dI/dbeta_k + dI/dalpha_k)/2.
 // The backward kernel walks the same tiles as the forward one and forms
-// these sums per point in FP32 (flushed to FP64 every 32 points).
+// these sums per point in FP32 (flushed to FP64 at least every 128 points).
 #include "tq_grid_tile.hpp"
 
 namespace tq {

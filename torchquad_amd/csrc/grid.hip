@@ -10,7 +10,7 @@
 // A wave owns one group of 64 lane combinations and a contiguous range of
 // (row, loop-index) items.  Per point the kernel does: fold of the loop axis
 // into the shared row+lane partial, the family's finish (transcendental),
-// and one FMA into an FP32 partial that is flushed to FP64 every 32 points.
+// and one FMA into an FP32 partial that is flushed to FP64 at least every 128 points.
 // Block partials are FP64 and the final sum runs in a fixed order, so the
 // result is deterministic.
 #include "tq_host.hpp"

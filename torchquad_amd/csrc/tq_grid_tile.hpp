@@ -5,7 +5,7 @@
 
 namespace tq {
 
-constexpr int kFlush = 32;       // FP32 partial length before FP64 flush
+constexpr int kFlush = 128;      // FP32 partial length before FP64 flush (at most 128 terms)
 constexpr int kLoopTable = 2048;  // largest n kept in the LDS loop table
 
 // Per-axis affine map.  Newton-Cotes: x_j = a + j*h (x_{n-1} = b exactly, as
