@@ -240,6 +240,9 @@ def bench_serving(dev, rate=2000.0, count=4000):
     tickets = serve_open_loop(server, [req(it) for it in items], rate=rate, seed=3)
     wall = time.perf_counter() - t0
     st = latency_stats(tickets)
+    gl_code = 3
+    st_gl = latency_stats([t for t in tickets if t.rule == gl_code])
+    st_nc = latency_stats([t for t in tickets if t.rule != gl_code])
     evals = sum(t.evals for t in tickets)
     # saturation: every request submitted at once
     server.completed.clear()
@@ -254,6 +257,8 @@ def bench_serving(dev, rate=2000.0, count=4000):
         "p99_ms": st.get("p99_ms"),
         "mean_ms": st.get("mean_ms"),
         "max_ms": st.get("max_ms"),
+        "gauss_legendre_p50_p99_ms": [st_gl.get("p50_ms"), st_gl.get("p99_ms")],
+        "newton_cotes_p50_p99_ms": [st_nc.get("p50_ms"), st_nc.get("p99_ms")],
         "evals_per_s_at_rate": evals / wall,
         "batches": server.batches_launched,
         "saturated_evals_per_s": sum(t.evals for t in tickets2) / wall2,

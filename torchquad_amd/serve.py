@@ -77,6 +77,7 @@ class Ticket:
         self.error = None
         self.cancelled = False
         self.state = None  # VEGAS requests: the run's state_dict(), for a later warm start
+        self.rule = None  # grid requests: rule code
         self._event = threading.Event()
 
     def done(self):
@@ -219,7 +220,8 @@ class IntegrationServer:
         self._next_slot = 0
         self._gl_cache = OrderedDict()
         self._gl_cache_bytes = 0
-        self._gl_stream = None
+        self._gl_streams = []  # tables of new n are generated round-robin on these
+        self._gl_next = 0
         self.gl_cache_limit = 256 << 20
         self._thread = None
         self._stop = threading.Event()
@@ -264,6 +266,7 @@ class IntegrationServer:
         gl_n = int(n) if code == _lib.RULE_GAUSS_LEGENDRE else 0
         nbytes = (desc_bytes.value + n_tiles.value * (TILE_BYTES + 8) + 16 + 16 + ITEM_BYTES + 16 * gl_n)
         t = Ticket("grid", nbytes, int(n) ** dim)
+        t.rule = code
         if arrival is not None:
             t.t_arrival = arrival
         self._enqueue(_GridRequest(t, item, gl_n))
@@ -386,12 +389,14 @@ class IntegrationServer:
         if e is not None:
             self._gl_cache.move_to_end(n)
             return e
-        if self._gl_stream is None:
-            self._gl_stream = torch.cuda.Stream(device=self.device)
-        with torch.cuda.stream(self._gl_stream):
+        if not self._gl_streams:
+            self._gl_streams = [torch.cuda.Stream(device=self.device) for _ in range(4)]
+        stream = self._gl_streams[self._gl_next % len(self._gl_streams)]
+        self._gl_next += 1
+        with torch.cuda.stream(stream):
             x, w = gauss_legendre(n, device=self.device)
             ev = torch.cuda.Event()
-            ev.record(self._gl_stream)
+            ev.record(stream)
         e = (x, w, ev)
         self._gl_cache[n] = e
         self._gl_cache_bytes += 16 * n
