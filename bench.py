@@ -236,9 +236,41 @@ def bench_serving(dev, rate=2000.0, count=4000):
     # warm-up: Gauss-Legendre tables, pinned buffers, kernels
     serve_open_loop(server, [req(it) for it in items[:200]], rate=1e6, seed=2)
     server.completed.clear()
+    # host-side pauses during the run: garbage-collector passes and the
+    # longest scheduling round
+    import gc
+
+    gc_pauses = []
+    gc_t = [0.0]
+
+    def gc_cb(phase, info):
+        if phase == "start":
+            gc_t[0] = time.perf_counter()
+        else:
+            gc_pauses.append((info.get("generation"), time.perf_counter() - gc_t[0]))
+
+    step_max = [0.0]
+    orig_step = server.step
+
+    def timed_step():
+        t = time.perf_counter()
+        r = orig_step()
+        step_max[0] = max(step_max[0], time.perf_counter() - t)
+        return r
+
+    server.step = timed_step
+    # as a serving process does after start-up: long-lived objects leave the
+    # collector's generations, so a full pass during serving does not walk
+    # the whole heap (a 37 ms pause otherwise set the p99 in this process)
+    gc.collect()
+    gc.freeze()
+    gc.callbacks.append(gc_cb)
     t0 = time.perf_counter()
     tickets = serve_open_loop(server, [req(it) for it in items], rate=rate, seed=3)
     wall = time.perf_counter() - t0
+    gc.callbacks.remove(gc_cb)
+    gc.unfreeze()
+    server.step = orig_step
     st = latency_stats(tickets)
     gl_code = 3
     st_gl = latency_stats([t for t in tickets if t.rule == gl_code])
@@ -259,6 +291,11 @@ def bench_serving(dev, rate=2000.0, count=4000):
         "max_ms": st.get("max_ms"),
         "gauss_legendre_p50_p99_ms": [st_gl.get("p50_ms"), st_gl.get("p99_ms")],
         "newton_cotes_p50_p99_ms": [st_nc.get("p50_ms"), st_nc.get("p99_ms")],
+        "gc_frozen_at_start": True,
+        "gc_passes": len(gc_pauses),
+        "gc_pause_max_ms": max((p for _, p in gc_pauses), default=0.0) * 1e3,
+        "gc_gen2_passes": sum(1 for g, _ in gc_pauses if g == 2),
+        "longest_round_ms": step_max[0] * 1e3,
         "evals_per_s_at_rate": evals / wall,
         "batches": server.batches_launched,
         "saturated_evals_per_s": sum(t.evals for t in tickets2) / wall2,

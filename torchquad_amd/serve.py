@@ -27,6 +27,11 @@ on the other integrals of the batch); a VEGAS integral's value depends only
 on its own arguments and seed.  So every result is the same alone and in
 any mix of other requests.
 
+Latency: the loop runs in Python, so a full pass of the cyclic garbage
+collector over a large heap stalls it (tens of ms in a process holding
+hundreds of thousands of objects).  A serving process should call
+``gc.collect(); gc.freeze()`` once its long-lived state is built.
+
 The reference has no serving layer; the integrals it serves are those of
 Trapezoid/Simpson/Boole/GaussLegendre.integrate (trapezoid.py:12,
 simpson.py:14, boole.py:14, gaussian.py:29) and VEGAS.integrate
