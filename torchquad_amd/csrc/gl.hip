@@ -13,10 +13,10 @@
 // and weights w_k = 2 / (1 - x_k^2) / P_n'(x_k)^2 = 2 / (dP_n/dtheta)^2.
 // For n >= 200 the boundary nodes (n sin(theta) < 50, about 16 per end) use
 // the same recurrence, spread over a workgroup: the step (D_{j-1}, P_{j-1})
-// -> (D_j, P_j) is linear, so each of the 1024 lanes forms the 2x2 product of
-// its contiguous range of j, and an ordered tree (wave shuffles, then LDS
-// across waves) multiplies the 1024 products -- O(n/1024 + log) steps
-// instead of O(n) on one lane.
+// -> (D_j, P_j) is linear, so each of the 256 (1024 for n >= 16384) lanes
+// forms the 2x2 product of its contiguous range of j, and an ordered tree
+// (wave shuffles, then LDS across waves) multiplies the products --
+// O(n/lanes + log) steps instead of O(n) on one lane.
 // Interior nodes of large n: Newton in theta on the Stieltjes asymptotic
 // expansion used by Hale & Townsend (SIAM J. Sci. Comput. 35(2), 2013)
 //     P_n(cos t) = C_n sum_m h_m cos(a_m) / (2 sin t)^(m+1/2),
@@ -47,15 +47,17 @@ __device__ inline void legendre_theta(int n, double t, double* pn, double* dpdt)
 // P_n(cos t) and dP_n/dt by the workgroup-parallel product of the
 // recurrence's transfer matrices.  Uniform in t across the workgroup;
 // s_mat holds 4 doubles per wave.
-constexpr int kBoundaryBlock = 1024;  // lanes per boundary node
-constexpr int kBoundaryWaves = kBoundaryBlock / kWave;
+// lanes per boundary node: 256, or 1024 for long recurrences (n >= 16384)
+constexpr int kBoundaryLongN = 16384;
 
+template <int BLOCK>
 __device__ inline void legendre_theta_block(int n, double t, double* s_mat, double* pn, double* dpdt) {
+    constexpr int kBoundaryWaves = BLOCK / kWave;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
     const double sh = sin(0.5 * t);
     const double z = 2.0 * sh * sh;
     const long long steps = (long long)n - 1;  // j = 2..n
-    const long long c = (steps + kBoundaryBlock - 1) / kBoundaryBlock;
+    const long long c = (steps + BLOCK - 1) / BLOCK;
     const long long jlo = 2 + (long long)tid * c;
     const long long jhi = (jlo + c - 1 < (long long)n) ? (jlo + c - 1) : (long long)n;
     double p00 = 1.0, p01 = 0.0, p10 = 0.0, p11 = 1.0;
@@ -153,18 +155,19 @@ __device__ inline double tricomi_theta(int n, int k) {
 }
 
 // Boundary nodes k < nb of n >= kAsymMinN: one workgroup per node.
-__global__ __launch_bounds__(kBoundaryBlock) void k_gl_boundary(int n, double* xs, double* ws) {
-    __shared__ double s_mat[4 * kBoundaryWaves];
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_gl_boundary(int n, double* xs, double* ws) {
+    __shared__ double s_mat[4 * (BLOCK / kWave)];
     const int k = blockIdx.x;
     double t = tricomi_theta(n, k);
     double pn, dp;
     for (int it = 0; it < 16; ++it) {
-        legendre_theta_block(n, t, s_mat, &pn, &dp);
+        legendre_theta_block<BLOCK>(n, t, s_mat, &pn, &dp);
         const double dt = pn / dp;
         t -= dt;
         if (fabs(dt) <= 1e-11 * t) break;
     }
-    legendre_theta_block(n, t, s_mat, &pn, &dp);
+    legendre_theta_block<BLOCK>(n, t, s_mat, &pn, &dp);
     if (threadIdx.x == 0) {
         const double x = cos(t);
         const double w = 2.0 / (dp * dp);
@@ -237,8 +240,11 @@ extern "C" int tq_gauss_legendre(int32_t n, double* nodes, double* weights, void
     if (n >= kAsymMinN) {
         for (int j = 1; j <= n; ++j) cn *= (long double)j / ((long double)j + 0.5L);
         while (nb < half && (double)n * sin(M_PI * (nb + 0.75) / (n + 0.5)) < kAsymMinNSin) ++nb;
-        hipLaunchKernelGGL(k_gl_boundary, dim3(nb), dim3(kBoundaryBlock), 0, as_stream(stream), (int)n, nodes,
-                           weights);
+        if (n >= kBoundaryLongN)
+            hipLaunchKernelGGL((k_gl_boundary<1024>), dim3(nb), dim3(1024), 0, as_stream(stream), (int)n, nodes, weights);
+        else
+            hipLaunchKernelGGL((k_gl_boundary<kBlock>), dim3(nb), dim3(kBlock), 0, as_stream(stream), (int)n, nodes,
+                               weights);
         int rc = check_launch("k_gl_boundary");
         if (rc) return rc;
     }
