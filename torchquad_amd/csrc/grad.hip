@@ -25,8 +25,14 @@ struct SweepGeom {
     long long blocks_x;  // blocks per integral
 };
 
-static int sweep_geom(const tq_grid_desc* gd, SweepGeom& S) {
-    int rc = plan_geom(gd->dim, gd->n, gd->rule, gd->lane_axes, gd->items_per_wave, 256, S.g);
+// Waves per launch when items_per_wave is left to the planner: about 16
+// per SIMD over the whole sweep, so each wave's fixed cost (parameters,
+// loop-axis table, 1 + 3 dim block sums) is spread over many points.
+constexpr long long kSweepWaves = 16384;
+
+static int sweep_geom(const tq_grid_desc* gd, long long B, SweepGeom& S) {
+    const long long per_integral = B > 0 ? (kSweepWaves + B - 1) / B : kSweepWaves;
+    int rc = plan_geom(gd->dim, gd->n, gd->rule, gd->lane_axes, gd->items_per_wave, per_integral, S.g);
     if (rc) return rc;
     S.blocks_x = (S.g.num_waves + kWavesPerBlock - 1) / kWavesPerBlock;
     return TQ_OK;
@@ -113,18 +119,24 @@ __device__ inline double node_r(int j, int n, int rule, const double* glx) {
     return (n > 1) ? 2.0 * (double)j / (double)(n - 1) - 1.0 : 0.0;
 }
 
+// One loop-axis table entry of the backward kernel: exponent term
+// -(a x + b)^2 in log2 units, weight, t = x - u, normalised node r.
+template <typename T>
+struct alignas(4 * sizeof(T) < 16 ? 4 * sizeof(T) : 16) BwdEntry {
+    T term, w, t, r;
+};
+
 // Backward tile for D axes: flat walk of the tile's points (lane axes per
-// lane, the rest decoded per row), full per-point accumulation.
-template <typename T, int D>
-__device__ inline void sweep_bwd_tile(const GridGeom& g, const Params<T>& p, const Axis* ax, const T* tab,
+// lane, the rest decoded per row).  Only the loop axis is accumulated per
+// point; row axes are accumulated per row and lane axes once per tile.
+template <typename T, int D, bool TABLE>
+__device__ inline void sweep_bwd_tile(const GridGeom& g, const Params<T>& p, const Axis* ax, const BwdEntry<T>* tab,
                                       const double* glx, const double* glw, int lw, long long t_begin,
                                       long long t_end, double* accF, double* accE, double* accQ, double* accR) {
     const int n = g.n, m = g.m, L = g.L, rule = g.rule;
     const int lane = threadIdx.x & (kWave - 1);
     // per-point coordinates of the current point, per axis
-    T xk[D], rk[D], wk[D];
-    bool active = true;
-    int li = 0;
+    T xk[D], rk[D];
     if (m == 0) {
         // single axis split: node 64 t + lane
         for (long long t = t_begin; t < t_end; ++t) {
@@ -145,20 +157,21 @@ __device__ inline void sweep_bwd_tile(const GridGeom& g, const Params<T>& p, con
         }
         return;
     }
-    li = lw * kWave + lane;
-    active = li < g.lane_count;
+    const int li = lw * kWave + lane;
+    const bool active = li < g.lane_count;
     T w_lane = active ? T(1) : T(0);
     {
         int rem = active ? li : 0;
 #pragma unroll
         for (int k = D - 1; k >= 0; --k) {
+            xk[k] = T(0);
+            rk[k] = T(0);
             if (k >= D - m) {
                 const int j = rem % n;
                 rem /= n;
                 xk[k] = (T)node_of(ax[k], j, n, rule, glx);
                 rk[k] = (T)node_r(j, n, rule, glx);
-                wk[k] = (T)weight_of(rule, j, n, glw);
-                w_lane *= wk[k];
+                w_lane *= (T)weight_of(rule, j, n, glw);
             }
         }
     }
@@ -198,54 +211,52 @@ __device__ inline void sweep_bwd_tile(const GridGeom& g, const Params<T>& p, con
             }
         }
     }
-    T aL = T(0), bL = T(0), uL = T(0);
+    T aL = T(0), bL = T(0), uL = T(0), s_lane = T(0);
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         if (k == L) {
             aL = p.a[k];
             bL = p.b[k];
             uL = p.u[k];
-            xk[k] = T(0);
-            rk[k] = T(0);
+        }
+        if (k > L) {
+            const T e = fma(xk[k], p.a[k], p.b[k]);
+            s_lane = fma(-e, e, s_lane);
         }
     }
     // Along a row every axis but L is fixed, so for k != L the sums over the
     // row factor: sum W f t_k = t_k F_row, sum W f t_k^2 = t_k^2 F_row,
-    // sum W f t_k r_k = t_k r_k F_row.  Only the loop axis is accumulated
-    // per point, from its table entry (exponent term, weight, t, r).
+    // sum W f t_k r_k = t_k r_k F_row.  Lane axes (k > L) keep their node for
+    // the whole tile, so theirs are t_k F_tile etc., formed after the walk.
+    double F_tile = 0.0, EL = 0.0, QL = 0.0, RL = 0.0;  // loop-axis sums
     while (t < t_end) {
         double w_row = (double)w_lane;
+        T s_fix = s_lane;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             if (k < L) {
                 xk[k] = (T)node_of(ax[k], rdig[k], n, rule, glx);
                 rk[k] = (T)node_r(rdig[k], n, rule, glx);
                 w_row *= weight_of(rule, rdig[k], n, glw);
-            }
-        }
-        // exponent and t of the fixed axes
-        T s_fix = T(0), tk[D];
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            tk[k] = xk[k] - p.u[k];
-            if (k != L) {
                 const T e = fma(xk[k], p.a[k], p.b[k]);
                 s_fix = fma(-e, e, s_fix);
             }
         }
         const long long left = t_end - t;
         const int jend = (left < (long long)(n - j)) ? (j + (int)left) : n;
+        double Frow = 0.0;
         for (int jb = j; jb < jend; jb += kFlush) {
             const int je = (jb + kFlush < jend) ? (jb + kFlush) : jend;
             T fF = T(0), fE = T(0), fQ = T(0), fR = T(0);
 #pragma unroll 4
             for (int jj = jb; jj < je; ++jj) {
                 T term, wL, tL, rL;
-                if (tab) {
-                    term = tab[4 * jj];
-                    wL = tab[4 * jj + 1];
-                    tL = tab[4 * jj + 2];
-                    rL = tab[4 * jj + 3];
+                if (TABLE) {
+                    const BwdEntry<T> en = tab[jj];
+                    term = en.term;
+                    wL = en.w;
+                    tL = en.t;
+                    rL = en.r;
                 } else {
                     const T xL = (T)node_of(ax[L], jj, n, rule, glx);
                     const T e = fma(xL, aL, bL);
@@ -261,20 +272,20 @@ __device__ inline void sweep_bwd_tile(const GridGeom& g, const Params<T>& p, con
                 fQ = fma(wt, tL, fQ);
                 fR = fma(wt, rL, fR);
             }
-            const double Fr = w_row * (double)fF;
-            accF[0] += Fr;
+            Frow += w_row * (double)fF;
+            EL += w_row * (double)fE;
+            QL += w_row * (double)fQ;
+            RL += w_row * (double)fR;
+        }
+        F_tile += Frow;
 #pragma unroll
-            for (int k = 0; k < D; ++k) {
-                if (k == L) {
-                    accE[k] += w_row * (double)fE;
-                    accQ[k] += w_row * (double)fQ;
-                    accR[k] += w_row * (double)fR;
-                } else {
-                    const double tt = (double)tk[k];
-                    accE[k] += Fr * tt;
-                    accQ[k] += Fr * tt * tt;
-                    accR[k] += Fr * tt * (double)rk[k];
-                }
+        for (int k = 0; k < D; ++k) {
+            if (k < L) {
+                const double tt = (double)(xk[k] - p.u[k]);
+                const double Ft = Frow * tt;
+                accE[k] += Ft;
+                accQ[k] += Ft * tt;
+                accR[k] += Ft * (double)rk[k];
             }
         }
         t += jend - j;
@@ -290,16 +301,31 @@ __device__ inline void sweep_bwd_tile(const GridGeom& g, const Params<T>& p, con
             }
         }
     }
+    accF[0] += F_tile;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        if (k == L) {
+            accE[k] += EL;
+            accQ[k] += QL;
+            accR[k] += RL;
+        }
+        if (k > L) {
+            const double tt = (double)(xk[k] - p.u[k]);
+            const double Ft = F_tile * tt;
+            accE[k] += Ft;
+            accQ[k] += Ft * tt;
+            accR[k] += Ft * (double)rk[k];
+        }
+    }
 }
 
 // partial layout per (integral, block): [F, E_0..E_{D-1}, Q_0.., R_0..]
-template <typename T, int D>
+template <typename T, int D, bool TABLE>
 __global__ __launch_bounds__(kBlock) void k_sweep_bwd(SweepGeom S, const T* a, const T* u, const T* dom,
                                                      const double* glx, const double* glw, double* partials) {
-    // loop-axis table (dynamic LDS, 4 T per node): exponent term
-    // -(a x + b)^2 in log2 units, weight, t = x - u, normalised node r
+    // loop-axis table in dynamic LDS, one BwdEntry per node
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* s_tab = reinterpret_cast<T*>(smem_raw);
+    BwdEntry<T>* s_tab = reinterpret_cast<BwdEntry<T>*>(smem_raw);
     __shared__ Params<T> s_p;
     __shared__ Axis s_axis[TQ_MAX_DIM];
     __shared__ double s_red[kWavesPerBlock];
@@ -307,19 +333,20 @@ __global__ __launch_bounds__(kBlock) void k_sweep_bwd(SweepGeom S, const T* a, c
     const long long b = blockIdx.y;
     load_sweep_params(g.d, g.n, g.rule, b, a, u, dom, &s_p, s_axis);
     __syncthreads();
-    const bool table = g.m > 0 && g.L >= 0 && g.n <= kBwdTable;
-    if (table) {
+    if (TABLE) {
         const int L = g.L;
         for (int j = threadIdx.x; j < g.n; j += kBlock) {
             const T x = (T)node_of(s_axis[L], j, g.n, g.rule, glx);
             const T e = fma(x, s_p.a[L], s_p.b[L]);
-            s_tab[4 * j] = -e * e;
-            s_tab[4 * j + 1] = (T)weight_of(g.rule, j, g.n, glw);
-            s_tab[4 * j + 2] = x - s_p.u[L];
-            s_tab[4 * j + 3] = (T)node_r(j, g.n, g.rule, glx);
+            BwdEntry<T> en;
+            en.term = -e * e;
+            en.w = (T)weight_of(g.rule, j, g.n, glw);
+            en.t = x - s_p.u[L];
+            en.r = (T)node_r(j, g.n, g.rule, glx);
+            s_tab[j] = en;
         }
+        __syncthreads();
     }
-    __syncthreads();
     double F[1] = {0.0}, E[D], Q[D], R[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) E[k] = Q[k] = R[k] = 0.0;
@@ -329,7 +356,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_bwd(SweepGeom S, const T* a, c
         const long long chunk = wave / g.lane_waves;
         const long long t0 = chunk * g.ipw;
         const long long t1 = (t0 + g.ipw < g.items) ? (t0 + g.ipw) : g.items;
-        sweep_bwd_tile<T, D>(g, s_p, s_axis, table ? s_tab : nullptr, glx, glw, lw, t0, t1, F, E, Q, R);
+        sweep_bwd_tile<T, D, TABLE>(g, s_p, s_axis, s_tab, glx, glw, lw, t0, t1, F, E, Q, R);
     }
     const int nv = 1 + 3 * D;
     double* out = partials + (b * gridDim.x + blockIdx.x) * nv;
@@ -425,7 +452,7 @@ template <typename T>
 static int launch_sweep_fwd(const tq_grid_desc* gd, int64_t B, const void* a, const void* u, const void* dom,
                             void* ws, size_t wsb, void* out, double* out64, hipStream_t s) {
     SweepGeom S;
-    int rc = sweep_geom(gd, S);
+    int rc = sweep_geom(gd, B, S);
     if (rc) return rc;
     if (!ws || wsb < (size_t)(S.blocks_x * B) * sizeof(double)) return fail(TQ_E_WORKSPACE, "sweep workspace too small");
     if (S.blocks_x > 0x7fffffff) return fail(TQ_E_ARG, "too many blocks");
@@ -451,9 +478,12 @@ template <typename T, int D>
 static int launch_bwd_d(const SweepGeom& S, int64_t B, const void* a, const void* u, const void* dom,
                         const tq_grid_desc* gd, void* ws, hipStream_t s) {
     dim3 grid((unsigned)S.blocks_x, (unsigned)B);
-    const size_t lds = (S.g.m > 0 && S.g.L >= 0 && S.g.n <= kBwdTable) ? (size_t)4 * S.g.n * sizeof(T) : 0;
-    hipLaunchKernelGGL((k_sweep_bwd<T, D>), grid, dim3(kBlock), lds, s, S, (const T*)a, (const T*)u, (const T*)dom,
-                       gd->gl_nodes, gd->gl_weights, (double*)ws);
+    if (S.g.m > 0 && S.g.L >= 0 && S.g.n <= kBwdTable)
+        hipLaunchKernelGGL((k_sweep_bwd<T, D, true>), grid, dim3(kBlock), (size_t)S.g.n * sizeof(BwdEntry<T>), s, S,
+                           (const T*)a, (const T*)u, (const T*)dom, gd->gl_nodes, gd->gl_weights, (double*)ws);
+    else
+        hipLaunchKernelGGL((k_sweep_bwd<T, D, false>), grid, dim3(kBlock), 0, s, S, (const T*)a, (const T*)u,
+                           (const T*)dom, gd->gl_nodes, gd->gl_weights, (double*)ws);
     return check_launch("k_sweep_bwd");
 }
 
@@ -462,7 +492,7 @@ static int launch_sweep_bwd(const tq_grid_desc* gd, int64_t B, const void* a, co
                             const void* grad_out, void* ws, size_t wsb, void* ga, void* gu, void* gdom,
                             hipStream_t s) {
     SweepGeom S;
-    int rc = sweep_geom(gd, S);
+    int rc = sweep_geom(gd, B, S);
     if (rc) return rc;
     const int D = gd->dim;
     if (!ws || wsb < (size_t)(S.blocks_x * B * (1 + 3 * D)) * sizeof(double))
@@ -494,7 +524,7 @@ extern "C" {
 size_t tq_sweep_workspace_size(const tq_grid_desc* g, int64_t B) {
     if (check_sweep(g, B)) return 0;
     SweepGeom S;
-    if (sweep_geom(g, S)) return 0;
+    if (sweep_geom(g, B, S)) return 0;
     return (size_t)(S.blocks_x * B * (1 + 3 * g->dim)) * sizeof(double);
 }
 
