@@ -37,6 +37,8 @@ extern "C" {
 /* element types */
 #define TQ_F32 0
 #define TQ_F64 1
+#define TQ_I32 2 /* tq_add_at_indices only */
+#define TQ_I64 3 /* tq_add_at_indices only */
 
 /* quadrature rules (reference: trapezoid.py, simpson.py, boole.py, gaussian.py) */
 #define TQ_RULE_TRAPEZOID 0
@@ -354,6 +356,21 @@ int tq_vegas_strat_accumulate(const tq_vegas_state* st, const double* w, int32_t
 int tq_vegas_update_dh(const tq_vegas_state* st, int32_t it, void* stream);
 /* dh <- dh / iter_stats[it].dh_sum when the sum is non-zero. */
 int tq_vegas_normalize_dh(const tq_vegas_state* st, int32_t it, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Scatter-add: target[indices[i]] += source[i] for i < count, in place.
+ * Replaces _add_at_indices (torchquad/integration/utils.py:61-107; its torch
+ * branch is target.scatter_add_(0, indices, source), :75-76).  dtype:
+ * TQ_F32, TQ_F64, TQ_I32 or TQ_I64 for target and source; indices int64.
+ * Each wave sorts its 64 (index, value) pairs (skipped when they are
+ * already ascending), sums runs of equal indices and adds each run with one
+ * atomic.  Integer results are exact; float results depend on the order in
+ * which waves reach a shared entry, as with scatter_add_ on a GPU.
+ * Indices outside [0, target_len) are skipped; if status is not NULL the
+ * kernel then sets *status (device int32, zero on entry) to 1.
+ * ------------------------------------------------------------------------ */
+int tq_add_at_indices(int32_t dtype, void* target, int64_t target_len, const int64_t* indices,
+                      const void* source, int64_t count, int32_t* status, void* stream);
 
 #ifdef __cplusplus
 }

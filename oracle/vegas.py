@@ -7,6 +7,7 @@ Follows torchquad/integration:
   vegas_map.py:185-261   update_map (fp64 cumsum, add-at, edge interpolation, repair)
   vegas_stratification.py:13-165   N_strat, get_NH, get_Y, accumulate_weight, update_DH
   vegas.py:30-362        driver: budget split, warmup, iterations, abort rules, EQ 30-32
+  utils.py:61-107        _add_at_indices (add_at_indices below)
 
 State is FP64 and samples use `dtype`, the precisions of the device
 implementation.  Random numbers come from the library's documented Philox
@@ -26,6 +27,26 @@ import numpy as np
 from . import native
 
 SB = 1024
+
+
+def add_at_indices(target, indices, source):
+    """target[indices[i]] += source[i] in place (torchquad/integration/utils.py:61-107).
+
+    Follows the reference's numpy branch (:78-107): order the pairs by index,
+    sum every run of equal indices, add each run's sum to its entry.  Pinned by
+    the reference's known answer (tests/utils_integration_test.py:109-115, in
+    tests/golden/reference_vectors.json), its identity and single-entry cases
+    (:92-104) and numpy's np.add.at on random inputs (tests/test_oracle.py).
+    """
+    idx = np.asarray(indices)
+    if idx.size == 0:
+        return target
+    order = np.argsort(idx, kind="stable")
+    idx = idx[order]
+    src = np.asarray(source)[order]
+    starts = np.flatnonzero(np.r_[True, idx[1:] != idx[:-1]])
+    target[idx[starts]] += np.add.reduceat(src, starts).astype(target.dtype)
+    return target
 
 
 def n_strat_for(N_increment, dim):

@@ -54,6 +54,13 @@ def test_invalid_arguments_return_error_codes():
     m.dtype, m.dim, m.N = 0, 0, 10
     assert L.tq_mc_workspace_size(ctypes.byref(m)) == 0
     assert L.tq_uniform(1, 0, 5, 1, 0, None, None) < 0
+    # scatter-add: argument checks return before any launch
+    fake = ctypes.c_void_p(256)
+    assert L.tq_add_at_indices(0, None, 3, None, None, 0, None, None) == 0  # nothing to add
+    assert L.tq_add_at_indices(0, fake, 3, None, fake, 4, None, None) == -2
+    assert L.tq_add_at_indices(0, fake, 3, fake, fake, -1, None, None) == -1
+    assert L.tq_add_at_indices(7, fake, 3, fake, fake, 4, None, None) == -1
+    assert b"dtype" in L.tq_last_error()
 
 
 def test_workspace_sizes_positive():
@@ -104,6 +111,21 @@ def test_calculate_result_missing_values_message():
                                         integration_domain=torch.tensor([[0.0, 1.0]]))
     with pytest.raises(ValueError, match="Please provide function_values"):
         tq.Trapezoid().calculate_result()
+
+
+@pytest.mark.parametrize("cls", ["Trapezoid", "Simpson", "Boole", "MonteCarlo"])
+def test_calculate_result_error_handling_all_integrators(cls):
+    # trapezoid_test.py:177-198, simpson_test.py:177-198, boole_test.py:148-169,
+    # monte_carlo_test.py:166-184
+    integ = getattr(tq, cls)()
+    kw = ({"integration_domain": torch.tensor([[0.0, 1.0]])} if cls == "MonteCarlo" else
+          {"dim": 1, "n_per_dim": 101, "hs": torch.tensor([0.01]), "integration_domain": torch.tensor([[0.0, 1.0]])})
+    with pytest.raises(ValueError) as exc_info:
+        integ.calculate_result(**kw)
+    assert "function_values argument not found" in str(exc_info.value)
+    assert "Please provide function_values" in str(exc_info.value)
+    with pytest.raises(ValueError, match="function_values argument not found"):
+        integ.calculate_result()
 
 
 def test_grid_checks():

@@ -96,3 +96,19 @@ def test_mc_gradients_reference_answers(ref_vectors, dev):
                        integration_domain=[[0.0, 1.0], [0.0, 2.0]], seed=0)
     val.backward()
     np.testing.assert_array_less(np.abs(param.grad.cpu().numpy() - np.array([2.0, 1.0, 8.0 / 3.0])), 5e-2)
+
+
+def test_split_api_kwargs(dev):
+    # monte_carlo_test.py:136-163: positional, keyword and mixed calls agree
+    dom = torch.tensor([[0.0, 1.0]], device=dev)
+    mc = tq.MonteCarlo()
+    pts = mc.calculate_sample_points(1000, dom, seed=42)
+    assert pts.shape == (1000, 1) and pts.device == dom.device
+    vals, _ = mc.evaluate_integrand(lambda x: torch.exp(x[:, 0]), pts)
+    r1 = mc.calculate_result(vals, dom)
+    r2 = mc.calculate_result(function_values=vals, integration_domain=dom)
+    r3 = mc.calculate_result(vals, integration_domain=dom)
+    assert torch.allclose(r1, r2, rtol=1e-10) and torch.allclose(r1, r3, rtol=1e-10)
+    # the split path and integrate() see the same points
+    whole = tq.MonteCarlo().integrate(lambda x: torch.exp(x[:, 0]), 1, N=1000, integration_domain=dom, seed=42)
+    assert float(whole) == float(r1)

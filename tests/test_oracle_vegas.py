@@ -98,3 +98,34 @@ def test_driver_peak_accuracy():
     r = ov.run(lambda x: fam.double_gaussian(x, 0.1), 3, 30000, [[0.0, 1.0]] * 3, seed=1)
     exact = fam.double_gaussian_integral(3, [0.0] * 3, [1.0] * 3)
     assert abs(r["result"] - exact) < 0.03 * exact
+
+
+def test_add_at_indices_known_answers(ref_vectors):
+    # utils_integration_test.py:92-115
+    A = ref_vectors["add_at_indices"]
+    t = np.array(A["target"], dtype=np.int64)
+    ov.add_at_indices(t, np.array(A["indices"]), np.array(A["source_values"], dtype=np.int64))
+    assert t.dtype == np.int64 and t.tolist() == A["expected"]
+    t = np.zeros(500)
+    ov.add_at_indices(t, np.arange(500), np.ones(500))
+    assert np.max(np.abs(t - 1.0)) == 0.0
+    t = np.zeros(500)
+    ov.add_at_indices(t, np.full(500, 203), np.ones(500))
+    assert t[203] == 500.0 and np.count_nonzero(t) == 1
+
+
+@pytest.mark.parametrize("n,m", [(1, 1), (1000, 7), (5000, 4096), (3000, 50)])
+def test_add_at_indices_vs_numpy_add_at(n, m):
+    rng = np.random.default_rng(n + m)
+    idx = rng.integers(0, m, n)
+    src = rng.integers(-1000, 1000, n).astype(np.int64)
+    t = rng.integers(-5, 5, m).astype(np.int64)
+    ref = t.copy()
+    np.add.at(ref, idx, src)
+    assert np.array_equal(ov.add_at_indices(t, idx, src), ref)
+    # a dropped run (first or last) or a wrong run boundary fails this too
+    f = rng.standard_normal(n)
+    tf = np.zeros(m)
+    reff = np.zeros(m)
+    np.add.at(reff, idx, f)
+    np.testing.assert_allclose(ov.add_at_indices(tf, idx, f), reff, rtol=1e-12, atol=1e-12)

@@ -270,3 +270,70 @@ def test_served_vegas_warm_start(dev):
     ref = float(tq.VEGAS().integrate(f, 3, N=60_000, integration_domain=dom, seed=2, N_intervals=20,
                                      warm_start=t1.state, use_warmup=False))
     assert t2.result() == ref
+
+
+def test_add_at_indices_reference_cases(ref_vectors, dev):
+    # utils_integration_test.py:83-115, through the HIP kernel
+    from torchquad_amd.integration.utils import _add_at_indices
+
+    for dtype in (torch.float32, torch.float64):
+        target = torch.zeros(500, dtype=dtype, device=dev)
+        indices = torch.arange(500, device=dev)
+        source = torch.ones(500, dtype=dtype, device=dev)
+        _add_at_indices(target, indices, source, is_sorted=True)
+        assert target.dtype == dtype and target.shape == (500,)
+        assert float(torch.max(torch.abs(target - source))) == 0.0
+        target.zero_()
+        _add_at_indices(target, indices * 0 + 203, source, is_sorted=True)
+        assert float(target[203]) == 500.0
+        target[203] = 0.0
+        assert float(torch.max(torch.abs(target))) == 0.0
+    A = ref_vectors["add_at_indices"]
+    target = torch.tensor(A["target"], device=dev)
+    _add_at_indices(target, torch.tensor(A["indices"], device=dev), torch.tensor(A["source_values"], device=dev))
+    assert target.dtype == torch.int64 and target.tolist() == A["expected"]
+
+
+@pytest.mark.parametrize("n,m,order", [(1, 1, "random"), (63, 5, "random"), (65, 3, "sorted"),
+                                       (100_003, 50, "random"), (100_003, 50, "sorted"),
+                                       (1_000_000, 117_649, "random"), (200_000, 1, "random")])
+def test_add_at_indices_vs_oracle(n, m, order, dev):
+    from torchquad_amd.integration.utils import _add_at_indices
+
+    rng = np.random.default_rng(n + m)
+    idx = rng.integers(0, m, n)
+    if order == "sorted":
+        idx = np.sort(idx)
+    # integers: bit-exact in every dtype (the float sums stay below 2^24)
+    src = rng.integers(-8, 9, n)
+    for dtype, npd in ((torch.int64, np.int64), (torch.int32, np.int32), (torch.float64, np.float64),
+                       (torch.float32, np.float32)):
+        t0 = rng.integers(-3, 4, m).astype(npd)
+        ref = ov.add_at_indices(t0.copy(), idx, src.astype(npd))
+        t = torch.tensor(t0, device=dev)
+        _add_at_indices(t, torch.tensor(idx, device=dev), torch.tensor(src.astype(npd), device=dev),
+                        is_sorted=order == "sorted")
+        assert np.array_equal(t.cpu().numpy(), ref), dtype
+    # real values: FP64 to 1e-12 of the entry's absolute sum
+    f = rng.standard_normal(n)
+    ref = ov.add_at_indices(np.zeros(m), idx, f)
+    scale = ov.add_at_indices(np.zeros(m), idx, np.abs(f)) + 1e-300
+    t = torch.zeros(m, dtype=torch.float64, device=dev)
+    _add_at_indices(t, torch.tensor(idx, device=dev), torch.tensor(f, device=dev))
+    assert np.max(np.abs(t.cpu().numpy() - ref) / scale) <= 1e-12
+
+
+def test_add_at_indices_errors(dev):
+    from torchquad_amd.integration.utils import _add_at_indices
+
+    t = torch.zeros(4, device=dev)
+    with pytest.raises(ValueError):
+        _add_at_indices(t, torch.tensor([0, 1], device=dev), torch.ones(2, dtype=torch.float64, device=dev))
+    with pytest.raises(ValueError):
+        _add_at_indices(t, torch.tensor([0, 1], device=dev), torch.ones(3, device=dev))
+    # out-of-range entries are skipped, the rest still land, and the call raises
+    with pytest.raises(IndexError):
+        _add_at_indices(t, torch.tensor([0, 4, -1, 3], device=dev), torch.ones(4, device=dev))
+    assert t.tolist() == [1.0, 0.0, 0.0, 1.0]
+    _add_at_indices(t, torch.tensor([], dtype=torch.int64, device=dev), torch.ones(0, device=dev))
+    assert t.tolist() == [1.0, 0.0, 0.0, 1.0]

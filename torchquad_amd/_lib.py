@@ -16,7 +16,7 @@ _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libtorchquad_amd.so"
 
 TQ_MAX_DIM = 16
-TQ_F32, TQ_F64 = 0, 1
+TQ_F32, TQ_F64, TQ_I32, TQ_I64 = 0, 1, 2, 3
 RULE_TRAPEZOID, RULE_SIMPSON, RULE_BOOLE, RULE_GAUSS_LEGENDRE = 0, 1, 2, 3
 
 FAM_GAUSSIAN_PEAK = 0
@@ -203,6 +203,7 @@ VEGAS_SIGNATURES = {
     "tq_vegas_strat_accumulate": (ctypes.c_int, [_S, _P, _I32, _P]),
     "tq_vegas_update_dh": (ctypes.c_int, [_S, _I32, _P]),
     "tq_vegas_normalize_dh": (ctypes.c_int, [_S, _I32, _P]),
+    "tq_add_at_indices": (ctypes.c_int, [_I32, _P, _I64, _P, _P, _I64, _P, _P]),
 }
 
 

@@ -117,6 +117,54 @@ def _is_marked(t):
     return ref() is t and p == t.data_ptr() and v == t._version
 
 
+_ADD_AT_DTYPES = {torch.float32: 0, torch.float64: 1, torch.int32: 2, torch.int64: 3}
+
+
+def _add_at_indices(target, indices, source, is_sorted=False):
+    """Add source[i] to target[indices[i]] for every i, in place (utils.py:61-107).
+
+    For example target=[0,0,0], indices=[2,1,1,2], source=[a,b,c,d] gives
+    [0, b+c, a+d].  target, indices and source are 1-D tensors on the GPU;
+    target and source share a dtype (float32, float64, int32 or int64) and
+    indices are integers.  One HIP kernel (tq_add_at_indices) does the work;
+    is_sorted is accepted for the reference's signature and needs no care:
+    each wave checks whether its indices ascend and sorts them if not.
+    Integer sums are exact; float sums depend on the order in which waves
+    reach a shared entry, as torch's scatter_add_ on a GPU does.
+
+    Raises:
+        ValueError: shapes, dtypes or devices that do not fit.
+        IndexError: an index outside [0, len(target)) (checked after the
+            kernel, which skips such entries; this synchronises the stream).
+    """
+    from .. import _lib
+
+    for name, t in (("target", target), ("indices", indices), ("source", source)):
+        if not isinstance(t, torch.Tensor):
+            raise ValueError(f"{name} must be a torch tensor")
+        _lib.require_device(t)
+    if target.dim() != 1 or indices.dim() != 1 or source.shape != indices.shape:
+        raise ValueError("expected a 1-D target and 1-D indices and source of one length")
+    if target.dtype != source.dtype or target.dtype not in _ADD_AT_DTYPES:
+        raise ValueError("target and source must share a dtype among float32, float64, int32 and int64")
+    if indices.dtype.is_floating_point or indices.dtype == torch.bool:
+        raise ValueError("indices must be an integer tensor")
+    if not (target.device == indices.device == source.device):
+        raise ValueError("target, indices and source must be on one device")
+    if indices.numel() == 0:
+        return
+    idx = indices.to(torch.int64).contiguous()
+    src = source.contiguous()
+    dst = target if target.is_contiguous() else target.contiguous()
+    status = torch.zeros(1, dtype=torch.int32, device=target.device)
+    _lib.call("tq_add_at_indices", _ADD_AT_DTYPES[target.dtype], _lib.ptr(dst), dst.numel(), _lib.ptr(idx),
+              _lib.ptr(src), idx.numel(), _lib.ptr(status), _lib.stream_ptr(target.device))
+    if dst is not target:
+        target.copy_(dst)
+    if int(status.item()):
+        raise IndexError(f"_add_at_indices: an index is outside [0, {target.numel()})")
+
+
 def _check_integration_domain(integration_domain):
     """Validate shape and bounds and return the dimension (utils.py:162-206)."""
     if not isinstance(integration_domain, torch.Tensor):
