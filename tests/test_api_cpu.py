@@ -10,7 +10,7 @@ import torch
 
 import torchquad_amd as tq
 from torchquad_amd import _lib
-from torchquad_amd.integration.utils import _setup_integration_domain
+from torchquad_amd.integration.utils import _linspace_with_grads, _setup_integration_domain
 
 HEADERS = sorted((Path(__file__).resolve().parent.parent / "include").glob("*.h"))
 
@@ -126,6 +126,23 @@ def test_calculate_result_error_handling_all_integrators(cls):
     assert "Please provide function_values" in str(exc_info.value)
     with pytest.raises(ValueError, match="function_values argument not found"):
         integ.calculate_result()
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("requires_grad", [False, True])
+def test_linspace_with_grads(dtype, requires_grad):
+    # utils_integration_test.py:47-80
+    start = torch.tensor(-2.0, dtype=dtype, requires_grad=requires_grad)
+    stop = torch.tensor(-1.0, dtype=dtype, requires_grad=requires_grad)
+    g = _linspace_with_grads(start, stop, 10, requires_grad)
+    assert g.dtype == dtype and g.shape == (10,)
+    assert g[0] == start and g[9] == stop
+    assert all(start <= g[i] <= stop for i in range(10))
+    assert all(g[i] < g[i + 1] for i in range(9))
+    if requires_grad:
+        g.sum().backward()
+        # d/dstart sum_i (start + i/9 (stop - start)) = 10 - 45/9 = 5, same for stop
+        assert abs(float(start.grad) - 5.0) < 1e-6 and abs(float(stop.grad) - 5.0) < 1e-6
 
 
 def test_grid_checks():
