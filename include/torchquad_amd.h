@@ -212,15 +212,22 @@ int tq_batch_plan_sizes(const tq_batch_item* items, int64_t count, int32_t dtype
  * wave (or no loop axis), 1: within 8 KB, 2: larger (general kernel). */
 int tq_batch_plan(const tq_batch_item* items, int64_t count, int32_t dtype, int64_t tile_points, void* desc_out,
                   tq_batch_tile* tiles_out, int64_t* tile_begin, int64_t* tile_count, int64_t* class_offsets);
-/* Runs a plan whose arrays were copied to the device: clears counter[0..2],
+/* Work counters of tq_batch_run, in 32-bit words: per class, 16 counters
+ * 4 KB apart, so that waves pulling tiles spread their atomics over 16
+ * memory locations instead of serialising on one. */
+#define TQ_BATCH_COUNTER_WORDS (3 * 16 * 1024)
+/* Runs a plan whose arrays were copied to the device: clears the counters,
  * one persistent kernel per non-empty class of `blocks` workgroups (0 =
  * automatic) pulling that class's tiles, then out[i] (dtype) / out64[i] =
- * integral i.  tile_partials: [n_tiles]; counter: [3]; class_offsets: host
- * [4] from tq_batch_plan, or NULL to run every tile on the general kernel. */
+ * integral i.  tile_partials: [n_tiles]; counter: counter_words >=
+ * TQ_BATCH_COUNTER_WORDS;
+ * class_offsets: host [4] from tq_batch_plan, or NULL to run every tile on
+ * the general kernel.  Each class's tiles form 16 interleaved queues (tile
+ * t in queue t mod 16); a wave drains its home queue, then the others. */
 int tq_batch_run(int32_t dtype, const void* desc, int64_t count, const tq_batch_tile* tiles, int64_t n_tiles,
                  const int64_t* tile_begin, const int64_t* tile_count, const double* gl_nodes,
-                 const double* gl_weights, double* tile_partials, unsigned int* counter, void* out, double* out64,
-                 int32_t blocks, const int64_t* class_offsets, void* stream);
+                 const double* gl_weights, double* tile_partials, unsigned int* counter, int64_t counter_words,
+                 void* out, double* out64, int32_t blocks, const int64_t* class_offsets, void* stream);
 
 /* ---- differentiation ----------------------------------------------------
  * Parameter sweep of the Gaussian peak f = exp(-sum a_k^2 (x_k-u_k)^2) over B

@@ -169,7 +169,7 @@ class Batch:
         self.tile_begin = torch.from_numpy(tb).to(dev)
         self.tile_count = torch.from_numpy(tc).to(dev)
         self.tile_part = torch.empty(max(self.n_tiles, 1), dtype=torch.float64, device=dev)
-        self.counter = torch.zeros(4, dtype=torch.int32, device=dev)
+        self.counter = torch.zeros(_lib.BATCH_COUNTER_WORDS, dtype=torch.int32, device=dev)
         self.tile_points = int(tile_points)
 
     def retile(self, tile_points=None, blocks=None):
@@ -183,7 +183,7 @@ class Batch:
         """Launch the batch on torch's current stream; returns (out, out64)."""
         call("tq_batch_run", self.dt, ptr(self.desc), self.count, ptr(self.tiles), self.n_tiles,
              ptr(self.tile_begin), ptr(self.tile_count), ptr(self.gl_x), ptr(self.gl_w), ptr(self.tile_part),
-             ptr(self.counter), ptr(self.out), ptr(self.out64), self.blocks,
+             ptr(self.counter), self.counter.numel(), ptr(self.out), ptr(self.out64), self.blocks,
              self.class_offsets.ctypes.data_as(ctypes.c_void_p), stream_ptr(self.device))
         return self.out, self.out64
 

@@ -46,8 +46,13 @@ inline long long rowtab_bytes(int fam, int n, int L, int tsize) {
     return (long long)L * n * 8 + (long long)S * n * tsize + (long long)(S - 1) * L * n * tsize;
 }
 
+// Tile queues per class and the distance between their counters (words).
+constexpr int kQueues = 16;
+constexpr int kQueueStride = 1024;
+static_assert(kBatchClasses * kQueues * kQueueStride == TQ_BATCH_COUNTER_WORDS, "counter layout");
+
 __global__ void k_batch_reset(unsigned int* counter) {
-    if (threadIdx.x < kBatchClasses) counter[threadIdx.x] = 0u;
+    if (threadIdx.x < kBatchClasses * kQueues) counter[threadIdx.x * kQueueStride] = 0u;
 }
 
 // General kernel: loop-axis table when it fits, row axes recomputed per row.
@@ -139,11 +144,23 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchDev<T>* descs, cons
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & (kWave - 1);
     unsigned char* region = reinterpret_cast<unsigned char*>(s_region[wave]);
+    // Tile t belongs to queue t % kQueues; a wave starts on its home queue and
+    // then drains the others, looking at a counter before adding to it.
+    const int home = (int)(((long long)blockIdx.x * kWavesPerBlock + wave) % kQueues);
+    for (int j = 0; j < kQueues; ++j) {
+        const int q = (home + j) % kQueues;
+        if (q >= n_tiles) continue;
+        const unsigned int nq = (unsigned int)((n_tiles - q + kQueues - 1) / kQueues);
+        unsigned int* cq = counter + (long long)q * kQueueStride;
     for (;;) {
-        unsigned int t = 0;
-        if (lane == 0) t = atomicAdd(counter, 1u);
-        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, kWave));
-        if ((long long)t >= n_tiles) break;
+        unsigned int i = nq;
+        if (lane == 0) {
+            if (j == 0 || __hip_atomic_load(cq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nq)
+                i = atomicAdd(cq, 1u);
+        }
+        i = __builtin_amdgcn_readfirstlane(__shfl(i, 0, kWave));
+        if (i >= nq) break;
+        const long long t = q + (long long)i * kQueues;
         const tq_batch_tile tile = tiles[t];
         const BatchDev<T>& D = descs[tile.item];
         const double* gx = glx ? glx + D.gl_offset : nullptr;
@@ -161,6 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchDev<T>* descs, cons
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
     }
 }
 
@@ -350,7 +368,7 @@ static int run_batch(const void* desc, long long count, const tq_batch_tile* til
                      const int64_t* tile_begin, const int64_t* tile_count, const double* glx, const double* glw,
                      double* tile_part, unsigned int* counter, void* out, double* out64, int blocks,
                      const int64_t* class_offsets, hipStream_t s) {
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counter);
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(kBatchClasses * kQueues), 0, s, counter);
     int rc = check_launch("k_batch_reset");
     if (rc) return rc;
     if (blocks <= 0) blocks = 2048;
@@ -362,13 +380,13 @@ static int run_batch(const void* desc, long long count, const tq_batch_tile* til
                 return fail(TQ_E_ARG, "class_offsets do not partition the tiles");
     }
     rc = launch_class<T, kRegionSmall>(desc, tiles + off[0], off[1] - off[0], glx, glw, tile_part + off[0],
-                                       counter + 0, blocks, s);
+                                       counter, blocks, s);
     if (!rc)
         rc = launch_class<T, kRegionBig>(desc, tiles + off[1], off[2] - off[1], glx, glw, tile_part + off[1],
-                                         counter + 1, blocks, s);
+                                         counter + kQueues * kQueueStride, blocks, s);
     if (!rc)
-        rc = launch_class<T, 0>(desc, tiles + off[2], off[3] - off[2], glx, glw, tile_part + off[2], counter + 2,
-                                blocks, s);
+        rc = launch_class<T, 0>(desc, tiles + off[2], off[3] - off[2], glx, glw, tile_part + off[2],
+                                counter + 2 * kQueues * kQueueStride, blocks, s);
     if (rc) return rc;
     const long long fb = (count + kWavesPerBlock - 1) / kWavesPerBlock;
     hipLaunchKernelGGL((k_batch_finalize<T>), dim3((unsigned)(fb > 0 ? fb : 1)), dim3(kBlock), 0, s,
@@ -412,10 +430,11 @@ int tq_batch_plan(const tq_batch_item* items, int64_t count, int32_t dtype, int6
 
 int tq_batch_run(int32_t dtype, const void* desc, int64_t count, const tq_batch_tile* tiles, int64_t n_tiles,
                  const int64_t* tile_begin, const int64_t* tile_count, const double* gl_nodes,
-                 const double* gl_weights, double* tile_partials, unsigned int* counter, void* out, double* out64,
-                 int32_t blocks, const int64_t* class_offsets, void* stream) {
+                 const double* gl_weights, double* tile_partials, unsigned int* counter, int64_t counter_words,
+                 void* out, double* out64, int32_t blocks, const int64_t* class_offsets, void* stream) {
     if (!desc || !tiles || !tile_begin || !tile_count || !tile_partials || !counter)
         return fail(TQ_E_NULL, "NULL argument");
+    if (counter_words < TQ_BATCH_COUNTER_WORDS) return fail(TQ_E_WORKSPACE, "counter needs TQ_BATCH_COUNTER_WORDS words");
     if (count < 0 || n_tiles < 0 || n_tiles > 0xffffffffLL) return fail(TQ_E_ARG, "count/n_tiles out of range");
     if (!out && !out64) return fail(TQ_E_NULL, "no output pointer");
     if (dtype == TQ_F32)

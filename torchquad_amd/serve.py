@@ -163,7 +163,7 @@ class _BatchSlot:
         self.tb = _Pinned(device)
         self.tc = _Pinned(device)
         self.part = torch.empty(0, dtype=torch.float64, device=device)
-        self.counter = torch.zeros(4, dtype=torch.int32, device=device)
+        self.counter = torch.zeros(_lib.BATCH_COUNTER_WORDS, dtype=torch.int32, device=device)
         self.class_offsets = np.zeros(4, dtype=np.int64)
         self.out = torch.empty(0, dtype=dtype, device=device)
         self.out64 = torch.empty(0, dtype=torch.float64, device=device)
@@ -477,7 +477,7 @@ class IntegrationServer:
             cd[: count * 8].copy_(ch[: count * 8], non_blocking=True)
             slot.ensure_outputs(count, nt)
             call("tq_batch_run", self.dt, ptr(dd), count, ptr(td), int(n_tiles.value), ptr(bd), ptr(cd), ptr(gl_x),
-                 ptr(gl_w), ptr(slot.part), ptr(slot.counter), ptr(slot.out), ptr(slot.out64), 0,
+                 ptr(gl_w), ptr(slot.part), ptr(slot.counter), slot.counter.numel(), ptr(slot.out), ptr(slot.out64), 0,
                  slot.class_offsets.ctypes.data_as(ctypes.c_void_p), slot.stream.cuda_stream)
             slot.host_out[:count].copy_(slot.out64[:count], non_blocking=True)
             ev = torch.cuda.Event()
