@@ -223,14 +223,17 @@ static int check_item(const tq_batch_item& it) {
     return TQ_OK;
 }
 
-// Split-mode integrals (one axis, m == 0) walk one 64-node row per item
-// with global loads for Gauss-Legendre tables; shorter tiles spread a long
-// axis over more waves.
-constexpr long long kSplitMaxItems = 128;
+// Split-mode integrals (one axis, m == 0) walk one 64-node row per item.
+// Gauss-Legendre rows wait on global loads of the node tables, so shorter
+// tiles spread them over more waves; Newton-Cotes rows compute their nodes
+// and take longer tiles, which spread the per-tile cost over more points.
+constexpr long long kSplitMaxItemsGL = 128;
+constexpr long long kSplitMaxItemsNC = 512;
 
 static long long tile_ipw(const GridGeom& G, long long tile_points) {
     long long ipw = tile_points / kWave;
-    if (G.m == 0 && ipw > kSplitMaxItems) ipw = kSplitMaxItems;
+    const long long cap = G.rule == TQ_RULE_GAUSS_LEGENDRE ? kSplitMaxItemsGL : kSplitMaxItemsNC;
+    if (G.m == 0 && ipw > cap) ipw = cap;
     if (ipw < 1) ipw = 1;
     if (ipw > G.items) ipw = G.items;
     return ipw;

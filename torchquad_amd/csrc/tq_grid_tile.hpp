@@ -126,31 +126,58 @@ __device__ inline double grid_tile(int d, int n, int m, int L, int rule, int lan
     const int lane = threadIdx.x & (kWave - 1);
     double acc = 0.0;
     if (m == 0) {
-        // one axis split into rows of 64 consecutive nodes: node k = 64 t + lane.
-        // Branch-free body (index clamped, zero weight past n) so that the
-        // loop unrolls and the node/weight loads of later steps issue early.
+        // one axis split into rows of 64 consecutive nodes: node k = 64 t + lane
         const State<T> s0 = F::start(p);
         const AxisC<T> ac0 = axis_c(p, 0);
-        const bool gl = rule == TQ_RULE_GAUSS_LEGENDRE;
-        for (long long tb = t_begin; tb < t_end; tb += kFlush) {
-            const long long te = (tb + kFlush < t_end) ? (tb + kFlush) : t_end;
-            T fa = T(0);
+        const Axis a0 = ax[0];
+        // Items holding an end node (t = 0 and the item of node n - 1, which
+        // may also run past n) take the general body: clamped index, zero
+        // weight past n, end weights.
+        const long long t_end_item = (long long)(n - 1) / kWave;
+        auto edge_item = [&](long long t) -> double {
+            const long long k = t * kWave + lane;
+            const int kk = (int)(k < n ? k : n - 1);
+            const double w = (k < n) ? weight_of(rule, kk, n, glw) : 0.0;
+            const T f = F::finish(F::fold_c(s0, ac0, (T)node_of(a0, kk, n, rule, glx), p), p);
+            return w * (double)f;
+        };
+        long long ib = t_begin, ie = t_end;  // interior items [ib, ie)
+        if (ib == 0 && ie > 0) {
+            acc += edge_item(0);
+            ib = 1;
+        }
+        if (ie > t_end_item && t_end_item >= ib) {
+            acc += edge_item(t_end_item);
+            ie = t_end_item;
+        }
+        if (rule == TQ_RULE_GAUSS_LEGENDRE) {
+            for (long long tb = ib; tb < ie; tb += kFlush) {
+                const long long te = (tb + kFlush < ie) ? (tb + kFlush) : ie;
+                T fa = T(0);
 #pragma unroll 4
-            for (long long t = tb; t < te; ++t) {
-                const long long k = t * kWave + lane;
-                const int kk = (int)(k < n ? k : n - 1);
-                double xd, wd;
-                if (gl) {
-                    xd = fma(ax[0].a, glx[kk], ax[0].h);
-                    wd = glw[kk];
-                } else {
-                    xd = (kk == n - 1) ? ax[0].b : fma((double)kk, ax[0].h, ax[0].a);
-                    wd = weight_of(rule, kk, n, glw);
+                for (long long t = tb; t < te; ++t) {
+                    const int k = (int)(t * kWave) + lane;
+                    const T x = (T)fma(a0.a, glx[k], a0.h);
+                    fa = fma((T)glw[k], F::finish(F::fold_c(s0, ac0, x, p), p), fa);
                 }
-                const T w = (k < n) ? (T)wd : T(0);
-                fa = fma(w, F::finish(F::fold_c(s0, ac0, (T)xd, p), p), fa);
+                acc += (double)fa;
             }
-            acc += (double)fa;
+        } else {
+            // Newton-Cotes interior weights repeat with period 4 in the node
+            // index, and 64 t + lane = lane (mod 4): one weight per lane.
+            const double w_lane = weight_of(rule, kWave + lane, 0x7fffffff, glw);
+            for (long long tb = ib; tb < ie; tb += kFlush) {
+                const long long te = (tb + kFlush < ie) ? (tb + kFlush) : ie;
+                double kd = (double)(tb * kWave + lane);
+                T fa = T(0);
+#pragma unroll 4
+                for (long long t = tb; t < te; ++t) {
+                    const T x = (T)fma(kd, a0.h, a0.a);
+                    kd += (double)kWave;
+                    fa += F::finish(F::fold_c(s0, ac0, x, p), p);
+                }
+                acc += w_lane * (double)fa;
+            }
         }
         return acc;
     }
