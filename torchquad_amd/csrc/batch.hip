@@ -19,7 +19,7 @@ namespace tq {
 
 template <typename T>
 struct BatchDev {
-    int rule, fam, d, n, m, L, lane_count, lane_waves;
+    int rule, fam, d, n, m, L, q, nl, lane_count, lane_waves;
     long long items;
     long long gl_offset;
     double scale;
@@ -39,11 +39,11 @@ constexpr int kRegionBig = 8192;
 constexpr int kBatchClasses = 3;  // 0, 1: row-table kernels; 2: general kernel
 
 // LDS bytes of an integral's tables in the row-table kernels:
-// [row weights L*n FP64][loop table S*n T][row terms (S-1)*L*n T]
-inline long long rowtab_bytes(int fam, int n, int L, int tsize) {
+// [row weights L*n FP64][loop table S*nl T][row terms (S-1)*L*n T]
+inline long long rowtab_bytes(int fam, int n, int nl, int L, int tsize) {
     const int S = tab_stride(fam);
     if (L < 0) return 0;
-    return (long long)L * n * 8 + (long long)S * n * tsize + (long long)(S - 1) * L * n * tsize;
+    return (long long)L * n * 8 + (long long)S * nl * tsize + (long long)(S - 1) * L * n * tsize;
 }
 
 // Tile queues per class and the distance between their counters (words).
@@ -62,9 +62,8 @@ __device__ __forceinline__ double batch_tile_general(const BatchDev<T>& D, const
                                             long long t1) {
     if (table) {
         const int lane = threadIdx.x & (kWave - 1);
-        const AxisC<T> cL = axis_c(D.p, D.L);
-        const Axis axL = D.ax[D.L];
-        for (int j = lane; j < D.n; j += kWave) table_entry<T, FAM>(tab, j, axL, cL, D.n, D.rule, glx, glw, D.p);
+        for (int j = lane; j < D.nl; j += kWave)
+            loop_table_entry<T, FAM>(tab, j, ax, D.L, D.q, D.n, D.rule, glx, glw, D.p);
     }
     // make this wave's LDS writes visible to all of its lanes
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -72,10 +71,10 @@ __device__ __forceinline__ double batch_tile_general(const BatchDev<T>& D, const
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     double acc = 0.0;
     for (int lw = lw0; lw < lw1; ++lw)
-        acc += table ? grid_tile<T, FAM, true>(D.d, D.n, D.m, D.L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw, t0,
-                                               t1)
-                     : grid_tile<T, FAM, false>(D.d, D.n, D.m, D.L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw,
-                                                t0, t1);
+        acc += table ? grid_tile<T, FAM, true>(D.d, D.n, D.nl, D.m, D.L, D.rule, D.lane_count, D.p, ax, tab, glx,
+                                               glw, lw, t0, t1)
+                     : grid_tile<T, FAM, false>(D.d, D.n, D.nl, D.m, D.L, D.rule, D.lane_count, D.p, ax, tab, glx,
+                                                glw, lw, t0, t1);
     return acc;
 }
 
@@ -88,11 +87,11 @@ __device__ __forceinline__ double batch_tile_rowtab(const BatchDev<T>& D, const 
     const int n = D.n, L = D.L;
     double* rw = reinterpret_cast<double*>(region);
     T* tab = reinterpret_cast<T*>(region + (L > 0 ? (size_t)L * n * sizeof(double) : 0));
-    T* rt = tab + (size_t)TabStride<FAM>::v * n;
+    T* rt = tab + (size_t)TabStride<FAM>::v * D.nl;
     if (D.m > 0 && L >= 0) {
         const int lane = threadIdx.x & (kWave - 1);
-        const AxisC<T> cL = axis_c(D.p, L);
-        for (int j = lane; j < n; j += kWave) table_entry<T, FAM>(tab, j, ax[L], cL, n, D.rule, glx, glw, D.p);
+        for (int j = lane; j < D.nl; j += kWave)
+            loop_table_entry<T, FAM>(tab, j, ax, L, D.q, n, D.rule, glx, glw, D.p);
         for (int e = lane; e < L * n; e += kWave) {
             const int k = e / n;
             row_table_entry<T, FAM>(rt, rw, k, e - k * n, ax[k], axis_c(D.p, k), n, D.rule, glx, glw, D.p);
@@ -103,8 +102,8 @@ __device__ __forceinline__ double batch_tile_rowtab(const BatchDev<T>& D, const 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     double acc = 0.0;
     for (int lw = lw0; lw < lw1; ++lw)
-        acc += grid_tile<T, FAM, true, true>(D.d, n, D.m, L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw, t0, t1,
-                                             rt, rw);
+        acc += grid_tile<T, FAM, true, true>(D.d, n, D.nl, D.m, L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw,
+                                             t0, t1, rt, rw);
     return acc;
 }
 
@@ -169,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchDev<T>* descs, cons
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        const bool table = REGION == 0 && D.m > 0 && D.L >= 0 && D.n * tab_stride(D.fam) <= 2 * NT;
+        const bool table = REGION == 0 && D.m > 0 && D.L >= 0 && D.nl * tab_stride(D.fam) <= 2 * NT;
         double acc = batch_tile<T, REGION>(D, s_ax[wave], region, table, gx, gw, tile.lane_wave, tile.lane_wave_end,
                                            tile.t0, tile.t1);
         acc = wave_sum(acc);
@@ -262,6 +261,16 @@ static int plan_batch(const tq_batch_item* items, long long count, long long til
             delete[] group;
             return rc;
         }
+        // two loop axes when their table keeps the integral in the same
+        // row-table class (so the kernel's occupancy does not change)
+        if (geo[i].m > 0 && geo[i].L >= 0 && it.dim >= 3 && it.n <= kMergeMaxN) {
+            const long long b1 = rowtab_bytes(it.family, it.n, geo[i].nl, geo[i].L, (int)sizeof(T));
+            const long long region = b1 <= kRegionSmall ? kRegionSmall : (b1 <= kRegionBig ? kRegionBig : 0);
+            GridGeom g2;
+            if (region > 0 && plan_geom(it.dim, it.n, it.rule, 0, 1, 1, g2, kLoopTable) == TQ_OK && g2.q == 2 &&
+                rowtab_bytes(it.family, it.n, g2.nl, g2.L, (int)sizeof(T)) <= region)
+                geo[i] = g2;
+        }
         const long long ipw = tile_ipw(geo[i], tile_points);
         const long long chunks = (geo[i].items + ipw - 1) / ipw;
         geo[i].ipw = ipw;
@@ -278,7 +287,7 @@ static int plan_batch(const tq_batch_item* items, long long count, long long til
         ipow_checked(it.n, it.dim, 1LL << 62, &P);
         int cls = 0;
         if (geo[i].m > 0 && geo[i].L >= 0) {
-            const long long b = rowtab_bytes(it.family, it.n, geo[i].L, (int)sizeof(T));
+            const long long b = rowtab_bytes(it.family, it.n, geo[i].nl, geo[i].L, (int)sizeof(T));
             cls = b <= kRegionSmall ? 0 : (b <= kRegionBig ? 1 : 2);
         }
         order[i] = {cls, P, i};
@@ -291,6 +300,8 @@ static int plan_batch(const tq_batch_item* items, long long count, long long til
             D.n = it.n;
             D.m = geo[i].m;
             D.L = geo[i].L;
+            D.q = geo[i].q;
+            D.nl = geo[i].nl;
             D.lane_count = geo[i].lane_count;
             D.lane_waves = geo[i].lane_waves;
             D.items = geo[i].items;

@@ -106,8 +106,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_fwd(SweepGeom S, const T* a, c
         const long long chunk = wave / g.lane_waves;
         const long long t0 = chunk * g.ipw;
         const long long t1 = (t0 + g.ipw < g.items) ? (t0 + g.ipw) : g.items;
-        acc = grid_tile<T, TQ_FAM_GAUSSIAN_PEAK, TABLE, ROWTAB>(g.d, g.n, g.m, g.L, g.rule, g.lane_count, s_p, s_axis,
-                                                                s_tab, glx, glw, lw, t0, t1, s_rt, s_rw);
+        acc = grid_tile<T, TQ_FAM_GAUSSIAN_PEAK, TABLE, ROWTAB>(g.d, g.n, g.nl, g.m, g.L, g.rule, g.lane_count, s_p,
+                                                                s_axis, s_tab, glx, glw, lw, t0, t1, s_rt, s_rw);
     }
     const double bs = block_sum(acc, s_red);
     if (threadIdx.x == 0) partials[b * gridDim.x + blockIdx.x] = bs;

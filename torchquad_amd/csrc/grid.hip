@@ -62,19 +62,22 @@ static int check_gl_tables(const tq_grid_desc* g) {
     return TQ_OK;
 }
 
+// Loop tables of up to 1024 entries (n <= 32 with two loop axes).
+constexpr long long kFusedMergeCap = 1024;
+
 static int make_geom(const tq_grid_desc* gd, GridGeom& G) {
-    return plan_geom(gd->dim, gd->n, gd->rule, gd->lane_axes, gd->items_per_wave, 16384, G);
+    return plan_geom(gd->dim, gd->n, gd->rule, gd->lane_axes, gd->items_per_wave, 16384, G, kFusedMergeCap);
 }
 
 // ---------------------------------------------------------------------------
 // Fused kernel
 // ---------------------------------------------------------------------------
 // Dynamic LDS of the fused kernel: [row weights: L*n FP64][loop table:
-// S*n T][row contributions: (S-1)*L*n T] (the row parts only when ROWTAB).
+// S*nl T][row contributions: (S-1)*L*n T] (the row parts only when ROWTAB).
 template <typename T, int FAM>
 inline size_t fused_lds_bytes(const GridGeom& G, bool rowtab) {
     constexpr int S = TabStride<FAM>::v;
-    size_t b = (size_t)S * G.n * sizeof(T);
+    size_t b = (size_t)S * G.nl * sizeof(T);
     if (rowtab) b += (size_t)G.L * G.n * (sizeof(double) + (S - 1) * sizeof(T));
     return b;
 }
@@ -92,10 +95,10 @@ __global__ __launch_bounds__(kBlock) void k_grid_fused(GridArgs<T> A) {
     __syncthreads();
     double* s_rw = reinterpret_cast<double*>(smem_raw);
     T* s_tab = reinterpret_cast<T*>(smem_raw + (ROWTAB ? (size_t)L * n * sizeof(double) : 0));
-    T* s_rt = s_tab + (size_t)TabStride<FAM>::v * n;
+    T* s_rt = s_tab + (size_t)TabStride<FAM>::v * g.nl;
     if (TABLE) {
-        const AxisC<T> cL = axis_c(A.p, L);
-        for (int j = tid; j < n; j += kBlock) table_entry<T, FAM>(s_tab, j, s_axis[L], cL, n, rule, A.glx, A.glw, A.p);
+        for (int j = tid; j < g.nl; j += kBlock)
+            loop_table_entry<T, FAM>(s_tab, j, s_axis, L, g.q, n, rule, A.glx, A.glw, A.p);
         if (ROWTAB)
             for (int e = tid; e < L * n; e += kBlock) {
                 const int k = e / n;
@@ -114,8 +117,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_fused(GridArgs<T> A) {
         const int lw = (int)(uw - chunk * ulw);
         const long long t0 = (long long)chunk * g.ipw;
         const long long t1 = (t0 + g.ipw < g.items) ? (t0 + g.ipw) : g.items;
-        acc = grid_tile<T, FAM, TABLE, ROWTAB>(d, n, g.m, L, rule, g.lane_count, A.p, s_axis, s_tab, A.glx, A.glw, lw,
-                                               t0, t1, s_rt, s_rw);
+        acc = grid_tile<T, FAM, TABLE, ROWTAB>(d, n, g.nl, g.m, L, rule, g.lane_count, A.p, s_axis, s_tab, A.glx,
+                                               A.glw, lw, t0, t1, s_rt, s_rw);
     }
     const double bs = block_sum(acc, s_red);
     if (tid == 0) A.partials[blockIdx.x] = bs;

@@ -95,6 +95,27 @@ __device__ inline void table_entry(T* tab, int j, const Axis& axL, const AxisC<T
     tab[S * j + S - 1] = (T)weight_of(rule, j, n, glw);
 }
 
+// Loop-table entry j when the loop runs over q consecutive axes L .. L+q-1
+// (q = 1 or 2; for q = 2, j = j1 n + j2 with j1 on axis L): the axes'
+// contributions folded together, then the product of their weights.
+template <typename T, int FAM>
+__device__ inline void loop_table_entry(T* tab, int j, const Axis* ax, int L, int q, int n, int rule,
+                                        const double* glx, const double* glw, const Params<T>& p) {
+    using F = Family<FAM, T>;
+    constexpr int S = TabStride<FAM>::v;
+    const int j1 = q == 2 ? j / n : j;
+    State<T> s = F::fold_c(F::zero(), axis_c(p, L), (T)node_of(ax[L], j1, n, rule, glx), p);
+    double w = weight_of(rule, j1, n, glw);
+    if (q == 2) {
+        const int j2 = j - j1 * n;
+        s = F::fold_c(s, axis_c(p, L + 1), (T)node_of(ax[L + 1], j2, n, rule, glx), p);
+        w *= weight_of(rule, j2, n, glw);
+    }
+    tab[S * j] = s.s0;
+    if (S == 3) tab[S * j + 1] = s.s1;
+    tab[S * j + S - 1] = (T)w;
+}
+
 // Row-axis tables (ROWTAB): for the row axes k < L, entry k*n + j holds the
 // axis's contribution at node j (the state slots of table_entry) in `rt` and
 // its weight in FP64 in `rw`, so a row's state and weight are merged from
@@ -114,11 +135,12 @@ __device__ inline void row_table_entry(T* rt, double* rw, int k, int j, const Ax
 // One tile of the tensor-product index space.  m == 0 (one axis): node
 // 64 t + lane for loop items t in [t_begin, t_end).  Otherwise the lanes of the calling wave
 // take lane combinations lw*64 .. lw*64+63 of the last m axes, the wave walks
-// loop items [t_begin, t_end) (row axes [0, L) x loop axis L).  Returns the
-// lane's weighted sum.  `ax` holds the per-axis maps, `tab` the loop-axis
-// table (table_entry) when TABLE.
+// loop items [t_begin, t_end) (row axes [0, L) x a loop of nl points over
+// the axes between the row and lane axes: one axis, nl = n, or two, nl = n^2,
+// which needs the table).  Returns the lane's weighted sum.  `ax` holds the
+// per-axis maps, `tab` the loop table (loop_table_entry) when TABLE.
 template <typename T, int FAM, bool TABLE, bool ROWTAB = false>
-__device__ inline double grid_tile(int d, int n, int m, int L, int rule, int lane_count, const Params<T>& p,
+__device__ inline double grid_tile(int d, int n, int nl, int m, int L, int rule, int lane_count, const Params<T>& p,
                                    const Axis* ax, const T* tab, const double* glx, const double* glw, int lw,
                                    long long t_begin, long long t_end, const T* rt = nullptr,
                                    const double* rw = nullptr) {
@@ -211,9 +233,9 @@ __device__ inline double grid_tile(int d, int n, int m, int L, int rule, int lan
     // index fits (64-bit division is a long instruction sequence)
     int rdig[TQ_MAX_DIM];
     if (t < 0x7fffffffLL) {
-        const unsigned un = (unsigned)n;
-        unsigned r = (unsigned)t / un;
-        j = (int)((unsigned)t - r * un);
+        const unsigned un = (unsigned)n, unl = (unsigned)nl;
+        unsigned r = (unsigned)t / unl;
+        j = (int)((unsigned)t - r * unl);
         row = r;
 #pragma unroll
         for (int k = TQ_MAX_DIM - 1; k >= 0; --k) {
@@ -226,8 +248,8 @@ __device__ inline double grid_tile(int d, int n, int m, int L, int rule, int lan
             }
         }
     } else {
-        row = t / n;
-        j = (int)(t - row * n);
+        row = t / nl;
+        j = (int)(t - row * nl);
         long long r = row;
 #pragma unroll
         for (int k = TQ_MAX_DIM - 1; k >= 0; --k) {
@@ -266,7 +288,7 @@ __device__ inline double grid_tile(int d, int n, int m, int L, int rule, int lan
         }
         const State<T> s_rl = F::merge(s_row, s_lane);
         const long long left = t_end - t;
-        const int jend = (left < (long long)(n - j)) ? (j + (int)left) : n;
+        const int jend = (left < (long long)(nl - j)) ? (j + (int)left) : nl;
         double racc = 0.0;
         for (int jb = j; jb < jend; jb += kFlush) {
             const int je = (jb + kFlush < jend) ? (jb + kFlush) : jend;
@@ -293,7 +315,7 @@ __device__ inline double grid_tile(int d, int n, int m, int L, int rule, int lan
         acc = fma(w_row, racc, acc);
         t += jend - j;
         j = jend;
-        if (j == n) {
+        if (j == nl) {
             j = 0;
 #pragma unroll
             for (int k = TQ_MAX_DIM - 1; k >= 0; --k) {
@@ -309,7 +331,8 @@ __device__ inline double grid_tile(int d, int n, int m, int L, int rule, int lan
 
 // ---- host-side planning ---------------------------------------------------
 struct GridGeom {
-    int d, n, m, L;
+    int d, n, m, L;       // L: row axes [0, L), loop axes [L, L + q)
+    int q, nl;            // loop axes (1 or 2) and loop length n^q
     int rule;
     int lane_count;       // n^m
     int lane_waves;       // ceil(lane_count / 64)
@@ -331,13 +354,22 @@ inline int ipow_checked(long long base, int e, long long limit, long long* out) 
     return 0;
 }
 
-inline // Splits the n^d index space of one integral (see grid_tile).  lane_axes /
-// items_per_wave override the automatic choice (0 = automatic).
-int plan_geom(int d, int n, int rule, int lane_axes, long long items_per_wave, long long target_waves,
-                     GridGeom& G) {
+// Largest n whose loop may run over two axes (n^2 points per row).
+constexpr int kMergeMaxN = 32;
+
+// Splits the n^d index space of one integral (see grid_tile).  lane_axes /
+// items_per_wave override the automatic choice (0 = automatic).  When
+// merge_cap > 0, n <= kMergeMaxN, n^2 <= merge_cap and there are at least two
+// axes between the row and lane axes' ends, the loop runs over the last two
+// non-lane axes (n^2 points per row, one table entry per point pair), so a
+// short axis does not pay the per-row cost every n points.
+inline int plan_geom(int d, int n, int rule, int lane_axes, long long items_per_wave, long long target_waves,
+                     GridGeom& G, long long merge_cap = 0) {
     G.d = d;
     G.n = n;
     G.rule = rule;
+    G.q = 1;
+    G.nl = n;
     int m = 0;
     if (lane_axes > 0) {
         m = lane_axes < d ? lane_axes : d;
@@ -365,6 +397,15 @@ int plan_geom(int d, int n, int rule, int lane_axes, long long items_per_wave, l
             }
         }
         if (d == 1) m = 1;
+        // One lane axis fewer when that lets the loop run over two short
+        // axes (n^2 points per row instead of n) and the lanes still fill
+        // three quarters of their waves.
+        if (merge_cap > 0 && d >= 3 && d - m == 1 && m >= 2 && n <= kMergeMaxN && (long long)n * n <= merge_cap) {
+            long long c1 = 1;
+            for (int k = 0; k < m - 1; ++k) c1 *= n;
+            const double util = (double)c1 / (double)(((c1 + kWave - 1) / kWave) * kWave);
+            if (util >= 0.75) m -= 1;
+        }
     }
     long long items = 1;
     if (m == 0) {
@@ -381,9 +422,15 @@ int plan_geom(int d, int n, int rule, int lane_axes, long long items_per_wave, l
         G.lane_waves = (int)((lc + kWave - 1) / kWave);
         G.L = d - m - 1;
         if (G.L >= 0) ipow_checked(n, d - m, 1LL << 62, &items);
+        if (merge_cap > 0 && G.L >= 1 && n <= kMergeMaxN && (long long)n * n <= merge_cap &&
+            (long long)n * n <= kLoopTable) {
+            G.q = 2;
+            G.nl = n * n;
+            G.L -= 1;
+        }
     }
     G.items = items;
-    G.use_table = (G.m > 0 && G.L >= 0 && n <= kLoopTable) ? 1 : 0;
+    G.use_table = (G.m > 0 && G.L >= 0 && G.nl <= kLoopTable) ? 1 : 0;
     long long ipw = items_per_wave;
     if (ipw <= 0) {
         const long long work = items * (long long)G.lane_waves;
