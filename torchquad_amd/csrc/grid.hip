@@ -62,11 +62,17 @@ static int check_gl_tables(const tq_grid_desc* g) {
     return TQ_OK;
 }
 
-// Loop tables of up to 1024 entries (n <= 32 with two loop axes).
-constexpr long long kFusedMergeCap = 1024;
+// LDS for a merged (two-axis) loop table in the fused kernel; any cap of at
+// least 1024 entries gives the same launch size (see kMergeMaxN).
+constexpr size_t kFusedMergeLds = 40 * 1024;
 
-static int make_geom(const tq_grid_desc* gd, GridGeom& G) {
-    return plan_geom(gd->dim, gd->n, gd->rule, gd->lane_axes, gd->items_per_wave, 16384, G, kFusedMergeCap);
+inline long long fused_merge_cap(int fam, int tsize) {
+    const long long cap = (long long)(kFusedMergeLds / ((size_t)tab_stride(fam) * tsize));
+    return cap > 1024 ? cap : 1024;
+}
+
+static int make_geom(const tq_grid_desc* gd, GridGeom& G, long long merge_cap = 1024) {
+    return plan_geom(gd->dim, gd->n, gd->rule, gd->lane_axes, gd->items_per_wave, 16384, G, merge_cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -83,11 +89,14 @@ inline size_t fused_lds_bytes(const GridGeom& G, bool rowtab) {
 }
 constexpr size_t kFusedLdsMax = 48 * 1024;
 
-template <typename T, int FAM, bool TABLE, bool ROWTAB>
-__global__ __launch_bounds__(kBlock) void k_grid_fused(GridArgs<T> A) {
+// BW waves per block: 4, or 8 when a large loop table is shared by more
+// waves (the LDS per block, not the registers, then bounds occupancy).
+template <typename T, int FAM, bool TABLE, bool ROWTAB, int BW>
+__global__ __launch_bounds__(BW * kWave) void k_grid_fused(GridArgs<T> A) {
+    constexpr int NT = BW * kWave;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ Axis s_axis[TQ_MAX_DIM];
-    __shared__ double s_red[kWavesPerBlock];
+    __shared__ double s_red[BW];
     const GridGeom& g = A.g;
     const int tid = threadIdx.x;
     const int n = g.n, d = g.d, L = g.L, rule = g.rule;
@@ -97,10 +106,10 @@ __global__ __launch_bounds__(kBlock) void k_grid_fused(GridArgs<T> A) {
     T* s_tab = reinterpret_cast<T*>(smem_raw + (ROWTAB ? (size_t)L * n * sizeof(double) : 0));
     T* s_rt = s_tab + (size_t)TabStride<FAM>::v * g.nl;
     if (TABLE) {
-        for (int j = tid; j < g.nl; j += kBlock)
+        for (int j = tid; j < g.nl; j += NT)
             loop_table_entry<T, FAM>(s_tab, j, s_axis, L, g.q, n, rule, A.glx, A.glw, A.p);
         if (ROWTAB)
-            for (int e = tid; e < L * n; e += kBlock) {
+            for (int e = tid; e < L * n; e += NT) {
                 const int k = e / n;
                 row_table_entry<T, FAM>(s_rt, s_rw, k, e - k * n, s_axis[k], axis_c(A.p, k), n, rule, A.glx, A.glw,
                                         A.p);
@@ -108,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_fused(GridArgs<T> A) {
         __syncthreads();
     }
     // readfirstlane: the wave index is uniform, so the tile loop runs on SGPRs
-    const long long wave = (long long)blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(tid >> 6);
+    const long long wave = (long long)blockIdx.x * BW + __builtin_amdgcn_readfirstlane(tid >> 6);
     double acc = 0.0;
     if (wave < g.num_waves) {
         // wave < 2^31 (grid size limit): 32-bit division
@@ -143,11 +152,87 @@ __global__ __launch_bounds__(1024) void k_grid_finalize(const double* partials, 
     }
 }
 
+// Blocks of `kernel` (`threads` per block, `shmem` bytes of dynamic LDS)
+// that stay resident on the current device at once; 0 if unknown.  Answers are
+// kept per thread for the last few (device, kernel, shmem) keys, since the
+// runtime query costs more than a small launch.
+static long long resident_blocks(const void* kernel, int threads, size_t shmem) {
+    struct Entry {
+        int dev;
+        const void* kernel;
+        size_t shmem;
+        long long blocks;
+    };
+    constexpr int kEntries = 16;
+    thread_local Entry cache[kEntries] = {};
+    thread_local int next = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    for (const Entry& e : cache)
+        if (e.kernel == kernel && e.dev == dev && e.shmem == shmem) return e.blocks;
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, shmem) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    const long long r = (long long)cus * per_cu;
+    cache[next] = Entry{dev, kernel, shmem, r};
+    next = (next + 1) % kEntries;
+    return r;
+}
+
+// Workspace of the fused kernel, in block partials: at least this many, so
+// that the automatic items per wave below can fill whole rounds.
+constexpr long long kFusedMinBlocks = 8192;
+// Per-wave cost of a work unit beyond its items (set-up, tables), in items.
+constexpr long long kUnitOverheadItems = 128;
+
+// Automatic items per wave for a launch of `kernel`: the number of
+// resident rounds r (1..16) whose grid of units, each lane wave's items cut
+// into floor(r W / lane_waves) chunks (W resident waves), minimises
+// rounds x (items per unit + overhead), so the last round is not left
+// mostly idle.  Keeps G as it is when the occupancy is unknown.
+static void align_ipw(GridGeom& G, const void* kernel, int bw, size_t shmem, long long max_blocks) {
+    const long long W = resident_blocks(kernel, bw * kWave, shmem) * bw;
+    if (W <= 0 || G.items <= 0) return;
+    long long best_cost = -1, best_ipw = G.ipw;
+    for (long long r = 1; r <= 16; ++r) {
+        long long per_lw = (r * W) / G.lane_waves;
+        if (per_lw < 1) per_lw = 1;
+        const long long ipw = (G.items + per_lw - 1) / per_lw;
+        const long long units = ((G.items + ipw - 1) / ipw) * (long long)G.lane_waves;
+        if ((units + bw - 1) / bw > max_blocks) continue;
+        const long long cost = ((units + W - 1) / W) * (ipw + kUnitOverheadItems);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best_ipw = ipw;
+        }
+    }
+    G.ipw = best_ipw;
+    G.chunks = (G.items + best_ipw - 1) / best_ipw;
+    G.num_waves = G.chunks * (long long)G.lane_waves;
+}
+
+// Dynamic LDS above which 8-wave blocks share the tables.
+constexpr size_t kWideBlockLds = 16 * 1024;
+
+template <typename T, int FAM, bool TABLE, bool ROWTAB, int BW>
+static void launch_fused(GridGeom& G, GridArgs<T>& A, size_t shmem, bool auto_ipw, long long max_blocks,
+                         hipStream_t st) {
+    const void* kern = (const void*)(k_grid_fused<T, FAM, TABLE, ROWTAB, BW>);
+    if (auto_ipw) align_ipw(G, kern, BW, shmem, max_blocks);
+    G.blocks = (G.num_waves + BW - 1) / BW;
+    A.g = G;
+    hipLaunchKernelGGL((k_grid_fused<T, FAM, TABLE, ROWTAB, BW>), dim3((unsigned)G.blocks), dim3(BW * kWave), shmem,
+                       st, A);
+}
+
 template <typename T>
 static int launch_grid(const tq_grid_desc* gd, const tq_integrand* f, const void* domain,
                        void* ws, size_t ws_bytes, void* out, double* out64, hipStream_t st) {
     GridGeom G;
-    int rc = make_geom(gd, G);
+    int rc = make_geom(gd, G, fused_merge_cap(f->family, (int)sizeof(T)));
     if (rc) return rc;
     const size_t need = (size_t)G.blocks * sizeof(double);
     if (!ws || ws_bytes < need) return fail(TQ_E_WORKSPACE, "grid workspace too small");
@@ -158,17 +243,22 @@ static int launch_grid(const tq_grid_desc* gd, const tq_integrand* f, const void
     A.glw = gd->gl_weights;
     A.partials = (double*)ws;
     A.p = make_params<T>(*f);
+    const bool auto_ipw = gd->items_per_wave <= 0;
+    const long long max_blocks = (long long)(ws_bytes / sizeof(double));
     TQ_DISPATCH_FAMILY(f->family, {
         const bool rowtab = G.use_table && G.L >= 1 && (fused_lds_bytes<T, FAM>(G, true)) <= kFusedLdsMax;
         const size_t shmem = (fused_lds_bytes<T, FAM>(G, rowtab));
-        if (rowtab)
-            hipLaunchKernelGGL((k_grid_fused<T, FAM, true, true>), dim3((unsigned)G.blocks), dim3(kBlock), shmem, st,
-                               A);
+        const bool wide = shmem > kWideBlockLds;
+        if (rowtab && wide)
+            (launch_fused<T, FAM, true, true, 8>)(G, A, shmem, auto_ipw, max_blocks, st);
+        else if (rowtab)
+            (launch_fused<T, FAM, true, true, kWavesPerBlock>)(G, A, shmem, auto_ipw, max_blocks, st);
+        else if (G.use_table && wide)
+            (launch_fused<T, FAM, true, false, 8>)(G, A, shmem, auto_ipw, max_blocks, st);
         else if (G.use_table)
-            hipLaunchKernelGGL((k_grid_fused<T, FAM, true, false>), dim3((unsigned)G.blocks), dim3(kBlock), shmem, st,
-                               A);
+            (launch_fused<T, FAM, true, false, kWavesPerBlock>)(G, A, shmem, auto_ipw, max_blocks, st);
         else
-            hipLaunchKernelGGL((k_grid_fused<T, FAM, false, false>), dim3((unsigned)G.blocks), dim3(kBlock), 0, st, A);
+            (launch_fused<T, FAM, false, false, kWavesPerBlock>)(G, A, 0, auto_ipw, max_blocks, st);
     });
     rc = check_launch("k_grid_fused");
     if (rc) return rc;
@@ -456,7 +546,8 @@ size_t tq_grid_workspace_size(const tq_grid_desc* g) {
     if (check_grid_desc(g, TQ_MAX_DIM)) return 0;
     GridGeom G;
     if (make_geom(g, G)) return 0;
-    return (size_t)G.blocks * sizeof(double);
+    const long long blocks = G.blocks > kFusedMinBlocks ? G.blocks : kFusedMinBlocks;
+    return (size_t)blocks * sizeof(double);
 }
 
 int tq_grid_integrate(const tq_grid_desc* g, const tq_integrand* f, const void* domain, void* workspace,

@@ -354,15 +354,18 @@ inline int ipow_checked(long long base, int e, long long limit, long long* out) 
     return 0;
 }
 
-// Largest n whose loop may run over two axes (n^2 points per row).
+// Largest n for which the planner gives up a lane axis to merge two loop
+// axes (the choice of lane axes, and so the launch size, then does not
+// depend on merge caps of 1024 or more).
 constexpr int kMergeMaxN = 32;
 
 // Splits the n^d index space of one integral (see grid_tile).  lane_axes /
 // items_per_wave override the automatic choice (0 = automatic).  When
-// merge_cap > 0, n <= kMergeMaxN, n^2 <= merge_cap and there are at least two
-// axes between the row and lane axes' ends, the loop runs over the last two
-// non-lane axes (n^2 points per row, one table entry per point pair), so a
-// short axis does not pay the per-row cost every n points.
+// merge_cap > 0, n^2 <= merge_cap (the caller's loop-table capacity) and
+// there are at least two axes between the row axes and the lane axes, the
+// loop runs over the last two non-lane axes (n^2 points per row, one table
+// entry per point pair), so a short axis does not pay the per-row cost
+// every n points.
 inline int plan_geom(int d, int n, int rule, int lane_axes, long long items_per_wave, long long target_waves,
                      GridGeom& G, long long merge_cap = 0) {
     G.d = d;
@@ -422,15 +425,14 @@ inline int plan_geom(int d, int n, int rule, int lane_axes, long long items_per_
         G.lane_waves = (int)((lc + kWave - 1) / kWave);
         G.L = d - m - 1;
         if (G.L >= 0) ipow_checked(n, d - m, 1LL << 62, &items);
-        if (merge_cap > 0 && G.L >= 1 && n <= kMergeMaxN && (long long)n * n <= merge_cap &&
-            (long long)n * n <= kLoopTable) {
+        if (merge_cap > 0 && G.L >= 1 && (long long)n * n <= merge_cap) {
             G.q = 2;
             G.nl = n * n;
             G.L -= 1;
         }
     }
     G.items = items;
-    G.use_table = (G.m > 0 && G.L >= 0 && G.nl <= kLoopTable) ? 1 : 0;
+    G.use_table = (G.m > 0 && G.L >= 0 && (G.nl <= kLoopTable || G.q == 2)) ? 1 : 0;
     long long ipw = items_per_wave;
     if (ipw <= 0) {
         const long long work = items * (long long)G.lane_waves;
