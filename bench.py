@@ -49,6 +49,7 @@ DIM_C2A = 5
 FLOPS_PER_EVAL_GRID_GAUSSIAN = 4 * DIM_C2A + 3 + 1
 COPY_BYTES = 256 << 20
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_grid_fused_boole_d5_n65.json"
+SETTLE_STEPS = 200  # untimed steps before tuning (about 35 ms of GPU work at n=65)
 
 
 def load_traffic():
@@ -384,6 +385,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mc", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the VEGAS, batched and gradient lines")
+    ap.add_argument("--ipw", type=int, default=0,
+                    help="items per wave of the timed fused kernel (0: tune before timing)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run only a 256 MiB elementwise copy (counter calibration) and the timed step, "
                     "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes")
@@ -422,13 +425,23 @@ def main():
             torch.neg(src, out=dst)
         torch.cuda.synchronize()
 
-    # project autotuner: items per wave of the fused kernel for this geometry
-    # (and the 17^5 latency case), chosen before the timed region
+    # The GPU raises its clocks over the first tens of milliseconds of
+    # sustained work; run the step untimed until then, so that tuning and the
+    # timed steps both see the sustained clock.
+    for _ in range(SETTLE_STEPS):
+        boole.integrate(f, 5, N=N, integration_domain=dom)
+    torch.cuda.synchronize()
+
+    # project autotuner: items per wave of the fused kernel for this geometry,
+    # chosen before the timed region
     from torchquad_amd import _lib
     from torchquad_amd.autotune import tune_grid
 
+    if args.ipw > 0:
+        from torchquad_amd.autotune import set_grid_ipw
+
+        set_grid_ipw(_lib.RULE_BOOLE, f.descriptor(5).family, 5, n, dom.dtype, args.ipw)
     ipw, _ = tune_grid(_lib.RULE_BOOLE, f, 5, n, dom)
-    ipw17, _ = tune_grid(_lib.RULE_BOOLE, f, 5, 17, dom)
 
     for _ in range(args.warmup):
         boole.integrate(f, 5, N=N, integration_domain=dom)
@@ -452,6 +465,7 @@ def main():
     rel_vs_oracle = abs(val - oracle_same_rule) / abs(oracle_same_rule)
 
     # per-call latency of the literal 17^5 configuration (host call -> result on device)
+    ipw17, _ = tune_grid(_lib.RULE_BOOLE, f, 5, 17, dom)
     lat = []
     N17 = 17**5
     for _ in range(5):
@@ -511,7 +525,10 @@ def main():
         "unit": "TFLOP/s",
         "frac": achieved_tflops / PEAK_FP32_TFLOPS,
         "flops_per_eval": FLOPS_PER_EVAL_GRID_GAUSSIAN,
-        "flops_per_eval_note": "4d+3 arithmetic + 1 exp at d=5 (algorithmic count)",
+        "flops_per_eval_note": "4d+3 arithmetic + 1 exp at d=5 (algorithmic count; the kernel shares row and "
+                               "lane partial sums, so it executes fewer and frac can exceed 1)",
+        "bound_note": "vector ALU: FP32 157.3 TFLOP/s, the same figure as dense FP32 MFMA; no HBM traffic "
+                      "or matrix product on this path",
         "traffic": load_traffic(),
     }
 

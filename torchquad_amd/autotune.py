@@ -116,9 +116,17 @@ def grid_ipw(rule, family, dim, n, dtype):
     return int(hit["items_per_wave"]) if hit else 0
 
 
+def set_grid_ipw(rule, family, dim, n, dtype, items_per_wave):
+    """Fix the fused grid kernel's items per wave for this geometry (as if tuned)."""
+    with _lock:
+        _cache[_grid_key(rule, family, dim, n, dtype)] = {"items_per_wave": int(items_per_wave), "ms": None,
+                                                           "table": []}
+
+
 def tune_grid(rule, fn, dim, n, domain, candidates=IPW_CANDIDATES, reps=5, use_cache=True):
     """Time the fused grid kernel of (rule, fn, dim, n) on `domain` for each
-    items-per-wave candidate (HIP events, median of `reps` after a warm-up),
+    items-per-wave candidate (HIP events around `reps` back-to-back calls,
+    best of three such batches after a warm-up),
     cache the fastest for later calls with the same geometry and return
     (items_per_wave, table)."""
     from . import _ops
@@ -138,19 +146,22 @@ def tune_grid(rule, fn, dim, n, domain, candidates=IPW_CANDIDATES, reps=5, use_c
         gl = gauss_legendre(int(n), device=domain.device)
     table = []
     for ipw in candidates:
-        _ops.grid_integrate(rule, fn, dim, n, domain, gl, 0, ipw)
+        for _ in range(2):
+            _ops.grid_integrate(rule, fn, dim, n, domain, gl, 0, ipw)
         torch.cuda.synchronize(domain.device)
+        # back-to-back calls between two events, so the device never waits on
+        # the host's launch path inside the timed span; best of three batches
         times = []
-        for _ in range(reps):
+        for _ in range(3):
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
-            _ops.grid_integrate(rule, fn, dim, n, domain, gl, 0, ipw)
+            for _ in range(reps):
+                _ops.grid_integrate(rule, fn, dim, n, domain, gl, 0, ipw)
             e1.record()
             e1.synchronize()
-            times.append(e0.elapsed_time(e1))
-        times.sort()
-        table.append((ipw, times[len(times) // 2]))
+            times.append(e0.elapsed_time(e1) / reps)
+        table.append((ipw, min(times)))
     best = min(table, key=lambda r: r[1])
     with _lock:
         _cache[key] = {"items_per_wave": best[0], "ms": best[1], "table": table}
