@@ -339,3 +339,63 @@ def test_gauss_hermite_rule(dev):
     a = float(gh.integrate(f, 2, N=24**2, integration_domain=dom2))
     b = float(UserHermite().integrate(f, 2, N=24**2, integration_domain=dom2))
     assert abs(a - b) <= 1e-12 * abs(b)
+
+
+SPLIT_N = {
+    "trapezoid": (128, 129, 191, 192, 193, 255, 256, 257, 1000, 4097),
+    "simpson": (129, 191, 193, 255, 257, 1001, 4097),
+    "boole": (129, 193, 257, 1001, 4097),
+    "gauss_legendre": (128, 129, 192, 257, 1000),
+}
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("rule", list(RULES))
+def test_split_mode_sizes_vs_oracle(dtype, rule, dev):
+    """One axis of n >= 128 nodes is walked in rows of 64 nodes: interior rows in a tight loop with
+    one weight per lane, the rows holding the end nodes (and nodes past n) in the general body.
+    Sizes around the row boundaries, every family, against the FP64 brute force."""
+    rng = np.random.default_rng(17)
+    for n in SPLIT_N[rule]:
+        for fname in FAMS:
+            f, kw, sup = make_family(fname, 1, rng)
+            dom = domain_for(fname, 1, rng)
+            ref = oracle_value(rule, fname, 1, n, dom, kw)
+            val = RULES[rule]().integrate(f, 1, N=n, integration_domain=torch.tensor(dom, dtype=dtype, device=dev))
+            check(float(val), ref, dtype, (dom[0][1] - dom[0][0]) * sup)
+
+
+MERGE_SHAPES = [(3, 61), (4, 40), (4, 65), (5, 33)]
+
+
+def separable_ref(rule, fname, f, dom, n):
+    """FP64 sum of the rule for the product-form families (and sums of products)."""
+    d = len(dom)
+    if fname == "gaussian_peak":
+        return og.separable_gaussian_peak(rule, f.a, f.u, dom, n)
+    if fname == "oscillatory":
+        return og.separable_oscillatory(rule, f.c, f.w, dom, n)
+    if fname == "product_peak":
+        return og.separable_product(rule, [lambda x, a=a, u=u: 1.0 / (a**-2.0 + (x - u) ** 2)
+                                           for a, u in zip(f.a, f.u)], dom, n)
+    inv = 1.0 / (f.width * f.width)
+    return f.norm * sum(og.separable_product(rule, [lambda x, c=c: np.exp(-inv * (x - c) ** 2)] * d, dom, n)
+                        for c in f.centers)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("rule", list(RULES))
+def test_two_axis_loops_vs_oracle(dtype, rule, dev):
+    """Shapes whose loop runs over two axes (n^2-entry tables; above 16 KB of tables the blocks are
+    8 waves), for one- and two-slot families, against separable FP64 sums of the same rule."""
+    rng = np.random.default_rng(23)
+    for d, n in MERGE_SHAPES:
+        if rule in ("simpson", "boole") and n % 2 == 0:
+            n += 1
+        for fname in ("gaussian_peak", "oscillatory", "product_peak", "double_gaussian"):
+            f, kw, sup = make_family(fname, d, rng)
+            dom = domain_for(fname, d, rng)
+            vol = float(np.prod([b - a for a, b in dom]))
+            ref = separable_ref(rule, fname, f, dom, n)
+            val = RULES[rule]().integrate(f, d, N=n**d, integration_domain=torch.tensor(dom, dtype=dtype, device=dev))
+            check(float(val), ref, dtype, vol * sup)
