@@ -30,6 +30,9 @@ struct SweepGeom {
 // loop-axis table, 1 + 3 dim block sums) is spread over many points.
 constexpr long long kSweepWaves = 16384;
 
+// One loop axis in both kernels (the backward tile walks one; a two-axis
+// loop in the forward kernel measured slower at 1024 x GL 32^4, whose blocks
+// are short and spend their start building the n^2-entry table).
 static int sweep_geom(const tq_grid_desc* gd, long long B, SweepGeom& S) {
     const long long per_integral = B > 0 ? (kSweepWaves + B - 1) / B : kSweepWaves;
     int rc = plan_geom(gd->dim, gd->n, gd->rule, gd->lane_axes, gd->items_per_wave, per_integral, S.g);
@@ -65,10 +68,10 @@ __device__ inline void load_sweep_params(int d, int n, int rule, long long b, co
 constexpr int kSweepRowTab = 1024;  // row-axis table entries (L*n) kept in LDS
 constexpr int kBwdTable = 1024;     // largest loop axis tabulated by the backward kernel
 
-// Dynamic LDS: [row weights: L*n FP64][loop table: 2n T][row terms: L*n T]
+// Dynamic LDS: [row weights: L*n FP64][loop table: 2 nl T][row terms: L*n T]
 template <typename T>
 inline size_t sweep_fwd_lds(const GridGeom& g, bool table, bool rowtab) {
-    size_t b = table ? (size_t)2 * g.n * sizeof(T) : 0;
+    size_t b = table ? (size_t)2 * g.nl * sizeof(T) : 0;
     if (rowtab) b += (size_t)g.L * g.n * (sizeof(double) + sizeof(T));
     return b;
 }
@@ -82,15 +85,14 @@ __global__ __launch_bounds__(kBlock) void k_sweep_fwd(SweepGeom S, const T* a, c
     __shared__ double s_red[kWavesPerBlock];
     double* s_rw = reinterpret_cast<double*>(smem_raw);
     T* s_tab = reinterpret_cast<T*>(smem_raw + (ROWTAB ? (size_t)S.g.L * S.g.n * sizeof(double) : 0));
-    T* s_rt = s_tab + 2 * (size_t)S.g.n;
+    T* s_rt = s_tab + 2 * (size_t)S.g.nl;
     const GridGeom& g = S.g;
     const long long b = blockIdx.y;
     load_sweep_params(g.d, g.n, g.rule, b, a, u, dom, &s_p, s_axis);
     __syncthreads();
     if (TABLE) {
-        const AxisC<T> cL = axis_c(s_p, g.L);
-        for (int j = threadIdx.x; j < g.n; j += kBlock)
-            table_entry<T, TQ_FAM_GAUSSIAN_PEAK>(s_tab, j, s_axis[g.L], cL, g.n, g.rule, glx, glw, s_p);
+        for (int j = threadIdx.x; j < g.nl; j += kBlock)
+            loop_table_entry<T, TQ_FAM_GAUSSIAN_PEAK>(s_tab, j, s_axis, g.L, g.q, g.n, g.rule, glx, glw, s_p);
         if (ROWTAB)
             for (int e = threadIdx.x; e < g.L * g.n; e += kBlock) {
                 const int k = e / g.n;

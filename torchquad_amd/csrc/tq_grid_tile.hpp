@@ -365,9 +365,10 @@ constexpr int kMergeMaxN = 32;
 // there are at least two axes between the row axes and the lane axes, the
 // loop runs over the last two non-lane axes (n^2 points per row, one table
 // entry per point pair), so a short axis does not pay the per-row cost
-// every n points.
+// every n points.  trade_lane_axis = false keeps the lane axes (and so the
+// launch size) of the unmerged plan.
 inline int plan_geom(int d, int n, int rule, int lane_axes, long long items_per_wave, long long target_waves,
-                     GridGeom& G, long long merge_cap = 0) {
+                     GridGeom& G, long long merge_cap = 0, bool trade_lane_axis = true) {
     G.d = d;
     G.n = n;
     G.rule = rule;
@@ -403,7 +404,8 @@ inline int plan_geom(int d, int n, int rule, int lane_axes, long long items_per_
         // One lane axis fewer when that lets the loop run over two short
         // axes (n^2 points per row instead of n) and the lanes still fill
         // three quarters of their waves.
-        if (merge_cap > 0 && d >= 3 && d - m == 1 && m >= 2 && n <= kMergeMaxN && (long long)n * n <= merge_cap) {
+        if (trade_lane_axis && merge_cap > 0 && d >= 3 && d - m == 1 && m >= 2 && n <= kMergeMaxN &&
+            (long long)n * n <= merge_cap) {
             long long c1 = 1;
             for (int k = 0; k < m - 1; ++k) c1 *= n;
             const double util = (double)c1 / (double)(((c1 + kWave - 1) / kWave) * kWave);

@@ -29,11 +29,15 @@ constexpr int kHistLds = 768;         // dim*n_intervals handled with LDS histog
 constexpr int kMapLds = 1024;         // doubles of map tables staged in LDS
 constexpr int kOffLds = kSB / 2 + 8;  // cube offsets of one sample block
 constexpr size_t kMapUpdateLds = 48 * 1024;  // largest update_map scratch kept in LDS
+// Histogram keys are (bin << 6 | lane) in 32 bits: bins up to 2^26 - 2, the
+// largest value marks "no bin".
+constexpr unsigned kNoBin = (1u << 26) - 1;
 
 static int check_state(const tq_vegas_state* st) {
     if (!st) return fail(TQ_E_NULL, "vegas state is NULL");
     if (st->dim < 1 || st->dim > TQ_MAX_DIM) return fail(TQ_E_ARG, "vegas dim out of range (1..16)");
     if (st->n_intervals < 2) return fail(TQ_E_ARG, "n_intervals must be >= 2");
+    if (st->n_intervals >= (int)kNoBin) return fail(TQ_E_ARG, "n_intervals must be < 2^26 - 1");
     if (st->n_strat < 1 || st->n_cubes < 1) return fail(TQ_E_ARG, "n_strat/n_cubes must be >= 1");
     if (st->sample_dtype != TQ_F32 && st->sample_dtype != TQ_F64) return fail(TQ_E_ARG, "bad sample dtype");
     if (st->max_blocks < 4) return fail(TQ_E_ARG, "max_blocks must be >= 4");
@@ -173,12 +177,14 @@ __global__ __launch_bounds__(kBlock) void k_vegas_offsets(tq_vegas_state st, int
 // ---------------------------------------------------------------------------
 // histograms
 // ---------------------------------------------------------------------------
-// Per-bin sums of the wave's (bin, value) pairs.  The pairs are sorted by
-// (bin, lane) with a bitonic network over the 64 lanes, then a segmented
-// inclusive scan sums each run of equal bins; the last lane of a run gets
-// head = true with the run's bin, value sum and lane count.  Lanes with
-// id < 0 take no part.  The summation order depends only on the lanes' bins,
-// so the sums are reproducible (no floating-point atomics).
+// Per-bin sums of the wave's (bin, value) pairs.  The 32-bit keys
+// (bin << 6 | lane) are sorted with a bitonic network over the 64 lanes,
+// each lane then fetches the value of the lane its sorted key came from,
+// and a segmented inclusive scan (DPP, no LDS) sums each run of equal bins;
+// the last lane of a run gets head = true with the run's bin, value sum and
+// lane count.
+// Lanes with id < 0 take no part.  The summation order depends only on the
+// lanes' bins, so the sums are reproducible (no floating-point atomics).
 struct BinSum {
     long long bin;
     double sum;
@@ -186,42 +192,93 @@ struct BinSum {
     bool head;
 };
 
+// x of lane (lane ^ S).  Strides 1, 2, 4 and 8 stay inside 16-lane rows and
+// use DPP moves (VALU, no LDS round trip): quad_perm [1,0,3,2] and
+// [2,3,0,1]; row_half_mirror (lane ^ 7) then quad_perm [3,2,1,0] (^ 3) for
+// 4; row_ror:8 for 8.  16 and 32 cross rows and go through ds_bpermute.
+template <int S>
+__device__ inline unsigned wave_xor(unsigned x) {
+    if constexpr (S == 1) {
+        return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);
+    } else if constexpr (S == 2) {
+        return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);
+    } else if constexpr (S == 4) {
+        const unsigned y = __builtin_amdgcn_mov_dpp(x, 0x141, 0xF, 0xF, false);
+        return __builtin_amdgcn_mov_dpp(y, 0x1B, 0xF, 0xF, false);
+    } else if constexpr (S == 8) {
+        return __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);
+    } else {
+        return (unsigned)__shfl_xor((int)x, S, kWave);
+    }
+}
+
+// One compare-exchange stage of the bitonic network (merge size SIZE,
+// partner distance STRIDE): ascending overall.
+template <int SIZE, int STRIDE>
+__device__ inline unsigned bitonic_stage(unsigned key, int lane) {
+    const unsigned other = wave_xor<STRIDE>(key);
+    const bool up = (lane & SIZE) == 0;
+    const bool lower = (lane & STRIDE) == 0;
+    return ((lower == up) ? (other < key) : (other > key)) ? other : key;
+}
+
+template <int SIZE, int STRIDE>
+__device__ inline unsigned bitonic_merge(unsigned key, int lane) {
+    key = bitonic_stage<SIZE, STRIDE>(key, lane);
+    if constexpr (STRIDE > 1) key = bitonic_merge<SIZE, STRIDE / 2>(key, lane);
+    return key;
+}
+
+template <int SIZE>
+__device__ inline unsigned bitonic_sort(unsigned key, int lane) {
+    if constexpr (SIZE > 2) key = bitonic_sort<SIZE / 2>(key, lane);
+    return bitonic_merge<SIZE, SIZE / 2>(key, lane);
+}
+
+// One step of the run scan: take (bin, sum, count) from the lane that the
+// DPP control CTRL selects (rows outside ROW_MASK and lanes whose source
+// leaves the row read nothing) and add it when the bins are equal.
+template <int CTRL, int ROW_MASK>
+__device__ inline void run_scan_step(unsigned bin, double& sum, int& cnt) {
+    const unsigned pb = (unsigned)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)bin, CTRL, ROW_MASK, 0xF, false);
+    const int pc = __builtin_amdgcn_update_dpp(0, cnt, CTRL, ROW_MASK, 0xF, false);
+    union {
+        double d;
+        int i[2];
+    } mine, other;
+    mine.d = sum;
+    other.i[0] = __builtin_amdgcn_update_dpp(0, mine.i[0], CTRL, ROW_MASK, 0xF, false);
+    other.i[1] = __builtin_amdgcn_update_dpp(0, mine.i[1], CTRL, ROW_MASK, 0xF, false);
+    if (pb == bin) {
+        sum += other.d;
+        cnt += pc;
+    }
+}
+
 __device__ inline BinSum wave_bin_sums(int id, double v) {
     const int lane = threadIdx.x & (kWave - 1);
-    constexpr unsigned long long kNone = 0x7fffffffULL;
-    unsigned long long key = ((id >= 0 ? (unsigned long long)id : kNone) << 6) | (unsigned long long)lane;
-    double val = id >= 0 ? v : 0.0;
-    for (int size = 2; size <= kWave; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            const unsigned long long ok = __shfl_xor(key, stride, kWave);
-            const double ov = __shfl_xor(val, stride, kWave);
-            const bool up = (lane & size) == 0;
-            const bool lower = (lane & stride) == 0;
-            const bool take = (lower == up) ? (ok < key) : (ok > key);
-            if (take) {
-                key = ok;
-                val = ov;
-            }
-        }
-    }
-    const long long bin = (long long)(key >> 6);
-    double sum = val;
+    const unsigned key = bitonic_sort<kWave>(((id >= 0 ? (unsigned)id : kNoBin) << 6) | (unsigned)lane, lane);
+    const unsigned bin = key >> 6;
+    const double mine = id >= 0 ? v : 0.0;
+    double sum = __shfl(mine, (int)(key & (kWave - 1)), kWave);
     int cnt = 1;
-    for (int off = 1; off < kWave; off <<= 1) {
-        const double ps = __shfl_up(sum, off, kWave);
-        const int pc = __shfl_up(cnt, off, kWave);
-        const long long pb = __shfl_up(bin, off, kWave);
-        if (lane >= off && pb == bin) {
-            sum += ps;
-            cnt += pc;
-        }
-    }
-    const long long nb = __shfl_down(bin, 1, kWave);
+    // Inclusive scan over runs of equal bins, which are contiguous because
+    // the keys are sorted: Hillis-Steele inside each 16-lane row (row_shr 1,
+    // 2, 4, 8), then row r+1 takes lane 16r+15 of rows 0 and 2 (row_bcast15)
+    // and rows 2 and 3 take lane 31 (row_bcast31), each added only when the
+    // bin continues.
+    run_scan_step<0x111, 0xF>(bin, sum, cnt);
+    run_scan_step<0x112, 0xF>(bin, sum, cnt);
+    run_scan_step<0x114, 0xF>(bin, sum, cnt);
+    run_scan_step<0x118, 0xF>(bin, sum, cnt);
+    run_scan_step<0x142, 0xA>(bin, sum, cnt);
+    run_scan_step<0x143, 0xC>(bin, sum, cnt);
+    const unsigned nb = (unsigned)__shfl_down((int)bin, 1, kWave);
     BinSum r;
-    r.bin = bin;
+    r.bin = (long long)bin;
     r.sum = sum;
     r.count = cnt;
-    r.head = (lane == kWave - 1 || nb != bin) && bin != (long long)kNone;
+    r.head = (lane == kWave - 1 || nb != bin) && bin != kNoBin;
     return r;
 }
 
