@@ -337,3 +337,44 @@ def test_add_at_indices_errors(dev):
     assert t.tolist() == [1.0, 0.0, 0.0, 1.0]
     _add_at_indices(t, torch.tensor([], dtype=torch.int64, device=dev), torch.ones(0, device=dev))
     assert t.tolist() == [1.0, 0.0, 0.0, 1.0]
+
+
+@pytest.mark.parametrize("N,dim", [(1500, 2), (3000, 1), (4100, 3), (64, 4), (65, 2)])
+def test_map_update_many_bins_matches_oracle(N, dim, dev):
+    """update_map with more bins than one block's threads (chunked block scans and run sums),
+    zero-count stretches longer than the ten fill passes close, against the oracle."""
+    rng = np.random.default_rng(N + dim)
+    m = tq.VEGASMap(N, dim, "torch", torch.float64)
+    om = ov.Map(N, dim)
+    w = rng.uniform(0.0, 3.0, (dim, N)) ** 3
+    c = rng.integers(0, 4, (dim, N))
+    c[:, N // 3:N // 3 + min(30, N // 4)] = 0  # a run of empty bins
+    w[c == 0] = 0.0
+    m._dev.weights.copy_(torch.tensor(w))
+    m._dev.counts.copy_(torch.tensor(c, dtype=torch.float64))
+    om.weights, om.counts = w.copy(), c.copy()
+    sw_dev = tq.VEGASMap._smooth_map(torch.tensor(w, device=dev), torch.tensor(c, device=dev), 0.5)
+    sw_ref = ov.Map.smooth(w, c, 0.5)
+    np.testing.assert_allclose(sw_dev.cpu().numpy(), sw_ref, rtol=1e-13, atol=1e-15)
+    m.update_map()
+    om.update_map()
+    x_dev = m.x_edges.cpu().numpy()
+    np.testing.assert_allclose(x_dev, om.x_edges, rtol=0, atol=1e-12)
+    assert np.all(np.diff(x_dev, axis=1) > 0) and np.all(x_dev[:, 0] == 0.0) and np.all(x_dev[:, -1] == 1.0)
+
+
+def test_default_bins_vegas_runs_fast(dev):
+    """The reference's default map size (N_increment // 10 bins: 4000 here) on a 1e6-eval run."""
+    import time
+
+    f = tq.DoubleGaussian(6, width=0.1)
+    dom = torch.tensor([[0.0, 1.0]] * 6, dtype=torch.float32, device=dev)
+    v = tq.VEGAS()
+    v.integrate(f, 6, N=10**6, integration_domain=dom, seed=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    val = float(v.integrate(f, 6, N=10**6, integration_domain=dom, seed=2))
+    torch.cuda.synchronize()
+    assert v.map.N_intervals == 4000
+    assert abs(val - 1.0) < 0.02
+    assert time.perf_counter() - t0 < 1.0

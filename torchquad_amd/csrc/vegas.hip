@@ -9,7 +9,7 @@
 //   k_vegas_sample  y, map, Jacobian, integrand, jf, map histograms
 //   k_vegas_strat   per-cube JF/JF2 (segmented, fixed order), ih, sigma^2, dh
 //   k_vegas_strat_finalize   iteration result and dh normaliser
-//   k_vegas_map_reduce + k_vegas_map_update   update_map
+//   k_vegas_map_reduce + k_vegas_map_smooth + k_vegas_map_edges   update_map
 // Map histograms are accumulated per wave in LDS by looping over the distinct
 // bins present in the wave (ballot + fixed-order wave reduction), combined in
 // wave order per block and reduced over blocks in block order: the whole
@@ -859,169 +859,275 @@ __device__ inline void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Whole update_map in one block, wave k handles axis k (vegas_map.py:113-261).
-// Per-bin arithmetic (weight/count, smoothing with its log and sqrt, the new
-// edges) runs across the lanes; the zero-count fill, the sums and the
-// cumulative scans run on lane 0 in bin order, so the arithmetic is that of
-// smooth_fill_axis / smooth_finish_axis / update_axis.  Scratch per axis:
-// 5 rows of n+1 doubles, in LDS when lds_scratch (dynamic shared memory of
-// dim*5*(n+1) doubles), else in hist_w.
-__global__ __launch_bounds__(1024) void k_vegas_map_update(tq_vegas_state st, int reset, int lds_scratch) {
-    extern __shared__ double s_scr[];
-    __shared__ double s_sum[TQ_MAX_DIM];
-    __shared__ int s_status[TQ_MAX_DIM];
-    __shared__ int s_anyzero;
-    const int k = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
-    const int d = st.dim, n = st.n_intervals;
-    if (threadIdx.x == 0) s_anyzero = 0;
+// ---- update_map: one block per axis, two kernels (vegas_map.py:113-261) ----
+// k_vegas_map_smooth: weight/count (or the raw weight when a count is zero
+// anywhere), the zero-count fill, the row sum and the smoothed weights.
+// k_vegas_map_edges: the "cannot update" test over all axes' sums, then the
+// new edges from cumulative sums, all as block-wide scans, so the work per
+// axis is O(n / 1024) steps rather than a walk of n bins on one lane.
+// Scratch per axis (hist_w, 5 rows of n+1 doubles): row 1 slot n holds the
+// row sum for the second kernel, row 2 the smoothed weights; rows 0, 3, 4
+// are work space (in LDS instead when 5 (n+1) doubles fit).
+constexpr int kMapBlock = 1024;
+constexpr int kMapWaves = kMapBlock / kWave;
+
+// Sum over the block in a fixed order; every thread gets the result.
+__device__ inline double map_block_sum(double v, double* s_red) {
+    v = wave_sum(v);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_red[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (k < d)
-        for (int j = lane; j < n; j += kWave)
-            if (st.counts[k * n + j] == 0.0) atomicOr(&s_anyzero, 1);
+    double r = 0.0;
+    for (int w = 0; w < kMapWaves; ++w) r += s_red[w];
     __syncthreads();
-    const bool any_zero = s_anyzero != 0;
-    double* scr = (lds_scratch ? s_scr : st.hist_w) + (long long)k * 5 * (n + 1);
-    double* w = scr;
-    double* z = scr + (n + 1);
-    double* sw = scr + 2 * (n + 1);
-    double* newx = scr + 3 * (n + 1);
-    if (k < d) {
-        const double* wt = st.weights + k * n;
-        const double* ct = st.counts + k * n;
-        for (int j = lane; j < n; j += kWave) {
-            z[j] = (ct[j] == 0.0) ? 1.0 : 0.0;
-            w[j] = (!any_zero || ct[j] != 0.0) ? wt[j] / ct[j] : wt[j];
-        }
-        wave_lds_sync();
-        if (lane == 0) {
-            if (any_zero) {
-                for (int pass = 0; pass < 10; ++pass) {
-                    bool left = false;
-                    for (int j = 0; j < n - 1; ++j)
-                        if (z[j] != 0.0) w[j] = w[j + 1];
-                    for (int j = 0; j < n - 1; ++j) z[j] = (z[j] != 0.0 && z[j + 1] != 0.0) ? 1.0 : 0.0;
-                    for (int j = n - 1; j >= 1; --j)
-                        if (z[j] != 0.0) w[j] = w[j - 1];
-                    for (int j = n - 1; j >= 1; --j) z[j] = (z[j] != 0.0 && z[j - 1] != 0.0) ? 1.0 : 0.0;
-                    for (int j = 0; j < n; ++j) left = left || (z[j] != 0.0);
-                    if (!left) break;
-                }
-            }
-            double sum = 0.0;
-            for (int j = 0; j < n; ++j) sum += w[j];
-            s_sum[k] = sum;
-        }
+    return r;
+}
+
+// Inclusive scan of one value per thread, plus `carry`; `carry` becomes the
+// running total after this chunk.
+__device__ inline double map_block_scan(double v, double& carry, double* s_red) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
+    for (int off = 1; off < kWave; off <<= 1) {
+        const double o = __shfl_up(v, off, kWave);
+        if (lane >= off) v += o;
+    }
+    if (lane == kWave - 1) s_red[w] = v;
+    __syncthreads();
+    double pre = carry, all = carry;
+    for (int i = 0; i < kMapWaves; ++i) {
+        if (i < w) pre += s_red[i];
+        all += s_red[i];
     }
     __syncthreads();
-    bool cannot = false;
-    for (int i = 0; i < d; ++i) cannot = cannot || (s_sum[i] == 0.0);
-    if (cannot) {
-        if (threadIdx.x == 0) st.flags[0] += 1;
-    } else if (k < d) {
-        const double sum = s_sum[k];
-        const double alpha = st.alpha;
-        // [1,6,1]/8 smoothing (edges [7,1]/8) and range compression (EQ 18-19)
-        for (int j = lane; j < n; j += kWave) {
-            double dd;
-            if (j == 0) dd = 7.0 * w[0] + w[1];
-            else if (j == n - 1) dd = w[n - 2] + 7.0 * w[n - 1];
-            else dd = w[j - 1] + 6.0 * w[j] + w[j + 1];
-            dd = dd / (8.0 * sum);
-            if (dd != 0.0) {
-                const double r = (dd - 1.0) / log(dd);
-                dd = (alpha == 0.5) ? sqrt(r) : pow(r, alpha);
+    carry = all;
+    return v + pre;
+}
+
+__device__ inline double axis_weight(const tq_vegas_state& st, int k, int j, bool any_zero) {
+    const int n = st.n_intervals;
+    const double wt = st.weights[k * n + j], ct = st.counts[k * n + j];
+    return (!any_zero || ct != 0.0) ? wt / ct : wt;
+}
+
+__global__ __launch_bounds__(kMapBlock) void k_vegas_map_smooth(tq_vegas_state st, int lds) {
+    extern __shared__ double s_dyn[];
+    __shared__ double s_red[kMapWaves];
+    const int k = blockIdx.x, tid = threadIdx.x;
+    const int d = st.dim, n = st.n_intervals, W = d * n;
+    double* g = st.hist_w + (long long)k * 5 * (n + 1);
+    double* base = lds ? s_dyn : g;
+    double* w = base;
+    double* z = base + (n + 1);
+    double* w2 = base + 3 * (n + 1);
+    double* z2 = base + 4 * (n + 1);
+    int zero_here = 0;
+    for (int e = tid; e < W; e += kMapBlock) zero_here |= (st.counts[e] == 0.0) ? 1 : 0;
+    const bool any_zero = __syncthreads_or(zero_here) != 0;
+    for (int j = tid; j < n; j += kMapBlock) {
+        z[j] = (st.counts[k * n + j] == 0.0) ? 1.0 : 0.0;
+        w[j] = axis_weight(st, k, j, any_zero);
+    }
+    __syncthreads();
+    if (any_zero) {
+        // up to 10 passes, each reading only values from before the pass
+        for (int pass = 0; pass < 10; ++pass) {
+            for (int j = tid; j < n; j += kMapBlock) {
+                const bool zj = z[j] != 0.0;
+                w2[j] = (j < n - 1 && zj) ? w[j + 1] : w[j];
+                z2[j] = (j < n - 1) ? ((zj && z[j + 1] != 0.0) ? 1.0 : 0.0) : z[j];
             }
-            sw[j] = dd;
+            __syncthreads();
+            int left = 0;
+            for (int j = tid; j < n; j += kMapBlock) {
+                const bool zj = z2[j] != 0.0;
+                w[j] = (j >= 1 && zj) ? w2[j - 1] : w2[j];
+                const bool zn = (j >= 1) ? (zj && z2[j - 1] != 0.0) : zj;
+                z[j] = zn ? 1.0 : 0.0;
+                left |= zn ? 1 : 0;
+            }
+            if (!__syncthreads_or(left)) break;
         }
-        wave_lds_sync();
-        // rows 0 and 1 now hold num / val, then per edge the accumulated
-        // remainder and the source bin of the new edge
-        double* num = scr;
-        double* val = scr + (n + 1);
+    }
+    double part = 0.0;
+    for (int j = tid; j < n; j += kMapBlock) part += w[j];
+    const double sum = map_block_sum(part, s_red);
+    // [1,6,1]/8 smoothing (edges [7,1]/8) and range compression (EQ 18-19)
+    const double alpha = st.alpha;
+    double* sw = g + 2 * (n + 1);
+    for (int j = tid; j < n; j += kMapBlock) {
+        double dd;
+        if (j == 0) dd = 7.0 * w[0] + w[1];
+        else if (j == n - 1) dd = w[n - 2] + 7.0 * w[n - 1];
+        else dd = w[j - 1] + 6.0 * w[j] + w[j + 1];
+        dd = dd / (8.0 * sum);
+        if (dd != 0.0) {
+            const double r = (dd - 1.0) / log(dd);
+            dd = (alpha == 0.5) ? sqrt(r) : pow(r, alpha);
+        }
+        sw[j] = dd;
+    }
+    if (tid == 0) g[(n + 1) + n] = sum;
+}
+
+__global__ __launch_bounds__(kMapBlock) void k_vegas_map_edges(tq_vegas_state st, int reset, int lds) {
+    extern __shared__ double s_dyn[];
+    __shared__ double s_red[kMapWaves];
+    __shared__ unsigned s_rk[kMapWaves], s_ck[kMapWaves + 1];
+    __shared__ double s_rs[kMapWaves], s_cs[kMapWaves + 1];
+    __shared__ int s_rc[kMapWaves], s_cc[kMapWaves + 1];
+    const int k = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
+    const int d = st.dim, n = st.n_intervals;
+    double* g = st.hist_w + (long long)k * 5 * (n + 1);
+    bool cannot = false;
+    for (int a = 0; a < d; ++a) cannot = cannot || (st.hist_w[(long long)a * 5 * (n + 1) + (n + 1) + n] == 0.0);
+    if (cannot) {
+        if (k == 0 && tid == 0) st.flags[0] += 1;
+    } else {
+        double* base = lds ? s_dyn : g;
+        double* sw = base + 2 * (n + 1);
+        if (lds) {
+            for (int j = tid; j < n; j += kMapBlock) sw[j] = g[2 * (n + 1) + j];
+            __syncthreads();
+        }
+        int* mlt = reinterpret_cast<int*>(base);  // row 0, later newx
+        double* newx = base;
+        double* num = base + 3 * (n + 1);
+        double* val = base + 4 * (n + 1);
         double* xe = st.x_edges + k * (n + 1);
         double* dxe = st.dx_edges + k * n;
-        if (lane == 0) {
-            int status = 0;
-            double total = 0.0;
-            for (int j = 0; j < n; ++j) total += sw[j];
-            const double delta_d = total / (double)n;
-            for (int m = 0; m <= n; ++m) {
-                num[m] = 0.0;
-                val[m] = 0.0;
-            }
-            double cum = 0.0;
-            for (int j = 0; j < n - 1 && !status; ++j) {
-                cum += sw[j];
+        double part = 0.0;
+        for (int j = tid; j < n; j += kMapBlock) part += sw[j];
+        const double total = map_block_sum(part, s_red);
+        const double delta_d = total / (double)n;
+        // bin of each cumulative sum: mlt[j] = floor(cum_j / delta_d), j < n-1
+        int bad = 0;
+        double carry = 0.0;
+        for (int j0 = 0; j0 < n - 1; j0 += kMapBlock) {
+            const int j = j0 + tid;
+            const double cum = map_block_scan(j < n - 1 ? sw[j] : 0.0, carry, s_red);
+            if (j < n - 1) {
                 const double q = cum / delta_d;
-                const long long mlt = (q >= 0.0 && q <= (double)n) ? (long long)q : -1;
-                if (mlt < 0) {
-                    status = 3;  // NaN / out of range: edges cannot be formed
+                if (q >= 0.0 && q <= (double)n) {
+                    mlt[j] = (int)q;
                 } else {
-                    num[mlt] += 1.0;
-                    val[mlt] += sw[j];
+                    mlt[j] = 0;
+                    bad = 1;
                 }
             }
-            if (!status) {
-                long long idx = 0;
-                double accu = 0.0;
-                for (int i = 0; i < n - 1; ++i) {
-                    idx += (long long)num[i];
-                    accu += delta_d - val[i];
-                    num[i] = accu;
-                    val[i] = (double)(idx > n - 1 ? n - 1 : idx);
-                }
-            }
-            s_status[k] = status;
         }
-        wave_lds_sync();
-        if (s_status[k] == 0) {
-            for (int i = lane; i < n - 1; i += kWave) {
-                const int ix = (int)val[i];
-                newx[i + 1] = xe[ix] + num[i] / sw[ix] * dxe[ix];
-            }
-            if (lane == 0) {
-                newx[0] = xe[0];
-                newx[n] = xe[n];
-            }
-            wave_lds_sync();
-            if (lane == 0) {
-                int fl = 0;
-                bool all_finite = true;
-                for (int j = 0; j <= n; ++j) all_finite = all_finite && isfinite(newx[j]);
-                if (!all_finite) {
-                    fl |= 1;
-                    // replace by the mean of the neighbours (vegas_map.py:252-255)
-                    for (int j = 1; j < n; ++j) val[j] = isfinite(newx[j]) ? 1.0 : 0.0;
-                    for (int j = 1; j < n; ++j) num[j] = 0.5 * (newx[j - 1] + newx[j + 1]);
-                    for (int j = 1; j < n; ++j)
-                        if (val[j] == 0.0) newx[j] = num[j];
-                    for (int j = 0; j <= n; ++j)
-                        if (!isfinite(newx[j])) fl |= 2;
+        for (int m = tid; m <= n; m += kMapBlock) {
+            num[m] = 0.0;
+            val[m] = 0.0;
+        }
+        const int status = __syncthreads_or(bad) ? 3 : 0;
+        if (status == 0) {
+            // per-bin counts and weight sums of the cumulative bins: runs of
+            // equal mlt are contiguous (cum is nondecreasing); DPP run scan in
+            // each wave, carried across waves and chunks in wave order
+            unsigned ck = 0xFFFFFFFFu;
+            double cs = 0.0;
+            int cc = 0;
+            for (int j0 = 0; j0 < n - 1; j0 += kMapBlock) {
+                const int j = j0 + tid;
+                const bool valid = j < n - 1;
+                const unsigned key = valid ? (unsigned)mlt[j] : 0xFFFFFFFEu;
+                double sum = valid ? sw[j] : 0.0;
+                int cnt = valid ? 1 : 0;
+                run_scan_step<0x111, 0xF>(key, sum, cnt);
+                run_scan_step<0x112, 0xF>(key, sum, cnt);
+                run_scan_step<0x114, 0xF>(key, sum, cnt);
+                run_scan_step<0x118, 0xF>(key, sum, cnt);
+                run_scan_step<0x142, 0xA>(key, sum, cnt);
+                run_scan_step<0x143, 0xC>(key, sum, cnt);
+                if (lane == kWave - 1) {
+                    s_rk[wv] = key;
+                    s_rs[wv] = sum;
+                    s_rc[wv] = cnt;
                 }
-                s_status[k] = fl;
+                __syncthreads();
+                if (tid == 0) {
+                    for (int i = 0; i < kMapWaves; ++i) {
+                        s_ck[i] = ck;
+                        s_cs[i] = cs;
+                        s_cc[i] = cc;
+                        if (s_rk[i] == ck) {
+                            cs = s_rs[i] + cs;
+                            cc = s_rc[i] + cc;
+                        } else {
+                            ck = s_rk[i];
+                            cs = s_rs[i];
+                            cc = s_rc[i];
+                        }
+                    }
+                    s_ck[kMapWaves] = ck;
+                    s_cs[kMapWaves] = cs;
+                    s_cc[kMapWaves] = cc;
+                }
+                __syncthreads();
+                if (key == s_ck[wv]) {
+                    sum += s_cs[wv];
+                    cnt += s_cc[wv];
+                }
+                if (valid && (j == n - 2 || mlt[j + 1] != (int)key)) {
+                    num[key] = (double)cnt;
+                    val[key] = sum;
+                }
+                ck = s_ck[kMapWaves];
+                cs = s_cs[kMapWaves];
+                cc = s_cc[kMapWaves];
+                __syncthreads();
             }
-            wave_lds_sync();
-            const int fl = s_status[k];
+            // new edge i+1 from the running count and remainder (vegas_map.py:238-249)
+            double cidx = 0.0, cacc = 0.0;
+            for (int i0 = 0; i0 < n - 1; i0 += kMapBlock) {
+                const int i = i0 + tid;
+                const bool valid = i < n - 1;
+                const double idx = map_block_scan(valid ? num[i] : 0.0, cidx, s_red);
+                const double accu = map_block_scan(valid ? delta_d - val[i] : 0.0, cacc, s_red);
+                if (valid) {
+                    const long long ix0 = (long long)idx;
+                    const int ix = (int)(ix0 > n - 1 ? n - 1 : ix0);
+                    val[i] = xe[ix] + accu / sw[ix] * dxe[ix];  // staged: newx shares row 0 with mlt
+                }
+            }
+            __syncthreads();
+            for (int i = tid; i <= n; i += kMapBlock)
+                newx[i] = (i == 0) ? xe[0] : ((i == n) ? xe[n] : val[i - 1]);
+            __syncthreads();
+            int nonfinite = 0;
+            for (int i = tid; i <= n; i += kMapBlock) nonfinite |= isfinite(newx[i]) ? 0 : 1;
+            int fl = 0;
+            if (__syncthreads_or(nonfinite)) {
+                fl |= 1;
+                // replace by the mean of the neighbours (vegas_map.py:252-255)
+                for (int i = tid; i <= n; i += kMapBlock)
+                    num[i] = (i >= 1 && i < n && !isfinite(newx[i])) ? 0.5 * (newx[i - 1] + newx[i + 1]) : newx[i];
+                __syncthreads();
+                int still = 0;
+                for (int i = tid; i <= n; i += kMapBlock) {
+                    newx[i] = num[i];
+                    still |= isfinite(num[i]) ? 0 : 1;
+                }
+                if (__syncthreads_or(still)) fl |= 2;
+            }
             if (!(fl & 2)) {
-                for (int j = lane; j <= n; j += kWave) xe[j] = newx[j];
-                for (int j = lane; j < n; j += kWave) dxe[j] = newx[j + 1] - newx[j];
+                for (int i = tid; i <= n; i += kMapBlock) xe[i] = newx[i];
+                for (int i = tid; i < n; i += kMapBlock) dxe[i] = newx[i + 1] - newx[i];
             }
-            if (lane == 0) {
+            if (tid == 0) {
                 if (fl & 1) atomicAdd(&st.flags[1], 1);
                 if (fl & 2) atomicAdd(&st.flags[2], 1);
             }
-        } else if (lane == 0) {
+        } else if (tid == 0) {
             atomicAdd(&st.flags[1], 1);
             atomicAdd(&st.flags[2], 1);
         }
     }
     if (reset) {
-        __syncthreads();
-        if (k < d)
-            for (int j = lane; j < n; j += kWave) {
-                st.weights[k * n + j] = 0.0;
-                st.counts[k * n + j] = 0.0;
-            }
+        // the other axes' blocks read only the row sums and their own rows
+        for (int j = tid; j < n; j += kMapBlock) {
+            st.weights[k * n + j] = 0.0;
+            st.counts[k * n + j] = 0.0;
+        }
     }
 }
 
@@ -1103,6 +1209,19 @@ static int launch_strat(const tq_vegas_state* st, const double* w, int it, int u
     return check_launch("k_vegas_strat_finalize");
 }
 
+// update_map on the accumulated weights/counts: smoothing, then the edges;
+// `reset` zeroes the weights and counts afterwards (_reset_weight).
+static int launch_map_kernels(const tq_vegas_state* st, int reset, hipStream_t s) {
+    const size_t scr = (size_t)5 * (st->n_intervals + 1) * sizeof(double);
+    const int lds = scr <= kMapUpdateLds ? 1 : 0;
+    hipLaunchKernelGGL(k_vegas_map_smooth, dim3((unsigned)st->dim), dim3(kMapBlock), lds ? scr : 0, s, *st, lds);
+    int rc = check_launch("k_vegas_map_smooth");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_vegas_map_edges, dim3((unsigned)st->dim), dim3(kMapBlock), lds ? scr : 0, s, *st, reset,
+                       lds);
+    return check_launch("k_vegas_map_edges");
+}
+
 static int launch_map_update(const tq_vegas_state* st, long long nsamp, int it, hipStream_t s) {
     const int W = st->dim * st->n_intervals;
     if (W <= kHistLds) {
@@ -1111,11 +1230,7 @@ static int launch_map_update(const tq_vegas_state* st, long long nsamp, int it, 
         int rc = check_launch("k_vegas_map_reduce");
         if (rc) return rc;
     }
-    const size_t scr = (size_t)st->dim * 5 * (st->n_intervals + 1) * sizeof(double);
-    const int lds = scr <= kMapUpdateLds ? 1 : 0;
-    hipLaunchKernelGGL(k_vegas_map_update, dim3(1), dim3(kWave * st->dim), lds ? scr : 0, s, *st, W <= kHistLds ? 0 : 1,
-                       lds);
-    return check_launch("k_vegas_map_update");
+    return launch_map_kernels(st, W <= kHistLds ? 0 : 1, s);
 }
 
 __global__ void k_zero(double* p, long long n) {
@@ -1336,11 +1451,7 @@ int tq_vegas_map_accumulate(const tq_vegas_state* st, const void* y, const void*
 int tq_vegas_map_update_accumulated(const tq_vegas_state* st, void* stream) {
     int rc = check_state(st);
     if (rc) return rc;
-    const size_t scr = (size_t)st->dim * 5 * (st->n_intervals + 1) * sizeof(double);
-    const int lds = scr <= kMapUpdateLds ? 1 : 0;
-    hipLaunchKernelGGL(k_vegas_map_update, dim3(1), dim3(kWave * st->dim), lds ? scr : 0, as_stream(stream), *st, 1,
-                       lds);
-    return check_launch("k_vegas_map_update");
+    return launch_map_kernels(st, 1, as_stream(stream));
 }
 
 int tq_vegas_smooth(int32_t dim, int32_t n, const double* weights, const double* counts, double alpha, double* out,
