@@ -30,12 +30,15 @@ struct SweepGeom {
 // loop-axis table, 1 + 3 dim block sums) is spread over many points.
 constexpr long long kSweepWaves = 16384;
 
-// One loop axis in both kernels (the backward tile walks one; a two-axis
-// loop in the forward kernel measured slower at 1024 x GL 32^4, whose blocks
-// are short and spend their start building the n^2-entry table).
-static int sweep_geom(const tq_grid_desc* gd, long long B, SweepGeom& S) {
+// The backward tile walks one loop axis.  The forward kernel may loop over
+// two (tables up to kSweepMergeCap entries) with the same lane axes, so both
+// plans have the same waves and the backward's workspace covers both.
+constexpr long long kSweepMergeCap = 1024;
+
+static int sweep_geom(const tq_grid_desc* gd, long long B, SweepGeom& S, long long merge_cap = 0) {
     const long long per_integral = B > 0 ? (kSweepWaves + B - 1) / B : kSweepWaves;
-    int rc = plan_geom(gd->dim, gd->n, gd->rule, gd->lane_axes, gd->items_per_wave, per_integral, S.g);
+    int rc = plan_geom(gd->dim, gd->n, gd->rule, gd->lane_axes, gd->items_per_wave, per_integral, S.g, merge_cap,
+                       false);
     if (rc) return rc;
     S.blocks_x = (S.g.num_waves + kWavesPerBlock - 1) / kWavesPerBlock;
     return TQ_OK;
@@ -76,13 +79,16 @@ inline size_t sweep_fwd_lds(const GridGeom& g, bool table, bool rowtab) {
     return b;
 }
 
-template <typename T, bool TABLE, bool ROWTAB>
-__global__ __launch_bounds__(kBlock) void k_sweep_fwd(SweepGeom S, const T* a, const T* u, const T* dom,
-                                                     const double* glx, const double* glw, double* partials) {
+// BW waves per block: 16 with a two-axis loop (one block per integral
+// builds the n^2-entry table once), else 4.
+template <typename T, bool TABLE, bool ROWTAB, int BW>
+__global__ __launch_bounds__(BW * kWave) void k_sweep_fwd(SweepGeom S, const T* a, const T* u, const T* dom,
+                                                         const double* glx, const double* glw, double* partials) {
+    constexpr int NT = BW * kWave;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ Params<T> s_p;
     __shared__ Axis s_axis[TQ_MAX_DIM];
-    __shared__ double s_red[kWavesPerBlock];
+    __shared__ double s_red[BW];
     double* s_rw = reinterpret_cast<double*>(smem_raw);
     T* s_tab = reinterpret_cast<T*>(smem_raw + (ROWTAB ? (size_t)S.g.L * S.g.n * sizeof(double) : 0));
     T* s_rt = s_tab + 2 * (size_t)S.g.nl;
@@ -91,17 +97,17 @@ __global__ __launch_bounds__(kBlock) void k_sweep_fwd(SweepGeom S, const T* a, c
     load_sweep_params(g.d, g.n, g.rule, b, a, u, dom, &s_p, s_axis);
     __syncthreads();
     if (TABLE) {
-        for (int j = threadIdx.x; j < g.nl; j += kBlock)
+        for (int j = threadIdx.x; j < g.nl; j += NT)
             loop_table_entry<T, TQ_FAM_GAUSSIAN_PEAK>(s_tab, j, s_axis, g.L, g.q, g.n, g.rule, glx, glw, s_p);
         if (ROWTAB)
-            for (int e = threadIdx.x; e < g.L * g.n; e += kBlock) {
+            for (int e = threadIdx.x; e < g.L * g.n; e += NT) {
                 const int k = e / g.n;
                 row_table_entry<T, TQ_FAM_GAUSSIAN_PEAK>(s_rt, s_rw, k, e - k * g.n, s_axis[k], axis_c(s_p, k), g.n,
                                                          g.rule, glx, glw, s_p);
             }
         __syncthreads();
     }
-    const long long wave = (long long)blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long long wave = (long long)blockIdx.x * BW + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     double acc = 0.0;
     if (wave < g.num_waves) {
         const int lw = (int)(wave % g.lane_waves);
@@ -450,29 +456,40 @@ static int check_sweep(const tq_grid_desc* g, int64_t B) {
     return TQ_OK;
 }
 
+template <typename T, bool TABLE, bool ROWTAB, int BW>
+static void launch_fwd_kernel(const SweepGeom& S, int64_t B, const void* a, const void* u, const void* dom,
+                              const tq_grid_desc* gd, void* ws, size_t shmem, unsigned gx, hipStream_t s) {
+    hipLaunchKernelGGL((k_sweep_fwd<T, TABLE, ROWTAB, BW>), dim3(gx, (unsigned)B), dim3(BW * kWave), shmem, s, S,
+                       (const T*)a, (const T*)u, (const T*)dom, gd->gl_nodes, gd->gl_weights, (double*)ws);
+}
+
 template <typename T>
 static int launch_sweep_fwd(const tq_grid_desc* gd, int64_t B, const void* a, const void* u, const void* dom,
                             void* ws, size_t wsb, void* out, double* out64, hipStream_t s) {
     SweepGeom S;
-    int rc = sweep_geom(gd, B, S);
+    int rc = sweep_geom(gd, B, S, kSweepMergeCap);
     if (rc) return rc;
     if (!ws || wsb < (size_t)(S.blocks_x * B) * sizeof(double)) return fail(TQ_E_WORKSPACE, "sweep workspace too small");
     if (S.blocks_x > 0x7fffffff) return fail(TQ_E_ARG, "too many blocks");
-    dim3 grid((unsigned)S.blocks_x, (unsigned)B);
-    if (S.g.use_table && S.g.L >= 1 && (long long)S.g.L * S.g.n <= kSweepRowTab)
-        hipLaunchKernelGGL((k_sweep_fwd<T, true, true>), grid, dim3(kBlock), sweep_fwd_lds<T>(S.g, true, true), s, S,
-                           (const T*)a, (const T*)u, (const T*)dom, gd->gl_nodes, gd->gl_weights, (double*)ws);
-    else if (S.g.use_table)
-        hipLaunchKernelGGL((k_sweep_fwd<T, true, false>), grid, dim3(kBlock), sweep_fwd_lds<T>(S.g, true, false), s,
-                           S, (const T*)a, (const T*)u, (const T*)dom, gd->gl_nodes, gd->gl_weights, (double*)ws);
-    else
-        hipLaunchKernelGGL((k_sweep_fwd<T, false, false>), grid, dim3(kBlock), 0, s, S, (const T*)a, (const T*)u,
-                           (const T*)dom, gd->gl_nodes, gd->gl_weights, (double*)ws);
+    const bool wide = S.g.q == 2;
+    const int bw = wide ? 16 : kWavesPerBlock;
+    const unsigned gx = (unsigned)((S.g.num_waves + bw - 1) / bw);
+    if (S.g.use_table && S.g.L >= 1 && (long long)S.g.L * S.g.n <= kSweepRowTab) {
+        const size_t sh = sweep_fwd_lds<T>(S.g, true, true);
+        if (wide) launch_fwd_kernel<T, true, true, 16>(S, B, a, u, dom, gd, ws, sh, gx, s);
+        else launch_fwd_kernel<T, true, true, kWavesPerBlock>(S, B, a, u, dom, gd, ws, sh, gx, s);
+    } else if (S.g.use_table) {
+        const size_t sh = sweep_fwd_lds<T>(S.g, true, false);
+        if (wide) launch_fwd_kernel<T, true, false, 16>(S, B, a, u, dom, gd, ws, sh, gx, s);
+        else launch_fwd_kernel<T, true, false, kWavesPerBlock>(S, B, a, u, dom, gd, ws, sh, gx, s);
+    } else {
+        launch_fwd_kernel<T, false, false, kWavesPerBlock>(S, B, a, u, dom, gd, ws, 0, gx, s);
+    }
     rc = check_launch("k_sweep_fwd");
     if (rc) return rc;
     const long long fb = (B + kWavesPerBlock - 1) / kWavesPerBlock;
     hipLaunchKernelGGL((k_sweep_fwd_finalize<T>), dim3((unsigned)fb), dim3(kBlock), 0, s, (const double*)ws,
-                       S.blocks_x, (long long)B, (const T*)dom, gd->dim, gd->n, gd->rule, (T*)out, out64);
+                       (long long)gx, (long long)B, (const T*)dom, gd->dim, gd->n, gd->rule, (T*)out, out64);
     return check_launch("k_sweep_fwd_finalize");
 }
 

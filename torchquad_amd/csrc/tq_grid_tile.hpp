@@ -84,17 +84,6 @@ struct TabStride {
 
 __host__ __device__ inline int tab_stride(int fam) { return fam == TQ_FAM_DOUBLE_GAUSSIAN ? 3 : 2; }
 
-template <typename T, int FAM>
-__device__ inline void table_entry(T* tab, int j, const Axis& axL, const AxisC<T>& c, int n, int rule,
-                                   const double* glx, const double* glw, const Params<T>& p) {
-    using F = Family<FAM, T>;
-    constexpr int S = TabStride<FAM>::v;
-    const State<T> s = F::fold_c(F::zero(), c, (T)node_of(axL, j, n, rule, glx), p);
-    tab[S * j] = s.s0;
-    if (S == 3) tab[S * j + 1] = s.s1;
-    tab[S * j + S - 1] = (T)weight_of(rule, j, n, glw);
-}
-
 // Loop-table entry j when the loop runs over q consecutive axes L .. L+q-1
 // (q = 1 or 2; for q = 2, j = j1 n + j2 with j1 on axis L): the axes'
 // contributions folded together, then the product of their weights.
@@ -117,7 +106,7 @@ __device__ inline void loop_table_entry(T* tab, int j, const Axis* ax, int L, in
 }
 
 // Row-axis tables (ROWTAB): for the row axes k < L, entry k*n + j holds the
-// axis's contribution at node j (the state slots of table_entry) in `rt` and
+// axis's contribution at node j (the state slots of loop_table_entry) in `rt` and
 // its weight in FP64 in `rw`, so a row's state and weight are merged from
 // the tables instead of being recomputed from the axis maps.
 template <typename T, int FAM>
