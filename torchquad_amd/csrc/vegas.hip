@@ -348,13 +348,15 @@ __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
     using F = Family<FAM, T>;
     __shared__ long long s_off[kOffLds];
     __shared__ double s_map[kMapLds];
-    __shared__ double s_hw[LDS_HIST ? kWavesPerBlock * kHistLds : 1];
-    __shared__ uint32_t s_hc[LDS_HIST ? kWavesPerBlock * kHistLds : 1];
+    // wave-private histograms in dynamic LDS sized to the map (hist_lds_bytes)
+    extern __shared__ __align__(16) unsigned char s_hist_raw[];
+    double* s_hw = reinterpret_cast<double*>(s_hist_raw);
     __shared__ long long s_hrange[2];
     const tq_vegas_state& st = A.st;
     const int tid = threadIdx.x, wave = tid >> 6;
     const int d = st.dim, ni = st.n_intervals;
     const int W = d * ni;
+    uint32_t* s_hc = reinterpret_cast<uint32_t*>(s_hist_raw + (size_t)kWavesPerBlock * W * sizeof(double));
 
     long long total;
     if (A.warmup) {
@@ -379,8 +381,8 @@ __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
     }
     if (MODE == MODE_FUSED && LDS_HIST && A.hist) {
         for (int i = tid; i < kWavesPerBlock * W; i += kBlock) {
-            s_hw[(i / W) * kHistLds + (i % W)] = 0.0;
-            s_hc[(i / W) * kHistLds + (i % W)] = 0u;
+            s_hw[(i / W) * W + (i % W)] = 0.0;
+            s_hc[(i / W) * W + (i % W)] = 0u;
         }
     }
     // cube range of this block
@@ -501,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
             for (int k = 0; k < d; ++k) {
                 const int id = active ? ids[k] : -1;
                 if (LDS_HIST)
-                    wave_hist_add(s_hw + wave * kHistLds, s_hc + wave * kHistLds, k, ni, id, jf2);
+                    wave_hist_add(s_hw + wave * W, s_hc + wave * W, k, ni, id, jf2);
                 else
                     wave_hist_add_global(st.weights, st.counts, k, ni, id, jf2);
             }
@@ -513,8 +515,8 @@ __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
             double w = 0.0;
             uint32_t c = 0;
             for (int wv = 0; wv < kWavesPerBlock; ++wv) {
-                w += s_hw[wv * kHistLds + e];
-                c += s_hc[wv * kHistLds + e];
+                w += s_hw[wv * W + e];
+                c += s_hc[wv * W + e];
             }
             st.hist_w[(long long)blockIdx.x * W + e] = w;
             st.hist_c[(long long)blockIdx.x * W + e] = c;
@@ -1265,9 +1267,12 @@ static int vegas_iterate(const tq_vegas_state* st, const tq_integrand* f, const 
     A.p = make_params<T>(*f);
     A.u_in = nullptr;
     const long long nb = sample_blocks(st, warmup, n_warm);
+    // wave-private histograms: (FP64 weight + uint32 count) per bin and wave
+    const size_t hist_lds = (size_t)kWavesPerBlock * W * (sizeof(double) + sizeof(uint32_t));
     TQ_DISPATCH_FAMILY(f->family, {
         if (lds_hist)
-            hipLaunchKernelGGL((k_vegas_sample<T, FAM, MODE_FUSED, true>), dim3((unsigned)nb), dim3(kBlock), 0, s, A);
+            hipLaunchKernelGGL((k_vegas_sample<T, FAM, MODE_FUSED, true>), dim3((unsigned)nb), dim3(kBlock), hist_lds,
+                               s, A);
         else
             hipLaunchKernelGGL((k_vegas_sample<T, FAM, MODE_FUSED, false>), dim3((unsigned)nb), dim3(kBlock), 0, s, A);
     });
