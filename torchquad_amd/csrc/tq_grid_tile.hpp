@@ -6,7 +6,7 @@
 namespace tq {
 
 constexpr int kFlush = 128;      // FP32 partial length before FP64 flush (at most 128 terms)
-constexpr int kLoopTable = 2048;  // largest n kept in the LDS loop table
+constexpr int kLoopTable = 2048;  // longest one-axis loop kept in the LDS loop table
 
 // Per-axis affine map.  Newton-Cotes: x_j = a + j*h (x_{n-1} = b exactly, as
 // linspace).  Gauss-Legendre: x_j = alpha*r_j + beta (gaussian.py:159-162).
@@ -73,10 +73,11 @@ __device__ __host__ inline double axis_scale(double a, double b, int n, int rule
     }
 }
 
-// Loop-axis LDS table: entry j holds the loop axis's contribution to the
-// integrand state at node j, fold_c(zero, c_L, x_j), then the node's weight;
-// per point the tile loop only merges it into the row/lane state.  Families
-// with a two-component state take three slots per entry.
+// Loop LDS table: entry j holds the loop's contribution to the integrand
+// state at its point j (fold_c(zero, c_L, x_j) for one loop axis, both axes
+// folded for two), then the point's weight; per point the tile loop only
+// merges it into the row/lane state.  Families with a two-component state
+// take three slots per entry.
 template <int FAM>
 struct TabStride {
     static constexpr int v = (FAM == TQ_FAM_DOUBLE_GAUSSIAN) ? 3 : 2;
