@@ -11,8 +11,9 @@
 //     x_k ~ (1 - 1/(8 n^2) + 1/(8 n^3)) cos(pi (k - 1/4) / (n + 1/2)),
 // with dP_n/dtheta = n (x P_n - P_{n-1}) / sin(theta) = n (D_n - z P_n) / sin(theta)
 // and weights w_k = 2 / (1 - x_k^2) / P_n'(x_k)^2 = 2 / (dP_n/dtheta)^2.
-// For n >= 200 the boundary nodes (n sin(theta) < 50, about 16 per end) use
-// the same recurrence, spread over a workgroup: the step (D_{j-1}, P_{j-1})
+// For n >= 200 the boundary nodes (n sin(theta) < 50, about 16 per end),
+// started from Gatteschi's Bessel-zero estimate, use the same recurrence,
+// spread over a workgroup: the step (D_{j-1}, P_{j-1})
 // -> (D_j, P_j) is linear, so each of the 256 (1024 for n >= 16384) lanes
 // forms the 2x2 product of its contiguous range of j, and an ordered tree
 // (wave shuffles, then LDS across waves) multiplies the products --
@@ -154,12 +155,38 @@ __device__ inline double tricomi_theta(int n, int k) {
     return (n == 1) ? 0.5 * M_PI : t0 + eps * cos(t0) / sin(t0);
 }
 
+// Zeros j_{0,k}, k = 1..24, of the Bessel function J_0 (A&S table 9.5).
+__constant__ double kBesselJ0Zeros[24] = {
+    2.4048255576957724, 5.5200781102863106, 8.6537279129110125, 11.791534439014281, 14.930917708487787,
+    18.071063967910924, 21.211636629879258, 24.352471530749302, 27.493479132040253, 30.634606468431976,
+    33.775820213573567, 36.917098353664045, 40.05842576462824,  43.19979171317673,  46.341188371661815,
+    49.482609897397815, 52.624051841114998, 55.765510755019982, 58.90698392608094,  62.048469190227166,
+    65.189964800206866, 68.331469329856802, 71.472981603593738, 74.614500643701831};
+
+// j_{0,k} (k >= 1): the table, then McMahon's expansion (A&S 9.5.12) with
+// beta = (k - 1/4) pi, good to 1e-14 relative from k = 20 on.
+__device__ inline double bessel_j0_zero(int k) {
+    if (k <= 24) return kBesselJ0Zeros[k - 1];
+    const double b = ((double)k - 0.25) * M_PI, e = 8.0 * b, e2 = e * e;
+    return b + 1.0 / e - 124.0 / (3.0 * e * e2) + 120928.0 / (15.0 * e * e2 * e2);
+}
+
+// Gatteschi's estimate of the angle of the k-th node from x = 1 (0-based):
+// theta_k ~ j_{0,k+1} / sqrt((n + 1/2)^2 + 1/12), relative error about
+// j^2 / (720 nu^4) -- near 1e-9 at n = 200 and the 16th node, far below at
+// larger n -- so Newton needs one step where Tricomi's interior estimate
+// needed three or four.
+__device__ inline double gatteschi_theta(int n, int k) {
+    const double rho = (double)n + 0.5;
+    return bessel_j0_zero(k + 1) / sqrt(rho * rho + 1.0 / 12.0);
+}
+
 // Boundary nodes k < nb of n >= kAsymMinN: one workgroup per node.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_gl_boundary(int n, double* xs, double* ws) {
     __shared__ double s_mat[4 * (BLOCK / kWave)];
     const int k = blockIdx.x;
-    double t = tricomi_theta(n, k);
+    double t = gatteschi_theta(n, k);
     double pn, dp;
     for (int it = 0; it < 16; ++it) {
         legendre_theta_block<BLOCK>(n, t, s_mat, &pn, &dp);
