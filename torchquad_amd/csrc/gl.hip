@@ -261,11 +261,20 @@ extern "C" int tq_gauss_legendre(int32_t n, double* nodes, double* weights, void
     if (n < 1) return fail(TQ_E_ARG, "n must be >= 1");
     if (!nodes || !weights) return fail(TQ_E_NULL, "nodes/weights is NULL");
     const int half = (n + 1) / 2;
-    // C_n = (4/pi) prod_{j=1..n} j/(j+1/2), in extended precision
-    long double cn = 4.0L / 3.14159265358979323846264338327950288L;
+    // C_n = (4/pi) prod_{j=1..n} j/(j+1/2) = (2/sqrt(pi)) Gamma(n+1)/Gamma(n+3/2).
+    // The log of the ratio from Stirling's series, -ln(n)/2 + sum_k c_k n^-k
+    // with c_k = (-1)^(k+1) (B_{k+1}(1) - B_{k+1}(3/2)) / (k (k+1)) (Bernoulli
+    // polynomials); eight terms leave a relative error below 1e-23 from
+    // n = 200 on, in O(1) host work instead of n extended-precision steps.
+    long double cn = 0.0L;
     int nb = 0;  // boundary nodes (n sin(theta_k) < 50) of the workgroup kernel
     if (n >= kAsymMinN) {
-        for (int j = 1; j <= n; ++j) cn *= (long double)j / ((long double)j + 0.5L);
+        static const long double kC[8] = {-3.0L / 8.0L,  1.0L / 8.0L,   -3.0L / 64.0L,     1.0L / 64.0L,
+                                          -3.0L / 640.0L, 1.0L / 384.0L, -33.0L / 14336.0L, 1.0L / 2048.0L};
+        const long double inv = 1.0L / (long double)n;
+        long double series = 0.0L;
+        for (int k = 7; k >= 0; --k) series = (series + kC[k]) * inv;
+        cn = 2.0L / sqrtl(3.14159265358979323846264338327950288L) * expl(series - 0.5L * logl((long double)n));
         while (nb < half && (double)n * sin(M_PI * (nb + 0.75) / (n + 0.5)) < kAsymMinNSin) ++nb;
         if (n >= kBoundaryLongN)
             hipLaunchKernelGGL((k_gl_boundary<1024>), dim3(nb), dim3(1024), 0, as_stream(stream), (int)n, nodes, weights);
