@@ -223,11 +223,16 @@ int tq_batch_plan(const tq_batch_item* items, int64_t count, int32_t dtype, int6
  * TQ_BATCH_COUNTER_WORDS;
  * class_offsets: host [4] from tq_batch_plan, or NULL to run every tile on
  * the general kernel.  Each class's tiles form 16 interleaved queues (tile
- * t in queue t mod 16); a wave drains its home queue, then the others. */
+ * t in queue t mod 16); a wave drains its home queue, then the others.
+ * gl_nodes / gl_weights: every Gauss-Legendre item's table at its
+ * gl_offset; gl_pairs32 (optional, F32 batches only, else ignored): the
+ * same tables as interleaved FP32 (node, weight) pairs, pair k at
+ * [2 (gl_offset + k)], read by one-axis items instead of the FP64 tables. */
 int tq_batch_run(int32_t dtype, const void* desc, int64_t count, const tq_batch_tile* tiles, int64_t n_tiles,
                  const int64_t* tile_begin, const int64_t* tile_count, const double* gl_nodes,
-                 const double* gl_weights, double* tile_partials, unsigned int* counter, int64_t counter_words,
-                 void* out, double* out64, int32_t blocks, const int64_t* class_offsets, void* stream);
+                 const double* gl_weights, const float* gl_pairs32, double* tile_partials, unsigned int* counter,
+                 int64_t counter_words, void* out, double* out64, int32_t blocks, const int64_t* class_offsets,
+                 void* stream);
 
 /* ---- differentiation ----------------------------------------------------
  * Parameter sweep of the Gaussian peak f = exp(-sum a_k^2 (x_k-u_k)^2) over B

@@ -139,7 +139,8 @@ SIGNATURES = {
     ),
     "tq_batch_plan_sizes": (ctypes.c_int, [_P, _I64, _I32, _I64, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "tq_batch_plan": (ctypes.c_int, [_P, _I64, _I32, _I64, _P, _P, _P, _P, _P]),
-    "tq_batch_run": (ctypes.c_int, [_I32, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _I32, _P, _P]),
+    "tq_batch_run": (ctypes.c_int, [_I32, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _I32, _P,
+                                    _P]),
     "tq_mc_workspace_size": (_SZ, [ctypes.POINTER(McDesc)]),
     "tq_mc_integrate": (
         ctypes.c_int,

@@ -20,7 +20,7 @@ import torch
 
 from . import _lib, _ops
 from ._lib import call, lib, ptr, stream_ptr
-from .gl_nodes import gauss_legendre
+from .gl_nodes import fp32_pairs, gauss_legendre
 from .integrands import BuiltinIntegrand
 
 RULE_NAMES = {
@@ -125,7 +125,7 @@ class Batch:
             self.evals += n**dim
             self.n[i], self.dims[i], self.families[i] = n, dim, d.family
         # Gauss-Legendre node/weight tables, one per distinct n, concatenated
-        self.gl_x = self.gl_w = None
+        self.gl_x = self.gl_w = self.gl_p32 = None
         if gl_ns:
             offsets = {}
             parts_x, parts_w = [], []
@@ -139,6 +139,8 @@ class Batch:
                     off += n
             self.gl_x = torch.cat(parts_x)
             self.gl_w = torch.cat(parts_w)
+            if dtype == torch.float32:
+                self.gl_p32 = fp32_pairs(self.gl_x, self.gl_w)
             for i, n in gl_ns:
                 arr[i].gl_offset = offsets[n]
         self._items = arr
@@ -182,7 +184,8 @@ class Batch:
     def run(self):
         """Launch the batch on torch's current stream; returns (out, out64)."""
         call("tq_batch_run", self.dt, ptr(self.desc), self.count, ptr(self.tiles), self.n_tiles,
-             ptr(self.tile_begin), ptr(self.tile_count), ptr(self.gl_x), ptr(self.gl_w), ptr(self.tile_part),
+             ptr(self.tile_begin), ptr(self.tile_count), ptr(self.gl_x), ptr(self.gl_w), ptr(self.gl_p32),
+             ptr(self.tile_part),
              ptr(self.counter), self.counter.numel(), ptr(self.out), ptr(self.out64), self.blocks,
              self.class_offsets.ctypes.data_as(ctypes.c_void_p), stream_ptr(self.device))
         return self.out, self.out64

@@ -58,8 +58,8 @@ __global__ void k_batch_reset(unsigned int* counter) {
 // General kernel: loop-axis table when it fits, row axes recomputed per row.
 template <typename T, int FAM>
 __device__ __forceinline__ double batch_tile_general(const BatchDev<T>& D, const Axis* ax, T* tab, bool table,
-                                            const double* glx, const double* glw, int lw0, int lw1, long long t0,
-                                            long long t1) {
+                                            const double* glx, const double* glw, const float* gp32, int lw0,
+                                            int lw1, long long t0, long long t1) {
     if (table) {
         const int lane = threadIdx.x & (kWave - 1);
         for (int j = lane; j < D.nl; j += kWave)
@@ -72,9 +72,9 @@ __device__ __forceinline__ double batch_tile_general(const BatchDev<T>& D, const
     double acc = 0.0;
     for (int lw = lw0; lw < lw1; ++lw)
         acc += table ? grid_tile<T, FAM, true>(D.d, D.n, D.nl, D.m, D.L, D.rule, D.lane_count, D.p, ax, tab, glx,
-                                               glw, lw, t0, t1)
+                                               glw, lw, t0, t1, nullptr, nullptr, gp32)
                      : grid_tile<T, FAM, false>(D.d, D.n, D.nl, D.m, D.L, D.rule, D.lane_count, D.p, ax, tab, glx,
-                                                glw, lw, t0, t1);
+                                                glw, lw, t0, t1, nullptr, nullptr, gp32);
     return acc;
 }
 
@@ -82,8 +82,8 @@ __device__ __forceinline__ double batch_tile_general(const BatchDev<T>& D, const
 // every integral they see fit the region (or that it has no loop axis).
 template <typename T, int FAM>
 __device__ __forceinline__ double batch_tile_rowtab(const BatchDev<T>& D, const Axis* ax, unsigned char* region,
-                                           const double* glx, const double* glw, int lw0, int lw1, long long t0,
-                                           long long t1) {
+                                           const double* glx, const double* glw, const float* gp32, int lw0,
+                                           int lw1, long long t0, long long t1) {
     const int n = D.n, L = D.L;
     double* rw = reinterpret_cast<double*>(region);
     T* tab = reinterpret_cast<T*>(region + (L > 0 ? (size_t)L * n * sizeof(double) : 0));
@@ -103,20 +103,20 @@ __device__ __forceinline__ double batch_tile_rowtab(const BatchDev<T>& D, const 
     double acc = 0.0;
     for (int lw = lw0; lw < lw1; ++lw)
         acc += grid_tile<T, FAM, true, true>(D.d, n, D.nl, D.m, L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw,
-                                             t0, t1, rt, rw);
+                                             t0, t1, rt, rw, gp32);
     return acc;
 }
 
 template <typename T, int REGION>
 __device__ __forceinline__ double batch_tile(const BatchDev<T>& D, const Axis* ax, unsigned char* region, bool table,
-                                    const double* glx, const double* glw, int lw0, int lw1, long long t0,
-                                    long long t1) {
+                                    const double* glx, const double* glw, const float* gp32, int lw0, int lw1,
+                                    long long t0, long long t1) {
     switch (D.fam) {
 #define TQ_BT(F)                                                                                                    \
     case F:                                                                                                         \
-        return REGION > 0 ? batch_tile_rowtab<T, F>(D, ax, region, glx, glw, lw0, lw1, t0, t1)                      \
-                          : batch_tile_general<T, F>(D, ax, reinterpret_cast<T*>(region), table, glx, glw, lw0, lw1, \
-                                                     t0, t1);
+        return REGION > 0 ? batch_tile_rowtab<T, F>(D, ax, region, glx, glw, gp32, lw0, lw1, t0, t1)                \
+                          : batch_tile_general<T, F>(D, ax, reinterpret_cast<T*>(region), table, glx, glw, gp32, lw0, \
+                                                     lw1, t0, t1);
         TQ_BT(TQ_FAM_GAUSSIAN_PEAK)
         TQ_BT(TQ_FAM_OSCILLATORY)
         TQ_BT(TQ_FAM_PRODUCT_PEAK)
@@ -135,7 +135,7 @@ __device__ __forceinline__ double batch_tile(const BatchDev<T>& D, const Axis* a
 template <typename T, int REGION>
 __global__ __launch_bounds__(kBlock) void k_batch(const BatchDev<T>* descs, const tq_batch_tile* tiles,
                                                   long long n_tiles, const double* glx, const double* glw,
-                                                  double* tile_part, unsigned int* counter) {
+                                                  const float* glp32, double* tile_part, unsigned int* counter) {
     constexpr int NT = BatchTab<T>::N;
     constexpr int RB = REGION > 0 ? REGION : (int)(2 * NT * sizeof(T));
     __shared__ Axis s_ax[kWavesPerBlock][TQ_MAX_DIM];
@@ -164,13 +164,15 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchDev<T>* descs, cons
         const BatchDev<T>& D = descs[tile.item];
         const double* gx = glx ? glx + D.gl_offset : nullptr;
         const double* gw = glw ? glw + D.gl_offset : nullptr;
+        // FP32 (node, weight) pairs for one-axis Gauss-Legendre rows of FP32 batches
+        const float* gp = (glp32 && sizeof(T) == 4 && D.m == 0) ? glp32 + 2 * D.gl_offset : nullptr;
         for (int k = lane; k < D.d; k += kWave) s_ax[wave][k] = D.ax[k];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         const bool table = REGION == 0 && D.m > 0 && D.L >= 0 && D.nl * tab_stride(D.fam) <= 2 * NT;
-        double acc = batch_tile<T, REGION>(D, s_ax[wave], region, table, gx, gw, tile.lane_wave, tile.lane_wave_end,
-                                           tile.t0, tile.t1);
+        double acc = batch_tile<T, REGION>(D, s_ax[wave], region, table, gx, gw, gp, tile.lane_wave,
+                                           tile.lane_wave_end, tile.t0, tile.t1);
         acc = wave_sum(acc);
         if (lane == 0) tile_part[t] = acc;
         // all lanes are done with this tile's LDS before the next one writes it
@@ -368,19 +370,20 @@ static int plan_batch(const tq_batch_item* items, long long count, long long til
 
 template <typename T, int REGION>
 static int launch_class(const void* desc, const tq_batch_tile* tiles, long long n, const double* glx,
-                        const double* glw, double* tile_part, unsigned int* counter, int blocks, hipStream_t s) {
+                        const double* glw, const float* glp32, double* tile_part, unsigned int* counter, int blocks,
+                        hipStream_t s) {
     if (n <= 0) return TQ_OK;
     long long need = (n + kWavesPerBlock - 1) / kWavesPerBlock;
     if (blocks > need) blocks = (int)need;
     hipLaunchKernelGGL((k_batch<T, REGION>), dim3(blocks), dim3(kBlock), 0, s, (const BatchDev<T>*)desc, tiles, n,
-                       glx, glw, tile_part, counter);
+                       glx, glw, glp32, tile_part, counter);
     return check_launch("k_batch");
 }
 
 template <typename T>
 static int run_batch(const void* desc, long long count, const tq_batch_tile* tiles, long long n_tiles,
                      const int64_t* tile_begin, const int64_t* tile_count, const double* glx, const double* glw,
-                     double* tile_part, unsigned int* counter, void* out, double* out64, int blocks,
+                     const float* glp32, double* tile_part, unsigned int* counter, void* out, double* out64, int blocks,
                      const int64_t* class_offsets, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(kBatchClasses * kQueues), 0, s, counter);
     int rc = check_launch("k_batch_reset");
@@ -393,13 +396,13 @@ static int run_batch(const void* desc, long long count, const tq_batch_tile* til
             if (off[c] < 0 || off[c] > off[c + 1] || off[kBatchClasses] != n_tiles)
                 return fail(TQ_E_ARG, "class_offsets do not partition the tiles");
     }
-    rc = launch_class<T, kRegionSmall>(desc, tiles + off[0], off[1] - off[0], glx, glw, tile_part + off[0],
+    rc = launch_class<T, kRegionSmall>(desc, tiles + off[0], off[1] - off[0], glx, glw, glp32, tile_part + off[0],
                                        counter, blocks, s);
     if (!rc)
-        rc = launch_class<T, kRegionBig>(desc, tiles + off[1], off[2] - off[1], glx, glw, tile_part + off[1],
+        rc = launch_class<T, kRegionBig>(desc, tiles + off[1], off[2] - off[1], glx, glw, glp32, tile_part + off[1],
                                          counter + kQueues * kQueueStride, blocks, s);
     if (!rc)
-        rc = launch_class<T, 0>(desc, tiles + off[2], off[3] - off[2], glx, glw, tile_part + off[2],
+        rc = launch_class<T, 0>(desc, tiles + off[2], off[3] - off[2], glx, glw, glp32, tile_part + off[2],
                                 counter + 2 * kQueues * kQueueStride, blocks, s);
     if (rc) return rc;
     const long long fb = (count + kWavesPerBlock - 1) / kWavesPerBlock;
@@ -444,8 +447,9 @@ int tq_batch_plan(const tq_batch_item* items, int64_t count, int32_t dtype, int6
 
 int tq_batch_run(int32_t dtype, const void* desc, int64_t count, const tq_batch_tile* tiles, int64_t n_tiles,
                  const int64_t* tile_begin, const int64_t* tile_count, const double* gl_nodes,
-                 const double* gl_weights, double* tile_partials, unsigned int* counter, int64_t counter_words,
-                 void* out, double* out64, int32_t blocks, const int64_t* class_offsets, void* stream) {
+                 const double* gl_weights, const float* gl_pairs32, double* tile_partials, unsigned int* counter,
+                 int64_t counter_words, void* out, double* out64, int32_t blocks, const int64_t* class_offsets,
+                 void* stream) {
     if (!desc || !tiles || !tile_begin || !tile_count || !tile_partials || !counter)
         return fail(TQ_E_NULL, "NULL argument");
     if (counter_words < TQ_BATCH_COUNTER_WORDS) return fail(TQ_E_WORKSPACE, "counter needs TQ_BATCH_COUNTER_WORDS words");
@@ -453,10 +457,10 @@ int tq_batch_run(int32_t dtype, const void* desc, int64_t count, const tq_batch_
     if (!out && !out64) return fail(TQ_E_NULL, "no output pointer");
     if (dtype == TQ_F32)
         return run_batch<float>(desc, count, tiles, n_tiles, tile_begin, tile_count, gl_nodes, gl_weights,
-                                tile_partials, counter, out, out64, blocks, class_offsets, as_stream(stream));
+                                gl_pairs32, tile_partials, counter, out, out64, blocks, class_offsets, as_stream(stream));
     if (dtype == TQ_F64)
         return run_batch<double>(desc, count, tiles, n_tiles, tile_begin, tile_count, gl_nodes, gl_weights,
-                                 tile_partials, counter, out, out64, blocks, class_offsets, as_stream(stream));
+                                 nullptr, tile_partials, counter, out, out64, blocks, class_offsets, as_stream(stream));
     return fail(TQ_E_ARG, "dtype must be TQ_F32 or TQ_F64");
 }
 

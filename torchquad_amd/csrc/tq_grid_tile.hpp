@@ -133,7 +133,7 @@ template <typename T, int FAM, bool TABLE, bool ROWTAB = false>
 __device__ inline double grid_tile(int d, int n, int nl, int m, int L, int rule, int lane_count, const Params<T>& p,
                                    const Axis* ax, const T* tab, const double* glx, const double* glw, int lw,
                                    long long t_begin, long long t_end, const T* rt = nullptr,
-                                   const double* rw = nullptr) {
+                                   const double* rw = nullptr, const float* gl_pairs32 = nullptr) {
     using F = Family<FAM, T>;
     const int lane = threadIdx.x & (kWave - 1);
     double acc = 0.0;
@@ -162,7 +162,24 @@ __device__ inline double grid_tile(int d, int n, int nl, int m, int L, int rule,
             acc += edge_item(t_end_item);
             ie = t_end_item;
         }
-        if (rule == TQ_RULE_GAUSS_LEGENDRE) {
+        if (rule == TQ_RULE_GAUSS_LEGENDRE && gl_pairs32) {
+            // FP32 (node, weight) pairs: one 8-byte load per point instead of
+            // two 8-byte loads (FP32 results only; the node and weight carry
+            // 2^-24 relative rounding, far inside the FP32 tolerance)
+            const float2* pairs = reinterpret_cast<const float2*>(gl_pairs32);
+            const float sa = (float)a0.a, sh = (float)a0.h;
+            for (long long tb = ib; tb < ie; tb += kFlush) {
+                const long long te = (tb + kFlush < ie) ? (tb + kFlush) : ie;
+                T fa = T(0);
+#pragma unroll 4
+                for (long long t = tb; t < te; ++t) {
+                    const float2 e = pairs[(int)(t * kWave) + lane];
+                    const T x = (T)fmaf(sa, e.x, sh);
+                    fa = fma((T)e.y, F::finish(F::fold_c(s0, ac0, x, p), p), fa);
+                }
+                acc += (double)fa;
+            }
+        } else if (rule == TQ_RULE_GAUSS_LEGENDRE) {
             for (long long tb = ib; tb < ie; tb += kFlush) {
                 const long long te = (tb + kFlush < ie) ? (tb + kFlush) : ie;
                 T fa = T(0);

@@ -49,7 +49,7 @@ import torch
 from . import _lib
 from ._lib import NativeUnavailableError, call, lib, ptr
 from .batch import RULE_NAMES, n_per_axis
-from .gl_nodes import gauss_legendre
+from .gl_nodes import fp32_pairs, gauss_legendre
 from .integrands import BuiltinIntegrand
 
 ITEM_BYTES = ctypes.sizeof(_lib.BatchItem)
@@ -117,7 +117,7 @@ class _GridRequest:
         self.ticket = ticket
         self.item = item
         self.gl_n = gl_n
-        self.gl_entry = None  # (nodes, weights, event) once its table is started
+        self.gl_entry = None  # (nodes, weights, event, FP32 pairs) once its table is started
 
 
 class _VegasRequest:
@@ -269,7 +269,8 @@ class IntegrationServer:
         call("tq_batch_plan_sizes", ctypes.byref(item), 1, self.dt, self.tile_points, ctypes.byref(desc_bytes),
              ctypes.byref(n_tiles))
         gl_n = int(n) if code == _lib.RULE_GAUSS_LEGENDRE else 0
-        nbytes = (desc_bytes.value + n_tiles.value * (TILE_BYTES + 8) + 16 + 16 + ITEM_BYTES + 16 * gl_n)
+        gl_bytes = (24 if self.dt == _lib.TQ_F32 else 16) * gl_n  # FP64 table (+ FP32 pairs) in the batch
+        nbytes = (desc_bytes.value + n_tiles.value * (TILE_BYTES + 8) + 16 + 16 + ITEM_BYTES + gl_bytes)
         t = Ticket("grid", nbytes, int(n) ** dim)
         t.rule = code
         if arrival is not None:
@@ -381,7 +382,8 @@ class IntegrationServer:
                     self._vegas.append(r)
 
     def _gl_table(self, n):
-        """The n-point Gauss-Legendre table entry (nodes, weights, ready event).
+        """The n-point Gauss-Legendre table entry (nodes, weights, ready event,
+        FP32 (node, weight) pairs or None for a float64 server).
 
         A new n starts its table on a side stream, so a request whose table
         is still being generated waits for a later batch instead of delaying
@@ -400,14 +402,16 @@ class IntegrationServer:
         self._gl_next += 1
         with torch.cuda.stream(stream):
             x, w = gauss_legendre(n, device=self.device)
+            p32 = fp32_pairs(x, w) if self.dt == _lib.TQ_F32 else None
             ev = torch.cuda.Event()
             ev.record(stream)
-        e = (x, w, ev)
+        e = (x, w, ev, p32)
+        per = 24 if p32 is not None else 16
         self._gl_cache[n] = e
-        self._gl_cache_bytes += 16 * n
+        self._gl_cache_bytes += per * n
         while self._gl_cache_bytes > self.gl_cache_limit and len(self._gl_cache) > 1:
             old_n, _ = self._gl_cache.popitem(last=False)
-            self._gl_cache_bytes -= 16 * old_n
+            self._gl_cache_bytes -= per * old_n
         return e
 
     def _launch_grid(self):
@@ -438,7 +442,7 @@ class IntegrationServer:
             return
         count = len(reqs)
         # Gauss-Legendre tables of the batch (ready), concatenated on the slot's stream
-        gl_x = gl_w = None
+        gl_x = gl_w = gl_p = None
         ns = sorted({r.gl_n for r in reqs if r.gl_n})
         offsets, tabs, off = {}, [], 0
         for n in ns:
@@ -447,12 +451,16 @@ class IntegrationServer:
             off += n
         with torch.cuda.stream(slot.stream):
             if ns:
-                for x, w, ev in tabs:
+                for x, w, ev, p32 in tabs:
                     slot.stream.wait_event(ev)
                     x.record_stream(slot.stream)
                     w.record_stream(slot.stream)
+                    if p32 is not None:
+                        p32.record_stream(slot.stream)
                 gl_x = torch.cat([t[0] for t in tabs])
                 gl_w = torch.cat([t[1] for t in tabs])
+                if self.dt == _lib.TQ_F32:
+                    gl_p = torch.cat([t[3] for t in tabs])
                 for r in reqs:
                     if r.gl_n:
                         r.item.gl_offset = offsets[r.gl_n]
@@ -477,14 +485,14 @@ class IntegrationServer:
             cd[: count * 8].copy_(ch[: count * 8], non_blocking=True)
             slot.ensure_outputs(count, nt)
             call("tq_batch_run", self.dt, ptr(dd), count, ptr(td), int(n_tiles.value), ptr(bd), ptr(cd), ptr(gl_x),
-                 ptr(gl_w), ptr(slot.part), ptr(slot.counter), slot.counter.numel(), ptr(slot.out), ptr(slot.out64), 0,
+                 ptr(gl_w), ptr(gl_p), ptr(slot.part), ptr(slot.counter), slot.counter.numel(), ptr(slot.out), ptr(slot.out64), 0,
                  slot.class_offsets.ctypes.data_as(ctypes.c_void_p), slot.stream.cuda_stream)
             slot.host_out[:count].copy_(slot.out64[:count], non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(slot.stream)
         slot.event = ev
         slot.requests = reqs
-        slot.gl = (gl_x, gl_w)
+        slot.gl = (gl_x, gl_w, gl_p)
         self.batches_launched += 1
 
     def _advance_vegas(self):
