@@ -345,16 +345,18 @@ SPLIT_N = {
     "trapezoid": (128, 129, 191, 192, 193, 255, 256, 257, 1000, 4097),
     "simpson": (129, 191, 193, 255, 257, 1001, 4097),
     "boole": (129, 193, 257, 1001, 4097),
-    "gauss_legendre": (128, 129, 192, 257, 1000),
+    "gauss_legendre": (128, 129, 192, 255, 257, 1000, 4097),
 }
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("rule", list(RULES))
 def test_split_mode_sizes_vs_oracle(dtype, rule, dev):
-    """One axis of n >= 128 nodes is walked in rows of 64 nodes: interior rows in a tight loop with
-    one weight per lane, the rows holding the end nodes (and nodes past n) in the general body.
-    Sizes around the row boundaries, every family, against the FP64 brute force."""
+    """One axis of n >= 128 nodes is walked in rows of 64 nodes: Newton-Cotes interior rows in a tight
+    loop with one weight per lane, the rows holding the end nodes (and nodes past n) in the general
+    body; Gauss-Legendre rows over the lower half of the table, each node with its mirror (odd n:
+    the middle node once).  Sizes around the row boundaries, every family, against the FP64 brute
+    force."""
     rng = np.random.default_rng(17)
     for n in SPLIT_N[rule]:
         for fname in FAMS:

@@ -224,10 +224,11 @@ static int check_item(const tq_batch_item& it) {
     return TQ_OK;
 }
 
-// Split-mode integrals (one axis, m == 0) walk one 64-node row per item.
-// Gauss-Legendre rows wait on global loads of the node tables, so shorter
-// tiles spread them over more waves; Newton-Cotes rows compute their nodes
-// and take longer tiles, which spread the per-tile cost over more points.
+// Split-mode integrals (one axis, m == 0) walk one 64-node row per item
+// (Gauss-Legendre: 64 nodes and their 64 mirrors).  Gauss-Legendre rows
+// wait on global loads of the node tables, so shorter tiles spread them
+// over more waves; Newton-Cotes rows compute their nodes and take longer
+// tiles, which spread the per-tile cost over more points.
 constexpr long long kSplitMaxItemsGL = 128;
 constexpr long long kSplitMaxItemsNC = 512;
 

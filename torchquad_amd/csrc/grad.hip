@@ -37,8 +37,9 @@ constexpr long long kSweepMergeCap = 1024;
 
 static int sweep_geom(const tq_grid_desc* gd, long long B, SweepGeom& S, long long merge_cap = 0) {
     const long long per_integral = B > 0 ? (kSweepWaves + B - 1) / B : kSweepWaves;
+    // the backward's one-axis rows walk every node, so the forward does too
     int rc = plan_geom(gd->dim, gd->n, gd->rule, gd->lane_axes, gd->items_per_wave, per_integral, S.g, merge_cap,
-                       false);
+                       false, false);
     if (rc) return rc;
     S.blocks_x = (S.g.num_waves + kWavesPerBlock - 1) / kWavesPerBlock;
     return TQ_OK;
@@ -114,8 +115,9 @@ __global__ __launch_bounds__(BW * kWave) void k_sweep_fwd(SweepGeom S, const T* 
         const long long chunk = wave / g.lane_waves;
         const long long t0 = chunk * g.ipw;
         const long long t1 = (t0 + g.ipw < g.items) ? (t0 + g.ipw) : g.items;
-        acc = grid_tile<T, TQ_FAM_GAUSSIAN_PEAK, TABLE, ROWTAB>(g.d, g.n, g.nl, g.m, g.L, g.rule, g.lane_count, s_p,
-                                                                s_axis, s_tab, glx, glw, lw, t0, t1, s_rt, s_rw);
+        acc = grid_tile<T, TQ_FAM_GAUSSIAN_PEAK, TABLE, ROWTAB, false>(g.d, g.n, g.nl, g.m, g.L, g.rule,
+                                                                       g.lane_count, s_p, s_axis, s_tab, glx, glw,
+                                                                       lw, t0, t1, s_rt, s_rw);
     }
     const double bs = block_sum(acc, s_red);
     if (threadIdx.x == 0) partials[b * gridDim.x + blockIdx.x] = bs;

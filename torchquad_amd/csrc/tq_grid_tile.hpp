@@ -123,13 +123,17 @@ __device__ inline void row_table_entry(T* rt, double* rw, int k, int j, const Ax
 }
 
 // One tile of the tensor-product index space.  m == 0 (one axis): node
-// 64 t + lane for loop items t in [t_begin, t_end).  Otherwise the lanes of the calling wave
+// 64 t + lane for loop items t in [t_begin, t_end); with GL_FOLD, a
+// Gauss-Legendre axis instead walks the lower half of its symmetric table,
+// item t, lane l evaluating node k = 64 t + l < ceil(n/2) and its mirror
+// n - 1 - k (x_{n-1-k} = -x_k, w_{n-1-k} = w_k), so each table load serves
+// two points.  Otherwise the lanes of the calling wave
 // take lane combinations lw*64 .. lw*64+63 of the last m axes, the wave walks
 // loop items [t_begin, t_end) (row axes [0, L) x a loop of nl points over
 // the axes between the row and lane axes: one axis, nl = n, or two, nl = n^2,
 // which needs the table).  Returns the lane's weighted sum.  `ax` holds the
 // per-axis maps, `tab` the loop table (loop_table_entry) when TABLE.
-template <typename T, int FAM, bool TABLE, bool ROWTAB = false>
+template <typename T, int FAM, bool TABLE, bool ROWTAB = false, bool GL_FOLD = true>
 __device__ inline double grid_tile(int d, int n, int nl, int m, int L, int rule, int lane_count, const Params<T>& p,
                                    const Axis* ax, const T* tab, const double* glx, const double* glw, int lw,
                                    long long t_begin, long long t_end, const T* rt = nullptr,
@@ -137,6 +141,49 @@ __device__ inline double grid_tile(int d, int n, int nl, int m, int L, int rule,
     using F = Family<FAM, T>;
     const int lane = threadIdx.x & (kWave - 1);
     double acc = 0.0;
+    if (m == 0 && GL_FOLD && rule == TQ_RULE_GAUSS_LEGENDRE) {
+        const State<T> s0 = F::start(p);
+        const AxisC<T> ac0 = axis_c(p, 0);
+        const Axis a0 = ax[0];
+        const int h = (n + 1) >> 1;  // nodes 0 .. h-1; for odd n node h-1 is the middle one, counted once
+        for (long long tb = t_begin; tb < t_end; tb += kFlush) {
+            const long long te = (tb + kFlush < t_end) ? (tb + kFlush) : t_end;
+            T fa = T(0);
+            if (gl_pairs32) {
+                // FP32 (node, weight) pairs: one 8-byte load per two points
+                // (FP32 results only; the node and weight carry 2^-24
+                // relative rounding, far inside the FP32 tolerance)
+                const float2* pairs = reinterpret_cast<const float2*>(gl_pairs32);
+                const float sa = (float)a0.a, sh = (float)a0.h;
+#pragma unroll 2
+                for (long long t = tb; t < te; ++t) {
+                    const int k = (int)(t * kWave) + lane;
+                    const float2 e = pairs[k < h ? k : h - 1];
+                    const T w1 = (k < h) ? (T)e.y : T(0);
+                    const T w2 = (k < h && 2 * k + 1 != n) ? (T)e.y : T(0);
+                    const T x1 = (T)fmaf(sa, e.x, sh);
+                    const T x2 = (T)fmaf(-sa, e.x, sh);
+                    fa = fma(w1, F::finish(F::fold_c(s0, ac0, x1, p), p), fa);
+                    fa = fma(w2, F::finish(F::fold_c(s0, ac0, x2, p), p), fa);
+                }
+            } else {
+#pragma unroll 2
+                for (long long t = tb; t < te; ++t) {
+                    const int k = (int)(t * kWave) + lane;
+                    const int kk = k < h ? k : h - 1;
+                    const double xn = glx[kk], wk = glw[kk];
+                    const T w1 = (k < h) ? (T)wk : T(0);
+                    const T w2 = (k < h && 2 * k + 1 != n) ? (T)wk : T(0);
+                    const T x1 = (T)fma(a0.a, xn, a0.h);
+                    const T x2 = (T)fma(-a0.a, xn, a0.h);
+                    fa = fma(w1, F::finish(F::fold_c(s0, ac0, x1, p), p), fa);
+                    fa = fma(w2, F::finish(F::fold_c(s0, ac0, x2, p), p), fa);
+                }
+            }
+            acc += (double)fa;
+        }
+        return acc;
+    }
     if (m == 0) {
         // one axis split into rows of 64 consecutive nodes: node k = 64 t + lane
         const State<T> s0 = F::start(p);
@@ -162,24 +209,7 @@ __device__ inline double grid_tile(int d, int n, int nl, int m, int L, int rule,
             acc += edge_item(t_end_item);
             ie = t_end_item;
         }
-        if (rule == TQ_RULE_GAUSS_LEGENDRE && gl_pairs32) {
-            // FP32 (node, weight) pairs: one 8-byte load per point instead of
-            // two 8-byte loads (FP32 results only; the node and weight carry
-            // 2^-24 relative rounding, far inside the FP32 tolerance)
-            const float2* pairs = reinterpret_cast<const float2*>(gl_pairs32);
-            const float sa = (float)a0.a, sh = (float)a0.h;
-            for (long long tb = ib; tb < ie; tb += kFlush) {
-                const long long te = (tb + kFlush < ie) ? (tb + kFlush) : ie;
-                T fa = T(0);
-#pragma unroll 4
-                for (long long t = tb; t < te; ++t) {
-                    const float2 e = pairs[(int)(t * kWave) + lane];
-                    const T x = (T)fmaf(sa, e.x, sh);
-                    fa = fma((T)e.y, F::finish(F::fold_c(s0, ac0, x, p), p), fa);
-                }
-                acc += (double)fa;
-            }
-        } else if (rule == TQ_RULE_GAUSS_LEGENDRE) {
+        if (rule == TQ_RULE_GAUSS_LEGENDRE) {
             for (long long tb = ib; tb < ie; tb += kFlush) {
                 const long long te = (tb + kFlush < ie) ? (tb + kFlush) : ie;
                 T fa = T(0);
@@ -344,6 +374,7 @@ struct GridGeom {
     int lane_count;       // n^m
     int lane_waves;       // ceil(lane_count / 64)
     int use_table;        // loop axis nodes/weights in LDS
+    int gl_fold;          // m == 0 Gauss-Legendre: items cover nodes [0, ceil(n/2)) and their mirrors
     long long items;      // n^(d-m) (1 when there is no loop axis)
     long long ipw;        // items per wave
     long long chunks;     // ceil(items / ipw)
@@ -373,14 +404,16 @@ constexpr int kMergeMaxN = 32;
 // loop runs over the last two non-lane axes (n^2 points per row, one table
 // entry per point pair), so a short axis does not pay the per-row cost
 // every n points.  trade_lane_axis = false keeps the lane axes (and so the
-// launch size) of the unmerged plan.
+// launch size) of the unmerged plan.  gl_fold: a one-axis (m == 0)
+// Gauss-Legendre plan covers half the table (grid_tile's GL_FOLD).
 inline int plan_geom(int d, int n, int rule, int lane_axes, long long items_per_wave, long long target_waves,
-                     GridGeom& G, long long merge_cap = 0, bool trade_lane_axis = true) {
+                     GridGeom& G, long long merge_cap = 0, bool trade_lane_axis = true, bool gl_fold = true) {
     G.d = d;
     G.n = n;
     G.rule = rule;
     G.q = 1;
     G.nl = n;
+    G.gl_fold = 0;
     int m = 0;
     if (lane_axes > 0) {
         m = lane_axes < d ? lane_axes : d;
@@ -425,7 +458,9 @@ inline int plan_geom(int d, int n, int rule, int lane_axes, long long items_per_
         G.lane_count = kWave;
         G.lane_waves = 1;
         G.L = 0;
-        items = ((long long)n + kWave - 1) / kWave;
+        G.gl_fold = (gl_fold && rule == TQ_RULE_GAUSS_LEGENDRE) ? 1 : 0;
+        const long long nodes = G.gl_fold ? ((long long)n + 1) / 2 : (long long)n;
+        items = (nodes + kWave - 1) / kWave;
     } else {
         long long lc;
         if (ipow_checked(n, m, (1LL << 31) - 64, &lc) != 0) return fail(TQ_E_ARG, "lane combinations exceed 2^31");
