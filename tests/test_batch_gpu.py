@@ -77,6 +77,30 @@ def test_batch_matches_plain_path_and_is_deterministic(dev):
     np.testing.assert_allclose(b.out64.cpu().numpy(), first.cpu().numpy(), rtol=2e-7, atol=1e-9)
 
 
+def test_fp32_gl_pairs_vs_fp64_tables(dev):
+    """FP32 batches read one-axis Gauss-Legendre rows from FP32 (node, weight) pairs; without the
+    pairs (NULL) the same kernel reads the FP64 tables.  Both against the oracle at the FP32 bar,
+    odd and even n (the middle node of odd n is counted once)."""
+    rng = np.random.default_rng(21)
+    items = []
+    for n in (128, 129, 255, 256, 1001, 4096, 65537, 200_001):
+        a, u = rng.uniform(1.0, 8.0, 1), rng.uniform(0.2, 0.8, 1)
+        items.append({"rule": "gauss_legendre", "fn": tq.GaussianPeak(a, u), "dim": 1, "n": n,
+                      "domain": [[0.0, 1.0]], "family": "gaussian_peak", "a": a, "u": u, "w": None})
+    b = Batch(items, dtype=torch.float32)
+    assert b.gl_p32 is not None
+    with_pairs = b.run()[1].clone()
+    b.gl_p32 = None
+    fp64_tabs = b.run()[1].clone()
+    for i, it in enumerate(items):
+        if it["n"] <= 4096:
+            ref = og.separable_gaussian_peak("gauss_legendre", it["a"], it["u"], it["domain"], it["n"])
+        else:  # the rule is exact to FP64 rounding at these n: the analytic integral
+            ref = fam.gaussian_peak_integral(it["a"], it["u"], [0.0], [1.0])
+        for v in (float(with_pairs[i]), float(fp64_tabs[i])):
+            assert abs(v - ref) <= 1e-6 * max(abs(ref), 1.0), (it["n"], v, ref)
+
+
 def test_batch_graph_replay(dev):
     items = genz_batch(128, seed=4)
     b = Batch(items, dtype=torch.float32)

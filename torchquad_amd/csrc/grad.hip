@@ -130,10 +130,11 @@ __device__ inline double node_r(int j, int n, int rule, const double* glx) {
 }
 
 // One loop-axis table entry of the backward kernel: exponent term
-// -(a x + b)^2 in log2 units, weight, t = x - u, normalised node r.
+// -(a x + b)^2 + log2(w) in log2 units (the node weight folded into the
+// exponent, so w f is one exp2), t = x - u, t^2 and t r (r: normalised node).
 template <typename T>
 struct alignas(4 * sizeof(T) < 16 ? 4 * sizeof(T) : 16) BwdEntry {
-    T term, w, t, r;
+    T term, t, t2, tr;
 };
 
 // Backward tile for D axes: flat walk of the tile's points (lane axes per
@@ -260,27 +261,27 @@ __device__ inline void sweep_bwd_tile(const GridGeom& g, const Params<T>& p, con
             T fF = T(0), fE = T(0), fQ = T(0), fR = T(0);
 #pragma unroll 4
             for (int jj = jb; jj < je; ++jj) {
-                T term, wL, tL, rL;
                 if (TABLE) {
+                    // one add, one exp2 and four accumulates per point
                     const BwdEntry<T> en = tab[jj];
-                    term = en.term;
-                    wL = en.w;
-                    tL = en.t;
-                    rL = en.r;
+                    const T wf = tq_exp2(s_fix + en.term);
+                    fF += wf;
+                    fE = fma(wf, en.t, fE);
+                    fQ = fma(wf, en.t2, fQ);
+                    fR = fma(wf, en.tr, fR);
                 } else {
                     const T xL = (T)node_of(ax[L], jj, n, rule, glx);
                     const T e = fma(xL, aL, bL);
-                    term = -e * e;
-                    wL = (T)weight_of(rule, jj, n, glw);
-                    tL = xL - uL;
-                    rL = (T)node_r(jj, n, rule, glx);
+                    const T wL = (T)weight_of(rule, jj, n, glw);
+                    const T tL = xL - uL;
+                    const T rL = (T)node_r(jj, n, rule, glx);
+                    const T wf = wL * tq_exp2(s_fix - e * e);
+                    fF += wf;
+                    const T wt = wf * tL;
+                    fE += wt;
+                    fQ = fma(wt, tL, fQ);
+                    fR = fma(wt, rL, fR);
                 }
-                const T wf = wL * tq_exp2(s_fix + term);
-                fF += wf;
-                const T wt = wf * tL;
-                fE += wt;
-                fQ = fma(wt, tL, fQ);
-                fR = fma(wt, rL, fR);
             }
             Frow += w_row * (double)fF;
             EL += w_row * (double)fE;
@@ -348,11 +349,14 @@ __global__ __launch_bounds__(kBlock) void k_sweep_bwd(SweepGeom S, const T* a, c
         for (int j = threadIdx.x; j < g.n; j += kBlock) {
             const T x = (T)node_of(s_axis[L], j, g.n, g.rule, glx);
             const T e = fma(x, s_p.a[L], s_p.b[L]);
+            // composite-rule and Gauss-Legendre weights are positive
+            const double w = weight_of(g.rule, j, g.n, glw);
+            const T t = x - s_p.u[L];
             BwdEntry<T> en;
-            en.term = -e * e;
-            en.w = (T)weight_of(g.rule, j, g.n, glw);
-            en.t = x - s_p.u[L];
-            en.r = (T)node_r(j, g.n, g.rule, glx);
+            en.term = (T)((double)(-e * e) + log2(w));
+            en.t = t;
+            en.t2 = t * t;
+            en.tr = t * (T)node_r(j, g.n, g.rule, glx);
             s_tab[j] = en;
         }
         __syncthreads();
