@@ -208,7 +208,10 @@ __device__ inline unsigned wave_xor(unsigned x) {
     } else if constexpr (S == 8) {
         return __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);
     } else {
-        return (unsigned)__shfl_xor((int)x, S, kWave);
+        // lane ^ S stays inside the wave: ds_bpermute with the byte address
+        // of the partner lane (no width clamp needed)
+        const int lane = threadIdx.x & (kWave - 1);
+        return (unsigned)__builtin_amdgcn_ds_bpermute((lane ^ S) << 2, (int)x);
     }
 }
 
@@ -219,7 +222,11 @@ __device__ inline unsigned bitonic_stage(unsigned key, int lane) {
     const unsigned other = wave_xor<STRIDE>(key);
     const bool up = (lane & SIZE) == 0;
     const bool lower = (lane & STRIDE) == 0;
-    return ((lower == up) ? (other < key) : (other > key)) ? other : key;
+    // branch-free: min and max of the pair, then one select on the lane's
+    // (compile-time per stage) role -- the DPP move, v_min, v_max, v_cndmask
+    const unsigned mn = __builtin_elementwise_min(key, other);
+    const unsigned mx = __builtin_elementwise_max(key, other);
+    return (lower == up) ? mn : mx;
 }
 
 template <int SIZE, int STRIDE>
@@ -346,7 +353,7 @@ __device__ inline double uniform_coord<double>(uint64_t seed, uint64_t s, int k,
 template <typename T, int FAM, int MODE, bool LDS_HIST>
 __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
     using F = Family<FAM, T>;
-    __shared__ long long s_off[kOffLds];
+    __shared__ int s_off[kOffLds];  // cube offsets relative to the block's first sample, clamped
     __shared__ double s_map[kMapLds];
     // wave-private histograms in dynamic LDS sized to the map (hist_lds_bytes)
     extern __shared__ __align__(16) unsigned char s_hist_raw[];
@@ -398,8 +405,14 @@ __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
         const long long h_hi = s_hrange[1];
         n_off = h_hi - h_lo + 2;  // offsets h_lo .. h_hi+1
         off_lds = (n_off <= kOffLds);
+        // relative offsets clamped to [-1, kSB + 1]: the search compares them
+        // with s - s0 in [0, kSB), so the clamp changes no comparison
         if (off_lds)
-            for (long long i = tid; i < n_off; i += kBlock) s_off[i] = st.offsets[h_lo + i];
+            for (long long i = tid; i < n_off; i += kBlock) {
+                long long r = st.offsets[h_lo + i] - s0;
+                r = r < -1 ? -1 : (r > kSB + 1 ? kSB + 1 : r);
+                s_off[i] = (int)r;
+            }
     }
     __syncthreads();
 
@@ -420,6 +433,21 @@ __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
     const T tns = (T)st.n_strat;
     const T tni = (T)ni;
     const T ymax = (T)0.999999;
+    // Cube-index digits: division by n_strat as a multiply-high when every
+    // cube index is below 2^31 (instead of a 64-bit division per axis).
+    // With l = ceil(log2 n_strat) and M = ceil(2^(31+l) / n_strat) < 2^32,
+    // floor(r M / 2^(31+l)) = floor(r / n_strat) for all r < 2^31: the
+    // excess r (M n_strat - 2^(31+l)) / (n_strat 2^(31+l)) is below
+    // r / 2^(31+l) < 2^-l <= 1 / n_strat.
+    const bool fast_div = st.n_cubes <= 0x7fffffffLL;
+    uint32_t div_m = 1u;
+    int div_sh = 0;
+    if (st.n_strat > 1) {
+        const int l = 32 - __builtin_clz((unsigned)(st.n_strat - 1));
+        div_sh = l - 1;
+        div_m = (uint32_t)(((1ull << (31 + l)) + (unsigned long long)st.n_strat - 1ull) /
+                           (unsigned long long)st.n_strat);
+    }
 
     for (int j = 0; j < kSPT; ++j) {
         const long long s = s0 + (long long)j * kBlock + tid;
@@ -427,10 +455,11 @@ __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
         long long h = 0;
         if (active && !A.warmup) {
             if (off_lds) {
-                long long lo = 0, hi = n_off - 1;  // find last i with s_off[i] <= s
+                const int sr = (int)(s - s0);
+                int lo = 0, hi = (int)n_off - 1;  // find last i with s_off[i] <= s - s0
                 while (lo < hi) {
-                    const long long mid = (lo + hi + 1) >> 1;
-                    if (s_off[mid] <= s) lo = mid;
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_off[mid] <= sr) lo = mid;
                     else hi = mid - 1;
                 }
                 h = h_lo + lo;
@@ -461,8 +490,16 @@ __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
                 if (A.warmup) {
                     y = u * ymax;
                 } else {
-                    const long long dig = rem % st.n_strat;
-                    rem /= st.n_strat;
+                    long long dig;
+                    if (fast_div) {
+                        const uint32_t r32 = (uint32_t)rem;
+                        const uint32_t q = st.n_strat > 1 ? (__umulhi(r32, div_m) >> div_sh) : r32;
+                        dig = (long long)(r32 - q * (uint32_t)st.n_strat);
+                        rem = (long long)q;
+                    } else {
+                        dig = rem % st.n_strat;
+                        rem /= st.n_strat;
+                    }
                     y = ((T)dig + u) / tns;
                     if (y >= T(1)) y = ymax;
                 }
