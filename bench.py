@@ -394,7 +394,8 @@ def main():
     if args.gpus != 1:
         print(
             f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
-            "multi-GPU sharding is described in DESIGN.md, not run.",
+            "integrals shard across GPUs without cross-GPU traffic in the hot loop, but only the "
+            "one-GPU throughput is measured here.",
             file=sys.stderr,
         )
         sys.exit(2)
