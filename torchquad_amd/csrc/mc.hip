@@ -18,13 +18,15 @@ struct McArgs {
     Params<T> p;
 };
 
-template <typename T, int FAM>
+// MAXD: compile-time bound on the dimension (4, 8 or 16), so the unrolled
+// per-coordinate work is not predicated over 16 coordinates for d <= 8.
+template <typename T, int FAM, int MAXD>
 __global__ __launch_bounds__(kBlock) void k_mc(McArgs<T> A) {
     __shared__ double s_red[kWavesPerBlock];
     const int d = A.d;
-    T lo[TQ_MAX_DIM], sz[TQ_MAX_DIM];
+    T lo[MAXD], sz[MAXD];
 #pragma unroll
-    for (int k = 0; k < TQ_MAX_DIM; ++k) {
+    for (int k = 0; k < MAXD; ++k) {
         if (k < d) {
             lo[k] = A.domain[2 * k];
             sz[k] = A.domain[2 * k + 1] - A.domain[2 * k];
@@ -41,12 +43,12 @@ __global__ __launch_bounds__(kBlock) void k_mc(McArgs<T> A) {
         const long long se = (sb + kMcFlush < s1) ? (sb + kMcFlush) : s1;
         T fa = T(0), fa2 = T(0);
         for (long long s = sb; s < se; ++s) {
-            T x[TQ_MAX_DIM];
-            Uniform<T>::template draw<TQ_MAX_DIM>(A.seed, A.offset + (uint64_t)s, d, x);
+            T x[MAXD];
+            Uniform<T>::template draw<MAXD>(A.seed, A.offset + (uint64_t)s, d, x);
 #pragma unroll
-            for (int k = 0; k < TQ_MAX_DIM; ++k)
+            for (int k = 0; k < MAXD; ++k)
                 if (k < d) x[k] = fma(x[k], sz[k], lo[k]);
-            const T f = eval_point<FAM, T, TQ_MAX_DIM>(x, d, A.p);
+            const T f = eval_point<FAM, T, MAXD>(x, d, A.p);
             fa += f;
             fa2 = fma(f, f, fa2);
         }
@@ -234,7 +236,12 @@ static int launch_mc(const tq_mc_desc* m, const tq_integrand* f, const void* dom
     A.partials = (double*)ws;
     A.p = make_params<T>(*f);
     TQ_DISPATCH_FAMILY(f->family, {
-        hipLaunchKernelGGL((k_mc<T, FAM>), dim3((unsigned)nb), dim3(kBlock), 0, st, A);
+        if (m->dim <= 4)
+            hipLaunchKernelGGL((k_mc<T, FAM, 4>), dim3((unsigned)nb), dim3(kBlock), 0, st, A);
+        else if (m->dim <= 8)
+            hipLaunchKernelGGL((k_mc<T, FAM, 8>), dim3((unsigned)nb), dim3(kBlock), 0, st, A);
+        else
+            hipLaunchKernelGGL((k_mc<T, FAM, TQ_MAX_DIM>), dim3((unsigned)nb), dim3(kBlock), 0, st, A);
     });
     int rc = check_launch("k_mc");
     if (rc) return rc;
