@@ -350,7 +350,9 @@ __device__ inline double uniform_coord<double>(uint64_t seed, uint64_t s, int k,
     return (k & 1) ? u01d(blk.v[2], blk.v[3]) : u01d(blk.v[0], blk.v[1]);
 }
 
-template <typename T, int FAM, int MODE, bool LDS_HIST>
+// MAXD: compile-time bound on the dimension; the fused launches use 8 for
+// d <= 8 so the unrolled per-axis work is not predicated over 16 axes.
+template <typename T, int FAM, int MODE, bool LDS_HIST, int MAXD = TQ_MAX_DIM>
 __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
     using F = Family<FAM, T>;
     __shared__ int s_off[kOffLds];  // cube offsets relative to the block's first sample, clamped
@@ -417,10 +419,10 @@ __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
     __syncthreads();
 
     // domain transform (vegas.py:104-112)
-    T start[TQ_MAX_DIM], size[TQ_MAX_DIM];
+    T start[MAXD], size[MAXD];
     T vol = T(1);
 #pragma unroll
-    for (int k = 0; k < TQ_MAX_DIM; ++k) {
+    for (int k = 0; k < MAXD; ++k) {
         if (k < d && MODE != MODE_Y) {
             start[k] = A.domain[2 * k];
             size[k] = A.domain[2 * k + 1] - A.domain[2 * k];
@@ -473,13 +475,13 @@ __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
                 h = lo;
             }
         }
-        T x[TQ_MAX_DIM];
-        int ids[TQ_MAX_DIM];
+        T x[MAXD];
+        int ids[MAXD];
         double jac = 1.0;
         long long rem = h;
         u32x4 blk;
 #pragma unroll
-        for (int k = 0; k < TQ_MAX_DIM; ++k) {
+        for (int k = 0; k < MAXD; ++k) {
             ids[k] = -1;
             x[k] = T(0);
             if (k < d) {
@@ -532,17 +534,21 @@ __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
         }
         // fused: integrand
         T fv = T(0);
-        if (active) fv = (T)(A.p.c0 * eval_point<FAM, T, TQ_MAX_DIM>(x, d, A.p)) * vol;
+        if (active) fv = (T)(A.p.c0 * eval_point<FAM, T, MAXD>(x, d, A.p)) * vol;
         const double jf = (double)fv * jac;
         if (active && !A.warmup) st.jf[s] = jf;
         if (A.hist) {
             const double jf2 = jf * jf;
-            for (int k = 0; k < d; ++k) {
-                const int id = active ? ids[k] : -1;
-                if (LDS_HIST)
-                    wave_hist_add(s_hw + wave * W, s_hc + wave * W, k, ni, id, jf2);
-                else
-                    wave_hist_add_global(st.weights, st.counts, k, ni, id, jf2);
+            // unrolled with a uniform bound check, so ids[] stays in registers
+#pragma unroll
+            for (int k = 0; k < MAXD; ++k) {
+                if (k < d) {
+                    const int id = active ? ids[k] : -1;
+                    if (LDS_HIST)
+                        wave_hist_add(s_hw + wave * W, s_hc + wave * W, k, ni, id, jf2);
+                    else
+                        wave_hist_add_global(st.weights, st.counts, k, ni, id, jf2);
+                }
             }
         }
     }
@@ -1307,8 +1313,14 @@ static int vegas_iterate(const tq_vegas_state* st, const tq_integrand* f, const 
     // wave-private histograms: (FP64 weight + uint32 count) per bin and wave
     const size_t hist_lds = (size_t)kWavesPerBlock * W * (sizeof(double) + sizeof(uint32_t));
     TQ_DISPATCH_FAMILY(f->family, {
-        if (lds_hist)
+        if (lds_hist && st->dim <= 8)
+            hipLaunchKernelGGL((k_vegas_sample<T, FAM, MODE_FUSED, true, 8>), dim3((unsigned)nb), dim3(kBlock),
+                               hist_lds, s, A);
+        else if (lds_hist)
             hipLaunchKernelGGL((k_vegas_sample<T, FAM, MODE_FUSED, true>), dim3((unsigned)nb), dim3(kBlock), hist_lds,
+                               s, A);
+        else if (st->dim <= 8)
+            hipLaunchKernelGGL((k_vegas_sample<T, FAM, MODE_FUSED, false, 8>), dim3((unsigned)nb), dim3(kBlock), 0,
                                s, A);
         else
             hipLaunchKernelGGL((k_vegas_sample<T, FAM, MODE_FUSED, false>), dim3((unsigned)nb), dim3(kBlock), 0, s, A);
