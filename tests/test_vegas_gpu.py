@@ -246,6 +246,23 @@ def test_warm_start_and_state_dict(dev):
         tq.VEGAS().integrate(f, 3, N=60_000, integration_domain=dom, seed=2, N_intervals=10, warm_start=st)
 
 
+def test_reused_state_matches_fresh(dev):
+    """A VEGAS object reuses its device state across integrate() calls of the same shape (one fill
+    instead of new buffers): every call gives exactly what a fresh object gives, also after a larger
+    call grew the sample buffers and for a user callable."""
+    f = tq.DoubleGaussian(3, width=0.2)
+    dom = torch.tensor([[0.0, 1.0]] * 3, dtype=torch.float64, device=dev)
+
+    def user(x):
+        return torch.exp(-torch.sum((x - 0.5) ** 2, dim=1) / 0.08)
+
+    v = tq.VEGAS()
+    for fn, N, seed in ((f, 60_000, 1), (f, 200_000, 2), (f, 60_000, 1), (user, 60_000, 3), (f, 60_000, 1)):
+        again = float(v.integrate(fn, 3, N=N, integration_domain=dom, seed=seed, N_intervals=20))
+        fresh = float(tq.VEGAS().integrate(fn, 3, N=N, integration_domain=dom, seed=seed, N_intervals=20))
+        assert again == fresh, (N, seed, again, fresh)
+
+
 def test_n_strat_override(dev):
     f = tq.DoubleGaussian(3, width=0.2)
     dom = torch.tensor([[0.0, 1.0]] * 3, dtype=torch.float64, device=dev)

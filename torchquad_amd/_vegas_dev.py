@@ -103,6 +103,15 @@ class VegasDevice:
             self._struct = s
         return ctypes.byref(self._struct)
 
+    _BUFFERS = ("x_edges", "dx_edges", "weights", "counts", "dh", "jf_cube", "jf2_cube", "nh", "offsets",
+                "scan_sums", "iter_stats", "flags", "jf", "hist_w", "hist_c", "chunk_cube", "block_sums")
+
+    def reset(self):
+        """Bring a used state back to the freshly built one (every buffer zero, then the map and
+        stratification initialised) with one multi-tensor fill instead of new allocations."""
+        torch._foreach_zero_([getattr(self, name) for name in self._BUFFERS])
+        self.init()
+
     def stream(self):
         return stream_ptr(self.device)
 

@@ -113,7 +113,17 @@ class VEGAS(BaseIntegrator):
             n_strat = int(N_strat)
             if n_strat < 1 or n_strat**dim > 1 << 31:
                 raise ValueError("N_strat must be >= 1 with N_strat**dim < 2**31")
-        dev = _vegas_dev.VegasDevice(dim, N_intervals, max(n_strat, 1), self.dtype, domain.device)
+        # The device state of the previous call is reused when its shape, dtype, device and stream
+        # match: one fill restores the freshly built state instead of about twenty allocations.
+        key = (int(dim), int(N_intervals), max(n_strat, 1), self.dtype, domain.device,
+               torch.cuda.current_stream(domain.device).cuda_stream if domain.is_cuda else None)
+        cached = getattr(self, "_dev_cache", None)
+        if cached is not None and cached[0] == key:
+            dev = cached[1]
+            dev.reset()
+        else:
+            dev = _vegas_dev.VegasDevice(dim, N_intervals, max(n_strat, 1), self.dtype, domain.device)
+            self._dev_cache = (key, dev)
         self._dev = dev
         self.map = VEGASMap(N_intervals, dim, self.backend, self.dtype, _device=dev)
         self.strat = VEGASStratification(self._N_increment, dim, rng, self.backend, self.dtype, _device=dev)
