@@ -39,11 +39,12 @@ def test_rng_class_reproducible(dev):
     assert not torch.equal(b1, b2)
 
 
+@pytest.mark.parametrize("d", [3, 5, 12])  # the kernel's dimension bounds 4, 8 and 16
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("fname", ["gaussian_peak", "oscillatory", "product_peak", "exp_linear", "constant"])
-def test_fused_mc_vs_same_stream_oracle(dtype, fname, dev):
+def test_fused_mc_vs_same_stream_oracle(dtype, fname, d, dev):
     rng = np.random.default_rng(5)
-    d, N, seed = 5, 200_000, 77
+    N, seed = 200_000, 77
     a, u, c = rng.uniform(1, 3, d), rng.uniform(-0.5, 0.5, d), rng.uniform(0, 2, d)
     dom = [[-0.5, 1.0]] * d
     f, kw = {

@@ -108,9 +108,10 @@ def test_stratification_bit_exact(dtype, dev, ref_vectors):
     assert int(st.get_NH(4000)[-1]) > int(st.get_NH(4000)[0])
 
 
-def test_fused_end_to_end_fp64_matches_oracle(dev):
-    dim, N, seed = 3, 60000, 5
-    dom = [[0.0, 1.0], [-0.5, 1.0], [0.2, 0.9]]
+@pytest.mark.parametrize("dim", [3, 9])  # the sample kernel's dimension bounds 8 and 16
+def test_fused_end_to_end_fp64_matches_oracle(dim, dev):
+    N, seed = 60000, 5
+    dom = ([[0.0, 1.0], [-0.5, 1.0], [0.2, 0.9]] * 3)[:dim]
     f = tq.DoubleGaussian(dim, width=0.3)
     v = tq.VEGAS()
     val = float(v.integrate(f, dim, N=N, integration_domain=torch.tensor(dom, dtype=torch.float64, device=dev),
