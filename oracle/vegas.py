@@ -98,7 +98,14 @@ class Map:
             np.add.at(self.counts[i], ID[:, i], 1)
 
     @staticmethod
-    def smooth(weights, counts, alpha):
+    def fill_zero_counts(weights, counts):
+        """Average weight per bin, bins without samples filled from neighbours
+        (vegas_map.py:118-142): divide by the count where it is non-zero, then
+        up to ten rounds of: every empty bin takes its right neighbour's value
+        and stops being empty if that neighbour was not; then every empty bin
+        takes its left neighbour's value likewise.  Each round closes a gap by
+        one bin from each side; bins more than ten from either side of a gap
+        keep weight zero."""
         weights = np.array(weights, dtype=np.float64)
         counts = np.asarray(counts)
         z_idx = counts == 0
@@ -114,6 +121,15 @@ class Map:
                     break
         else:
             weights /= counts
+        return weights
+
+    @staticmethod
+    def smooth(weights, counts, alpha):
+        """_smooth_map (vegas_map.py:113-172): zero-count fill, [7,1] / [1,6,1]
+        / [1,7] smoothing over 8 times the row sum, then range compression
+        ((d - 1) / ln d)^alpha of the non-zero entries.  None when a row sums
+        to zero."""
+        weights = Map.fill_zero_counts(weights, counts)
         dim, n = weights.shape
         sums = np.reshape(np.sum(weights, axis=1), [dim, 1])
         if np.any(sums == 0.0):
@@ -270,42 +286,64 @@ def run(f, dim, N, domain, seed, dtype=np.float64, N_intervals=None, max_iterati
         st.update_DH()
         if it % 5 > 0:
             continue
-        res = _result(results, sigma2)
-        err = _error(sigma2)
-        chi2 = _chisq(results, sigma2)
-        if (err <= eps_rel * abs(res) or err <= eps_abs) and chi2 / 5.0 < 1.0:
-            break
-        if chi2 / 5.0 < 1.0:
-            if abs(res) == 0.0:
-                starting_N += N_inc
-            else:
-                acc = err / abs(res)
-                starting_N = min(starting_N + N_inc, int(starting_N * math.sqrt(acc / (eps_rel + 1e-8))))
-        elif chi2 / 5.0 > 1.0:
-            starting_N += N_inc
-        if fevals + starting_N * 5 > N:
-            break
-        if it + 5 > max_iterations:
+        stop, starting_N = abort_check(it, results, sigma2, eps_rel, eps_abs, starting_N, N_inc, fevals, N,
+                                       max_iterations)
+        if stop:
             break
         results, sigma2 = [], []
     return dict(result=_result(results, sigma2), results=results, sigma2=sigma2, all=all_results, fevals=fevals,
                 map=m, strat=st, iterations=it)
 
 
+def abort_check(it, results, sigma2, eps_rel, eps_abs, starting_N, N_increment, fevals, N, max_iterations):
+    """VEGAS._check_abort_conditions (vegas.py:161-209) as a pure function of
+    the driver state.  Returns (stop, starting_N); the caller clears results
+    and sigma2 when it goes on.  Called only when it % 5 == 0 (:171-173)."""
+    res_abs = abs(_result(results, sigma2))
+    err = _error(sigma2)
+    chi2 = _chisq(results, sigma2)
+    if (err <= eps_rel * res_abs or err <= eps_abs) and chi2 / 5.0 < 1.0:
+        return True, starting_N
+    if chi2 / 5.0 < 1.0:
+        if res_abs == 0.0:
+            starting_N += N_increment
+        else:
+            acc = err / res_abs
+            starting_N = min(starting_N + N_increment, int(starting_N * math.sqrt(acc / (eps_rel + 1e-8))))
+    elif chi2 / 5.0 > 1.0:
+        starting_N += N_increment
+    if fevals + starting_N * 5 > N:
+        return True, starting_N
+    if it + 5 > max_iterations:
+        return True, starting_N
+    return False, starting_N
+
+
 def _result(results, sigma2):
+    """EQ 30 (vegas.py:318-335): inverse-variance weighted mean, or the plain
+    mean when any variance is zero."""
     if any(s == 0.0 for s in sigma2):
         return sum(results) / len(results)
     return sum(r / s for r, s in zip(results, sigma2)) / sum(1.0 / s for s in sigma2)
 
 
 def _error(sigma2):
+    """EQ 31 (vegas.py:337-346): zero variances are skipped; sigma2[0] when
+    all are zero."""
     res = sum(1.0 / s for s in sigma2 if s != 0.0)
     return sigma2[0] if res == 0 else 1.0 / math.sqrt(res)
 
 
 def _chisq(results, sigma2):
+    """EQ 32 (vegas.py:348-362) over the entries that differ from the result.
+    The reference divides backend tensors, so a zero variance there gives
+    inf, not an exception."""
     I = _result(results, sigma2)
-    return sum(((r - I) ** 2 / s for r, s in zip(results, sigma2) if r != I), 0.0)
+    total = 0.0
+    for r, s in zip(results, sigma2):
+        if r != I:
+            total += (r - I) ** 2 / s if s != 0.0 else math.inf
+    return total
 
 
 def edges_by_inverse_cdf(sw, x_edges, dx_edges):

@@ -268,3 +268,51 @@ def test_serving_helpers():
         ts.append(t)
     st = latency_stats(ts)
     assert st["count"] == 100 and st["p50_ms"] == pytest.approx(51.0) and st["p99_ms"] == pytest.approx(100.0)
+
+
+def _driver(results, sigma2, it, eps_rel=0.0, eps_abs=0.0, sN=1000, inc=1000, fevals=0, N=10**9, max_it=20):
+    v = tq.VEGAS()
+    v.results, v.sigma2, v._slots = list(results), list(sigma2), []
+    v.it, v._eps_rel, v._eps_abs = it, eps_rel, eps_abs
+    v._starting_N, v._N_increment, v._nr_of_fevals, v.N, v._max_iterations = sN, inc, fevals, N, max_it
+    return v
+
+
+DRIVER_CASES = [
+    ([1.0, 2.0, 4.0], [1.0, 4.0, 16.0], {}),
+    ([1.0, 2.0, 6.0], [0.5, 0.0, 2.0], {}),
+    ([1.0] * 5, [0.01] * 5, dict(eps_rel=0.1)),
+    ([1.0] * 5, [0.01] * 5, dict(eps_abs=0.05)),
+    ([1.0] * 5, [0.01] * 5, {}),
+    ([1.0] * 5, [5 * 0.0121**2] * 5, dict(eps_rel=0.01, sN=10000, inc=10000)),
+    ([0.0] * 5, [0.01] * 5, {}),
+    ([3.0, -1.0, 3.0, -1.0, 1.0], [2.0] * 5, {}),
+    ([3.0, -1.0, 2.0, 0.0, 1.0], [2.0] * 5, {}),
+    ([1.0, 2.0, 1.0, 1.0, 1.0], [1.0, 0.0, 1.0, 1.0, 1.0], {}),
+    ([1.0] * 5, [0.01] * 5, dict(fevals=5001, N=15000)),
+    ([1.0] * 5, [0.01] * 5, dict(max_it=19)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(DRIVER_CASES)))
+def test_vegas_driver_host_logic_matches_oracle(case):
+    """EQ 30-32 and the abort / N schedule of the VEGAS driver (vegas.py:161-209,
+    318-362) against the oracle's restatement, whose cases are pinned by hand."""
+    from oracle import vegas as ov
+
+    r, s, kw = DRIVER_CASES[case]
+    v = _driver(r, s, 5, **kw)
+    assert float(v._get_result()) == ov._result(r, s)
+    assert float(v._get_error()) == ov._error(s)
+    assert float(v._get_chisq()) == ov._chisq(r, s)
+    args = dict(eps_rel=0.0, eps_abs=0.0, sN=1000, inc=1000, fevals=0, N=10**9, max_it=20)
+    args.update(kw)
+    stop, sN = ov.abort_check(5, r, s, args["eps_rel"], args["eps_abs"], args["sN"], args["inc"], args["fevals"],
+                              args["N"], args["max_it"])
+    assert v._check_abort_conditions() == stop
+    assert v._starting_N == sN
+    if not stop:
+        assert v.results == [] and v.sigma2 == []
+    # not a fifth iteration: never stops, nothing changes
+    v = _driver(r, s, 4, **kw)
+    assert v._check_abort_conditions() is False and v._starting_N == args["sN"]

@@ -129,3 +129,173 @@ def test_add_at_indices_vs_numpy_add_at(n, m):
     reff = np.zeros(m)
     np.add.at(reff, idx, f)
     np.testing.assert_allclose(ov.add_at_indices(tf, idx, f), reff, rtol=1e-12, atol=1e-12)
+
+
+# ---- zero-count fill (vegas_map.py:118-142), hand-traced expectations ------
+# Each round of the reference loop first lets every empty bin take its right
+# neighbour's value (an empty bin whose right neighbour is filled becomes
+# filled), then every still-empty bin its left neighbour's value.  The
+# expected rows below were traced by hand through those two steps.
+FILL_CASES = [
+    # one empty bin between two filled ones: filled from the right
+    ([[2.0, 0.0, 9.0]], [[1, 0, 3]], [[2.0, 3.0, 3.0]]),
+    # two empty bins: the right one from the right, the left one from the left
+    ([[4.0, 0.0, 0.0, 10.0]], [[2, 0, 0, 5]], [[2.0, 2.0, 2.0, 2.0]]),
+    ([[4.0, 0.0, 0.0, 30.0]], [[1, 0, 0, 5]], [[4.0, 4.0, 6.0, 6.0]]),
+    # three empty bins: the middle one is reached from the right in round two
+    ([[1.0, 0.0, 0.0, 0.0, 8.0]], [[1, 0, 0, 0, 1]], [[1.0, 1.0, 8.0, 8.0, 8.0]]),
+    # empty bins at the right edge: only the left fill reaches them
+    ([[3.0, 5.0, 0.0, 0.0]], [[1, 1, 0, 0]], [[3.0, 5.0, 5.0, 5.0]]),
+    # empty bins at the left edge: only the right fill reaches them
+    ([[0.0, 0.0, 7.0, 1.0]], [[0, 0, 1, 1]], [[7.0, 7.0, 7.0, 1.0]]),
+    # two axes are filled independently
+    ([[0.0, 6.0, 0.0], [5.0, 0.0, 0.0]], [[0, 2, 0], [1, 0, 0]], [[3.0, 3.0, 3.0], [5.0, 5.0, 5.0]]),
+]
+
+
+@pytest.mark.parametrize("case", range(len(FILL_CASES)))
+def test_zero_count_fill_hand_traced(case):
+    w, c, expected = FILL_CASES[case]
+    out = ov.Map.fill_zero_counts(np.array(w), np.array(c))
+    assert out.tolist() == expected
+
+
+def test_zero_count_fill_long_gap():
+    # a gap of 22 empty bins closes by one bin per side per round; after the
+    # ten rounds bins 11 and 12 are still empty and keep weight zero
+    n = 25
+    w = np.zeros((1, n))
+    c = np.zeros((1, n), dtype=np.int64)
+    w[0, 0], c[0, 0] = 2.0, 1
+    w[0, 23], c[0, 23] = 9.0, 3
+    w[0, 24], c[0, 24] = 5.0, 1
+    out = ov.Map.fill_zero_counts(w, c)[0]
+    assert out[:11].tolist() == [2.0] * 11
+    assert out[11:13].tolist() == [0.0, 0.0]
+    assert out[13:24].tolist() == [3.0] * 11
+    assert out[24] == 5.0
+    # the smoothed map then has non-zero entries next to the remaining gap
+    # ([1,6,1] of (2, 0, 0) and (0, 0, 3)) and none inside a wider one
+    d = ov.Map.smooth(w, c, 0.5)[0]
+    assert d[11] > 0 and d[12] > 0
+    w2 = np.zeros((1, 30))
+    c2 = np.zeros((1, 30), dtype=np.int64)
+    w2[0, 0], c2[0, 0] = 1.0, 1
+    w2[0, 29], c2[0, 29] = 1.0, 1
+    filled = ov.Map.fill_zero_counts(w2, c2)[0]
+    assert np.count_nonzero(filled == 0.0) == 8
+    d2 = ov.Map.smooth(w2, c2, 0.5)[0]
+    # bins 12..17 are empty with empty neighbours: smoothed weight stays 0
+    assert d2[12:18].tolist() == [0.0] * 6 and np.all(d2[:11] > 0) and np.all(d2[19:] > 0)
+
+
+def _compressed(raw):
+    out = np.zeros_like(raw)
+    nz = raw != 0
+    out[nz] = (raw[nz] - 1.0) / np.log(raw[nz])
+    return out
+
+
+def test_smooth_closed_form():
+    # uniform averages: every smoothed entry is 1/n before compression, and
+    # ((1/n - 1) / ln(1/n))^alpha after it
+    for n, alpha in ((4, 0.5), (50, 0.5), (7, 1.0)):
+        d = ov.Map.smooth(np.full((2, n), 3.0), np.full((2, n), 3), alpha)
+        expect = ((1.0 / n - 1.0) / math.log(1.0 / n)) ** alpha
+        np.testing.assert_allclose(d, expect, rtol=1e-15, atol=0)
+    # a single non-zero bin: [7,1] at the left edge, [1,6,1] inside, [1,7] at
+    # the right edge, all over 8 * sum
+    w = np.array([[0.0, 0.0, 8.0, 0.0, 0.0]])
+    c = np.ones((1, 5), dtype=np.int64)
+    d = ov.Map.smooth(w, c, 1.0)[0]
+    raw = np.array([0.0, 8.0, 48.0, 8.0, 0.0]) / 64.0
+    expect = _compressed(raw)
+    np.testing.assert_allclose(d, expect, rtol=1e-15, atol=0)
+    w = np.array([[8.0, 0.0, 0.0, 0.0, 16.0]])
+    d = ov.Map.smooth(w, c, 1.0)[0]
+    raw = np.array([56.0, 8.0, 0.0, 16.0, 112.0]) / 192.0
+    expect = _compressed(raw)
+    np.testing.assert_allclose(d, expect, rtol=1e-15, atol=0)
+    # a row that sums to zero cannot be smoothed
+    assert ov.Map.smooth(np.zeros((1, 4)), np.zeros((1, 4), dtype=np.int64), 0.5) is None
+
+
+# ---- driver estimators and abort schedule (vegas.py:161-209, 318-362) ------
+def test_driver_estimators_closed_form():
+    # weights 1, 1/4, 1/16: result (1 + 2/4 + 4/16) / (21/16) = 4/3,
+    # error 1/sqrt(21/16), chi^2 = (1/3)^2 + (2/3)^2/4 + (8/3)^2/16 = 2/3
+    r, s = [1.0, 2.0, 4.0], [1.0, 4.0, 16.0]
+    assert ov._result(r, s) == pytest.approx(4.0 / 3.0, rel=1e-15)
+    assert ov._error(s) == pytest.approx(4.0 / math.sqrt(21.0), rel=1e-15)
+    assert ov._chisq(r, s) == pytest.approx(2.0 / 3.0, rel=1e-14)
+    # a zero variance: plain mean, zero skipped in the error, inf in chi^2
+    r, s = [1.0, 2.0, 6.0], [0.5, 0.0, 2.0]
+    assert ov._result(r, s) == 3.0
+    assert ov._error(s) == pytest.approx(1.0 / math.sqrt(2.5), rel=1e-15)
+    assert ov._chisq(r, s) == math.inf
+    # every variance zero: the error is sigma2[0]
+    assert ov._error([0.0, 0.0]) == 0.0
+    # entries equal to the result are left out of chi^2 (even with zero variance)
+    assert ov._chisq([2.0, 2.0], [0.0, 0.0]) == 0.0
+
+
+def _abort(it, r, s, eps_rel=0.0, eps_abs=0.0, sN=1000, inc=1000, fevals=0, N=10**9, max_it=20):
+    return ov.abort_check(it, r, s, eps_rel, eps_abs, sN, inc, fevals, N, max_it)
+
+
+def test_abort_check_branches():
+    ok = [1.0] * 5
+    tiny = [0.01] * 5  # error 1/sqrt(500) = 0.0447, chi^2 = 0
+    # accuracy reached
+    assert _abort(5, ok, tiny, eps_rel=0.1) == (True, 1000)
+    assert _abort(5, ok, tiny, eps_abs=0.05) == (True, 1000)
+    # accuracy not reached, stable: N grows by the increment (the sqrt bound
+    # is far above it with eps_rel = 0)
+    assert _abort(5, ok, tiny) == (False, 2000)
+    # stable with eps_rel: N * sqrt(acc / (eps_rel + 1e-8)), acc = 0.0121,
+    # eps_rel = 0.01: 10000 * 1.09999945 -> 10999 (11000 without the 1e-8)
+    sig = [5 * 0.0121**2] * 5
+    assert _abort(5, ok, sig, eps_rel=0.01, sN=10000, inc=10000) == (False, 10999)
+    # a zero result adds the increment
+    assert _abort(5, [0.0] * 5, tiny) == (False, 2000)
+    # unstable: results 3,-1,3,-1,1 with variance 2 give I = 1 and
+    # chi^2 = (4 + 4 + 4 + 4 + 0) / 2 = 8 > 5: increment
+    assert _abort(5, [3.0, -1.0, 3.0, -1.0, 1.0], [2.0] * 5) == (False, 2000)
+    # chi^2 / 5 exactly 1 (results 3,-1,2,0,1, variance 2: (4+4+1+1)/2 = 5):
+    # neither branch, N unchanged
+    assert _abort(5, [3.0, -1.0, 2.0, 0.0, 1.0], [2.0] * 5) == (False, 1000)
+    # a zero variance that differs from the mean: chi^2 = inf, increment
+    assert _abort(5, [1.0, 2.0, 1.0, 1.0, 1.0], [1.0, 0.0, 1.0, 1.0, 1.0]) == (False, 2000)
+    # budget: fevals + 5 * new N > N stops, == N goes on
+    assert _abort(5, ok, tiny, fevals=5000, N=15000) == (False, 2000)
+    assert _abort(5, ok, tiny, fevals=5001, N=15000) == (True, 2000)
+    # iteration cap: it + 5 > max_iterations stops
+    assert _abort(15, ok, tiny, max_it=20) == (False, 2000)
+    assert _abort(20, ok, tiny, max_it=20) == (True, 2000)
+    assert _abort(15, ok, tiny, max_it=19) == (True, 2000)
+
+
+def test_abort_schedule_scripted_trace():
+    # A scripted run of the driver loop (vegas.py:130-146) with mocked
+    # iteration results: each iteration uses starting_N evaluations, the
+    # results of a block of five are noisy (chi^2 / 5 = 8 / 5 > 1) for the
+    # first block and then stable.  N = 230000, max_iterations = 20 give
+    # starting_N = N_increment = 230000 // 25 = 9200 (vegas.py:114-115).
+    N, max_it = 230000, 20
+    sN = inc = N // (max_it + 5)
+    assert sN == 9200
+    blocks = [([3.0, -1.0, 3.0, -1.0, 1.0], [2.0] * 5), ([1.0] * 5, [0.01] * 5), ([1.0] * 5, [0.01] * 5)]
+    fevals, it, trace = 0, 0, []
+    while True:
+        it += 1
+        fevals += sN
+        if it % 5 > 0:
+            continue
+        r, s = blocks[it // 5 - 1]
+        stop, sN = ov.abort_check(it, r, s, 0.0, 0.0, sN, inc, fevals, N, max_it)
+        trace.append((it, fevals, sN, stop))
+        if stop:
+            break
+    # block 1: unstable, N 9200 -> 18400; 46000 + 92000 <= 230000 goes on.
+    # block 2: stable, N 18400 -> 27600; 138000 + 138000 > 230000 stops.
+    assert trace == [(5, 46000, 18400, False), (10, 138000, 27600, True)]

@@ -352,10 +352,12 @@ class VEGAS(BaseIntegrator):
         return self.sigma2[0] if res == 0 else 1.0 / math.sqrt(res)
 
     def _get_chisq(self):
-        """Chi square, EQ 32 (vegas.py:348-362)."""
+        """Chi square, EQ 32 (vegas.py:348-362).  The reference divides backend
+        tensors, so an entry with zero variance that differs from the result
+        contributes inf."""
         I_final = float(self._get_result())
         return sum(
-            ((float(res) - I_final) ** 2 / sig2 for res, sig2 in zip(self.results, self.sigma2)
-             if float(res) != I_final),
+            (((float(res) - I_final) ** 2 / sig2 if sig2 != 0.0 else math.inf)
+             for res, sig2 in zip(self.results, self.sigma2) if float(res) != I_final),
             0.0,
         )
