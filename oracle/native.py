@@ -22,8 +22,8 @@ def lib():
         L.oracle_uniform_f32.argtypes = [U64, U64, I64, ctypes.c_int32, P]
         L.oracle_uniform_f64.argtypes = [U64, U64, I64, ctypes.c_int32, P]
         L.oracle_grid_sum.restype = D
-        L.oracle_grid_sum.argtypes = [I, I, I, I, P, D, D, D, D, P, P, P, P]
-        L.oracle_mc_sum.argtypes = [I, I, I, I64, U64, U64, P, D, D, D, D, P, P, P, P]
+        L.oracle_grid_sum.argtypes = [I, I, I, I, P, D, D, D, D, P, P, P, P, P]
+        L.oracle_mc_sum.argtypes = [I, I, I, I64, U64, U64, P, D, D, D, D, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -69,8 +69,10 @@ def _params(dim, a, u):
     return A, U
 
 
-def grid_sum(rule, family, dim, n, domain, c0=1.0, c1=0.0, c2=0.0, c3=0.0, a=None, u=None):
-    """FP64 brute-force tensor-product rule with full per-point evaluation."""
+def grid_sum(rule, family, dim, n, domain, c0=1.0, c1=0.0, c2=0.0, c3=0.0, a=None, u=None, with_abs=False):
+    """FP64 brute-force tensor-product rule with full per-point evaluation.
+
+    with_abs: return (sum w f, sum |w f|) instead of the sum alone."""
     dom = np.ascontiguousarray(np.asarray(domain, dtype=np.float64).reshape(-1))
     A, U = _params(dim, a, u)
     glx = glw = None
@@ -78,16 +80,21 @@ def grid_sum(rule, family, dim, n, domain, c0=1.0, c1=0.0, c2=0.0, c3=0.0, a=Non
         glx, glw = np.polynomial.legendre.leggauss(n)
         glx = np.ascontiguousarray(glx)
         glw = np.ascontiguousarray(glw)
-    return lib().oracle_grid_sum(RULES[rule], FAMILIES[family], dim, n, _p(dom), c0, c1, c2, c3, _p(A), _p(U),
-                                 _p(glx), _p(glw))
+    asum = ctypes.c_double()
+    total = lib().oracle_grid_sum(RULES[rule], FAMILIES[family], dim, n, _p(dom), c0, c1, c2, c3, _p(A), _p(U),
+                                  _p(glx), _p(glw), ctypes.byref(asum))
+    return (total, asum.value) if with_abs else total
 
 
-def mc_sums(dtype, family, dim, N, seed, offset, domain, c1=0.0, c2=0.0, c3=0.0, a=None, u=None):
-    """(sum f, sum f^2) over the library's sample stream, f in FP64 (no c0)."""
+def mc_sums(dtype, family, dim, N, seed, offset, domain, c1=0.0, c2=0.0, c3=0.0, a=None, u=None, with_abs=False):
+    """(sum f, sum f^2) over the library's sample stream, f in FP64 (no c0);
+    with_abs: (sum f, sum f^2, sum |f|)."""
     dom = np.ascontiguousarray(np.asarray(domain, dtype=np.float64).reshape(-1))
     A, U = _params(dim, a, u)
     s1 = ctypes.c_double()
     s2 = ctypes.c_double()
+    sa = ctypes.c_double()
     lib().oracle_mc_sum(0 if dtype == "float32" else 1, FAMILIES[family], dim, N, seed & 0xFFFFFFFFFFFFFFFF,
-                        offset, _p(dom), 1.0, c1, c2, c3, _p(A), _p(U), ctypes.byref(s1), ctypes.byref(s2))
-    return s1.value, s2.value
+                        offset, _p(dom), 1.0, c1, c2, c3, _p(A), _p(U), ctypes.byref(s1), ctypes.byref(s2),
+                        ctypes.byref(sa))
+    return (s1.value, s2.value, sa.value) if with_abs else (s1.value, s2.value)

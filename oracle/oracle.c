@@ -8,6 +8,8 @@
  *                         (include/torchquad_amd.h, tq_uniform).
  *   oracle_grid_sum       brute-force tensor-product rule: every point decoded
  *                         from its flat index, full per-axis weights
+ *                         (optionally also sum |w f|, the scale of the
+ *                         rounding error of any summation order)
  *                         (trapezoid.py:28-37, simpson.py:30-45, boole.py:30-48,
  *                         gaussian.py:44-69,132-144 written as weight vectors),
  *                         full integrand evaluation per point, FP64 sum.
@@ -157,7 +159,7 @@ static void axis_rule(int rule, double a, double b, int n, const double* glx, co
 
 double oracle_grid_sum(int rule, int family, int dim, int n, const double* domain, double c0, double c1,
                        double c2, double c3, const double* a, const double* u, const double* glx,
-                       const double* glw) {
+                       const double* glw, double* abs_sum) {
     fam_t F = {family, dim, c0, c1, c2, c3, a, u};
     double* X = (double*)malloc(sizeof(double) * dim * n);
     double* W = (double*)malloc(sizeof(double) * dim * n);
@@ -165,7 +167,7 @@ double oracle_grid_sum(int rule, int family, int dim, int n, const double* domai
     double* x = (double*)malloc(sizeof(double) * dim);
     int k;
     for (k = 0; k < dim; ++k) axis_rule(rule, domain[2 * k], domain[2 * k + 1], n, glx, glw, X + k * n, W + k * n);
-    double total = 0.0, comp = 0.0;
+    double total = 0.0, comp = 0.0, atotal = 0.0;
     for (;;) {
         double w = 1.0;
         for (k = 0; k < dim; ++k) {
@@ -173,8 +175,10 @@ double oracle_grid_sum(int rule, int family, int dim, int n, const double* domai
             w *= W[k * n + dig[k]];
         }
         /* Kahan-compensated FP64 sum */
-        double term = w * fam_eval(&F, x) - comp;
+        double wf = w * fam_eval(&F, x);
+        double term = wf - comp;
         double t = total + term;
+        atotal += fabs(wf);
         comp = (t - total) - term;
         total = t;
         for (k = dim - 1; k >= 0; --k) {
@@ -187,16 +191,17 @@ double oracle_grid_sum(int rule, int family, int dim, int n, const double* domai
     free(W);
     free(dig);
     free(x);
+    if (abs_sum) *abs_sum = atotal;
     return total;
 }
 
 /* ---- Monte Carlo on the library's stream -------------------------------- */
 void oracle_mc_sum(int dtype, int family, int dim, int64_t N, uint64_t seed, uint64_t offset,
                    const double* domain, double c0, double c1, double c2, double c3, const double* a,
-                   const double* u, double* sum, double* sum2) {
+                   const double* u, double* sum, double* sum2, double* sum_abs) {
     fam_t F = {family, dim, 1.0, c1, c2, c3, a, u};
     double x[64];
-    double s1 = 0.0, s2 = 0.0;
+    double s1 = 0.0, s2 = 0.0, sa = 0.0;
     int64_t i;
     int k;
     (void)c0;
@@ -219,7 +224,9 @@ void oracle_mc_sum(int dtype, int family, int dim, int64_t N, uint64_t seed, uin
         double f = fam_eval(&F, x);
         s1 += f;
         s2 += f * f;
+        sa += fabs(f);
     }
     *sum = s1;
     *sum2 = s2;
+    if (sum_abs) *sum_abs = sa;
 }

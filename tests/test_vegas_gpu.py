@@ -108,15 +108,18 @@ def test_stratification_bit_exact(dtype, dev, ref_vectors):
     assert int(st.get_NH(4000)[-1]) > int(st.get_NH(4000)[0])
 
 
-@pytest.mark.parametrize("dim", [3, 9])  # the sample kernel's dimension bounds 8 and 16
-def test_fused_end_to_end_fp64_matches_oracle(dim, dev):
+# (dim, bins): the sample kernel's instantiations -- dimension bound 8 or 16,
+# wave histograms in LDS (dim * bins <= 768) or global atomics (default bins)
+@pytest.mark.parametrize("dim,bins", [(3, None), (9, None), (3, 40), (9, 40)])
+def test_fused_end_to_end_fp64_matches_oracle(dim, bins, dev):
     N, seed = 60000, 5
     dom = ([[0.0, 1.0], [-0.5, 1.0], [0.2, 0.9]] * 3)[:dim]
     f = tq.DoubleGaussian(dim, width=0.3)
     v = tq.VEGAS()
     val = float(v.integrate(f, dim, N=N, integration_domain=torch.tensor(dom, dtype=torch.float64, device=dev),
-                            seed=seed))
-    r = ov.run(lambda x: fam.double_gaussian(x, 0.3), dim, N, dom, seed=seed)
+                            seed=seed, N_intervals=bins))
+    r = ov.run(lambda x: fam.double_gaussian(x, 0.3), dim, N, dom, seed=seed, N_intervals=bins)
+    assert v.map.N_intervals == r["map"].N
     assert v._nr_of_fevals == r["fevals"]
     assert abs(val - r["result"]) <= 1e-10 * abs(r["result"])
     # map state after the run
@@ -396,3 +399,89 @@ def test_default_bins_vegas_runs_fast(dev):
     assert v.map.N_intervals == 4000
     assert abs(val - 1.0) < 0.02
     assert time.perf_counter() - t0 < 1.0
+
+
+def test_c3_shape_iteration_and_full_run(configs, dev, parity):
+    """The VEGAS workload at its real shape: d = 6, 50 bins per axis, FP32
+    samples, N_increment = 1e6 (N = 2.5e7), so N_strat = 7 and 117,649 cubes
+    of 8 samples (941,192 evaluations per fresh iteration).
+
+    One fused iteration against the oracle on the same Philox stream:
+      * nh: bit-exact (8 per cube);
+      * map bin counts from the fused kernel's bin indices: bit-exact
+        (941,192 x 6 indices);
+      * tq_vegas_map_eval indices of the same 941,192 FP32 samples: bit-exact;
+      * per-cube JF / JF2 and the iteration's result: the kernel evaluates
+        exp(-|x - c|^2 / w^2) in FP32 with 1/w^2 = 100, so a relative error
+        of about 1e-7 in |x - c|^2 becomes up to ~1e-5 in f; the bar is
+        2e-5 relative for every cube that carries >= 1e-6 of the largest
+        JF, 1e-5 for the iteration result;
+      * get_NH of the dh that the iteration leaves: bit-exact given that dh.
+    The full run: within 3 reported standard errors of the analytic value.
+    """
+    c = configs["C3"]
+    N, seed, d = 25_000_000, 3, 6
+    f = tq.DoubleGaussian(d, width=c["width"])
+    dom = torch.tensor(c["domain"], dtype=torch.float32, device=dev)
+    v = tq.VEGAS()
+    v._setup(f, d, N, dom, seed, None, True, 0, 0, 20, None, c["N_intervals"])
+    dev_st = v._dev
+    assert v.strat.N_strat == 7 and dev_st.n_cubes == 117_649 and v._starting_N == 1_000_000
+    steps = v._steps(use_warmup=False)
+    next(steps)  # iteration 1 (fused: samples, map, integrand, histograms, per-cube sums, update)
+    torch.cuda.synchronize()
+    slot = v._slots[0][0]
+    nh = dev_st.nh.cpu().numpy()
+
+    ost = ov.Strat(1_000_000, d, np.float32)
+    om = ov.Map(c["N_intervals"], d, np.float32)
+    onh = ost.get_NH(1_000_000)
+    assert np.array_equal(nh, onh) and int(onh.sum()) == 941_192
+    total = int(onh.sum())
+    u = native.uniform(seed, 0, total, d, np.float32)
+    y = ost.get_Y(onh, u)
+    # the integrand on the inputs the kernel holds: centres rounded to FP32
+    cen = (float(np.float32(1.0 / 3.0)), float(np.float32(2.0 / 3.0)))
+    x = om.get_X(y).astype(np.float32).astype(np.float64)
+    fv = fam.double_gaussian(x, c["width"], cen).astype(np.float32).astype(np.float64)
+    jf = fv * om.get_Jac(y)
+    om.accumulate_weight(y, jf**2)
+    # bin counts of the fused kernel (left in the map state by the iteration's reduce)
+    assert np.array_equal(dev_st.counts.cpu().numpy(), om.counts.astype(np.float64))
+    np.testing.assert_allclose(dev_st.weights.cpu().numpy(), om.weights, rtol=2e-5)
+    # public map evaluation on the same samples
+    ids = v.map._get_interval_ID(torch.tensor(y, device=dev)).cpu().numpy()
+    assert np.array_equal(ids, om.ids(y))
+    JF, JF2 = ost.accumulate_weight(onh, jf)
+    got_jf, got_jf2 = dev_st.jf_cube.cpu().numpy(), dev_st.jf2_cube.cpu().numpy()
+    sig = JF >= 1e-6 * JF.max()
+    rel = np.abs(got_jf - JF)[sig] / JF[sig]
+    parity.record(float(rel.max()), 2e-5)
+    assert rel.max() <= 2e-5
+    assert np.all(np.abs(got_jf2 - JF2)[sig] <= 4e-5 * JF2[sig])
+    assert np.all(np.abs(got_jf - JF)[~sig] <= 1e-11 * JF.max())
+    ih = JF * (ost.V_cubes / onh)
+    res_dev = float(dev_st.iter_stats[slot, 0])
+    assert abs(res_dev - ih.sum()) <= 1e-5 * ih.sum()
+    # dh of the iteration, normalised as the next get_NH does; then nh from it
+    from torchquad_amd._lib import call
+
+    call("tq_vegas_normalize_dh", dev_st.struct, slot, dev_st.stream())
+    dh = dev_st.dh.cpu().numpy()
+    assert abs(dh.sum() - 1.0) < 1e-12
+    ost.JF, ost.JF2 = got_jf.copy(), got_jf2.copy()
+    ost.update_DH()
+    np.testing.assert_allclose(dh, ost.dh, rtol=1e-12, atol=1e-18)
+    ost.dh = dh
+    for n_exp in (1_000_000, 2_000_000, 3_141_593):
+        nh_dev = v.strat.get_NH(n_exp).cpu().numpy()
+        assert np.array_equal(nh_dev, ost.get_NH(n_exp)), n_exp
+    assert len(np.unique(nh_dev)) > 10  # the counts are no longer uniform
+
+    # the full run
+    v = tq.VEGAS()
+    val = float(v.integrate(f, d, N=N, integration_domain=dom, seed=seed, N_intervals=c["N_intervals"]))
+    err = float(v._get_error())
+    assert v.strat.N_strat == 7 and v.map.N_intervals == 50
+    assert 0 < err < 1e-3
+    assert abs(val - c["analytic"]) <= 3 * err, (val, c["analytic"], err)

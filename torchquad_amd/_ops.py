@@ -32,17 +32,34 @@ def grid_desc(rule, dim, n, dtype, gl=None, lane_axes=0, items_per_wave=0):
     return g
 
 
-def grid_integrate(rule, integrand, dim, n, domain, gl=None, lane_axes=0, items_per_wave=0, out_f64=None):
-    """Fused tensor-product rule for a built-in integrand; 0-dim result tensor."""
-    require_device(domain)
+def _fused_grid_desc(rule, integrand, dim, n, dtype, gl, lane_axes, items_per_wave):
     desc = integrand.descriptor(dim)
     if items_per_wave == 0 and lane_axes == 0:
         from .autotune import grid_ipw
 
-        items_per_wave = grid_ipw(rule, desc.family, dim, n, domain.dtype)
-    g = grid_desc(rule, dim, n, domain.dtype, gl, lane_axes, items_per_wave)
+        items_per_wave = grid_ipw(rule, desc.family, dim, n, dtype)
+    return grid_desc(rule, dim, n, dtype, gl, lane_axes, items_per_wave), desc
+
+
+def grid_workspace(rule, integrand, dim, n, domain, gl=None, lane_axes=0, items_per_wave=0):
+    """A new workspace tensor sized for grid_integrate with these arguments
+    (for callers that must own theirs, e.g. captured graphs)."""
+    g, _ = _fused_grid_desc(rule, integrand, dim, n, domain.dtype, gl, lane_axes, items_per_wave)
     nbytes = lib().tq_grid_workspace_size(ctypes.byref(g))
-    ws = workspace.get(max(nbytes, 8), domain.device)
+    return torch.empty(max(int(nbytes), 8), dtype=torch.uint8, device=domain.device)
+
+
+def grid_integrate(rule, integrand, dim, n, domain, gl=None, lane_axes=0, items_per_wave=0, out_f64=None, ws=None):
+    """Fused tensor-product rule for a built-in integrand; 0-dim result tensor.
+
+    ws: the caller's workspace (uint8 tensor); default the per-stream shared one."""
+    require_device(domain)
+    g, desc = _fused_grid_desc(rule, integrand, dim, n, domain.dtype, gl, lane_axes, items_per_wave)
+    nbytes = lib().tq_grid_workspace_size(ctypes.byref(g))
+    if ws is None:
+        ws = workspace.get(max(nbytes, 8), domain.device)
+    elif ws.numel() < nbytes:
+        raise ValueError(f"workspace of {ws.numel()} bytes, {nbytes} needed")
     out = torch.empty((), dtype=domain.dtype, device=domain.device)
     call(
         "tq_grid_integrate",

@@ -239,7 +239,7 @@ class GridIntegrator(BaseIntegrator):
             if entry is None:
                 entry = self._capture_step(fn, dim, N, dom, backend)
                 graphs[key] = entry
-            graph, sdom, sout, fevals = entry
+            graph, sdom, sout, fevals, _owned = entry
             sdom.copy_(dom)
             graph.replay()
             self._nr_of_fevals = fevals
@@ -248,20 +248,24 @@ class GridIntegrator(BaseIntegrator):
         return compiled_integrate
 
     def _capture_step(self, fn, dim, N, domain, backend):
-        """(graph, static domain, static result, evals) of one built-in integrand's fused step."""
+        """(graph, static domain, static result, evals, owned buffers) of one built-in integrand's
+        fused step.  The entry owns its workspace and holds its node tables: a shared workspace
+        could be regrown (and its memory freed) by a later call, and graphs replayed on different
+        streams would race on it."""
         sdom = domain.detach().clone()
-        # eager warm-up: checks, Gauss-Legendre tables, workspaces, lazy init
+        # eager warm-up: checks, Gauss-Legendre tables, lazy init
         self.integrate(fn, dim, N, sdom, backend)
         fevals = self._nr_of_fevals
         n = self._grid_n(N, dim)
         gl = self._gl_tables(n, sdom.device)
+        ws = _ops.grid_workspace(self._tq_rule, fn, dim, n, sdom, gl)
         torch.cuda.synchronize(sdom.device)
         graph = torch.cuda.CUDAGraph()
         with warnings.catch_warnings():
             warnings.simplefilter("ignore")
             with torch.cuda.graph(graph):
-                sout = _ops.grid_integrate(self._tq_rule, fn, dim, n, sdom, gl)
-        return (graph, sdom, sout, fevals)
+                sout = _ops.grid_integrate(self._tq_rule, fn, dim, n, sdom, gl, ws=ws)
+        return (graph, sdom, sout, fevals, (ws, gl))
 
 
 def _composite_by_kernel(rule, cur_dim_areas, dim, hs, domain):
