@@ -96,6 +96,20 @@ int tq_grid_integrate(const tq_grid_desc* g, const tq_integrand* f, const void* 
                       void* workspace, size_t workspace_bytes, void* out, double* out_f64,
                       void* stream);
 
+/* tq_grid_integrate without a workspace, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The block
+ * partials live in a static device array of 1 MiB per device, each device's
+ * address looked up once on that device's first call; the call allocates
+ * nothing.  Grids that need more than 1 MiB of partials (more than 131072
+ * workgroups, i.e. tq_grid_workspace_size() > 1 MiB) return TQ_E_WORKSPACE.
+ * Results equal tq_grid_integrate's with a workspace of
+ * tq_grid_workspace_size() bytes.  Every call on a device shares that array:
+ * calls that overlap on one device (different streams, or host threads)
+ * race and must be serialised by the caller. */
+int tq_grid_integrate_noalloc(const tq_grid_desc* g, const tq_integrand* f, const void* domain, void* out,
+                              double* out_f64, void* stream);
+
 /* Point writer for user integrands: writes the [n^dim, dim] grid in ij
  * order (last axis fastest), as IntegrationGrid.points
  * (integration_grid.py:98-99). */

@@ -316,3 +316,24 @@ def test_vegas_driver_host_logic_matches_oracle(case):
     # not a fifth iteration: never stops, nothing changes
     v = _driver(r, s, 4, **kw)
     assert v._check_abort_conditions() is False and v._starting_N == args["sN"]
+
+
+def test_noalloc_rejects_grids_beyond_its_scratch():
+    """tq_grid_integrate_noalloc refuses, before any device call, a grid whose
+    block partials exceed its 1 MiB static scratch."""
+    L = _lib.load_library()
+    g = _lib.GridDesc()
+    g.dtype, g.rule, g.dim, g.n, g.items_per_wave = 0, 0, 6, 1001, 1
+    need = L.tq_grid_workspace_size(ctypes.byref(g))
+    assert need > (1 << 20)
+    f = _lib.Integrand()
+    f.family, f.dim, f.c0 = 0, 6, 1.0
+    fake = ctypes.c_void_p(256)
+    rc = L.tq_grid_integrate_noalloc(ctypes.byref(g), ctypes.byref(f), fake, fake, None, None)
+    assert rc == -3
+    assert b"scratch" in L.tq_last_error()
+    g.n, g.dim, g.items_per_wave = 9, 3, 0
+    assert L.tq_grid_workspace_size(ctypes.byref(g)) <= (1 << 20)
+    g.rule = 2  # Boole needs n = 1 + 4k
+    g.n = 10
+    assert L.tq_grid_integrate_noalloc(ctypes.byref(g), ctypes.byref(f), fake, fake, None, None) == -1

@@ -123,6 +123,10 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.POINTER(GridDesc), ctypes.POINTER(Integrand), _P, _P, _SZ, _P, _P, _P],
     ),
+    "tq_grid_integrate_noalloc": (
+        ctypes.c_int,
+        [ctypes.POINTER(GridDesc), ctypes.POINTER(Integrand), _P, _P, _P, _P],
+    ),
     "tq_grid_points": (ctypes.c_int, [ctypes.POINTER(GridDesc), _P, _P, _P]),
     "tq_grid_h": (ctypes.c_int, [ctypes.POINTER(GridDesc), _P, _P, _P]),
     "tq_grid_reduce_workspace_size": (_SZ, [ctypes.POINTER(GridDesc), _I64]),

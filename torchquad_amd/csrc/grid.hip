@@ -13,6 +13,8 @@
 // and one FMA into an FP32 partial that is flushed to FP64 at least every 128 points.
 // Block partials are FP64 and the final sum runs in a fixed order, so the
 // result is deterministic.
+#include <mutex>
+
 #include "tq_host.hpp"
 #include "tq_grid_tile.hpp"
 
@@ -536,6 +538,46 @@ static void points_backward_shape(int d, long long P, int* bd, long long* nb) {
     *nb = b;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free entry point: the block partials live in a file-scope
+// device array instead of a caller workspace.  Each device holds its own copy
+// of the array; host code reaches it only through the address that
+// hipGetSymbolAddress returns with that device current (the array's name is
+// a host-side stand-in, not a device address).  Every block writes its
+// partial before the finalize reads it, so nothing needs to start at zero.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = (size_t)1 << 20;  // 131072 block partials
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array, looked up once per
+// device (with that device current) and kept.
+static int current_scratch(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_error("hipGetDevice: %s", hipGetErrorString(e));
+        return (int)e;
+    }
+    if (dev < 0 || dev >= kMaxDevices) return fail(TQ_E_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+            return e != hipSuccess ? (int)e : TQ_E_UNSUPPORTED;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TQ_OK;
+}
+
 }  // namespace tq
 
 using namespace tq;
@@ -563,6 +605,34 @@ int tq_grid_integrate(const tq_grid_desc* g, const tq_integrand* f, const void* 
     if (g->dtype == TQ_F32)
         return launch_grid<float>(g, f, domain, workspace, workspace_bytes, out, out_f64, as_stream(stream));
     return launch_grid<double>(g, f, domain, workspace, workspace_bytes, out, out_f64, as_stream(stream));
+}
+
+int tq_grid_integrate_noalloc(const tq_grid_desc* g, const tq_integrand* f, const void* domain, void* out,
+                              double* out_f64, void* stream) {
+    int rc = check_grid_desc(g, TQ_MAX_DIM);
+    if (rc) return rc;
+    rc = check_gl_tables(g);
+    if (rc) return rc;
+    rc = check_integrand(f, g->dim);
+    if (rc) return rc;
+    if (!domain) return fail(TQ_E_NULL, "domain is NULL");
+    if (!out && !out_f64) return fail(TQ_E_NULL, "no output pointer");
+    // the same partial count as a workspace of tq_grid_workspace_size bytes,
+    // so both entry points launch the same geometry
+    const size_t need = tq_grid_workspace_size(g);
+    if (need == 0) return fail(TQ_E_ARG, "invalid grid geometry");
+    if (need > kNoAllocScratchBytes) {
+        set_error("tq_grid_integrate_noalloc: %zu bytes of block partials exceed its %zu-byte scratch; use "
+                  "tq_grid_integrate with a workspace",
+                  need, kNoAllocScratchBytes);
+        return TQ_E_WORKSPACE;
+    }
+    void* ws = nullptr;
+    rc = current_scratch(&ws);
+    if (rc) return rc;
+    if (g->dtype == TQ_F32)
+        return launch_grid<float>(g, f, domain, ws, need, out, out_f64, as_stream(stream));
+    return launch_grid<double>(g, f, domain, ws, need, out, out_f64, as_stream(stream));
 }
 
 int tq_grid_points(const tq_grid_desc* g, const void* domain, void* points, void* stream) {
