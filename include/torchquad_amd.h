@@ -173,6 +173,14 @@ size_t tq_mc_workspace_size(const tq_mc_desc* m);
 int tq_mc_integrate(const tq_mc_desc* m, const tq_integrand* f, const void* domain,
                     void* workspace, size_t workspace_bytes, void* out, double* stats,
                     void* stream);
+/* tq_mc_integrate with the first Philox counter read on the device from
+ * *offset_dev (m->offset is ignored), so that a captured HIP graph of the
+ * call draws the samples of whatever stream position the caller stores there
+ * before each replay: MonteCarlo.get_jit_compiled_integrate's compiled
+ * function draws new points on every call (monte_carlo.py:108-225). */
+int tq_mc_integrate_dev_offset(const tq_mc_desc* m, const tq_integrand* f, const void* domain,
+                               const uint64_t* offset_dev, void* workspace, size_t workspace_bytes, void* out,
+                               double* stats, void* stream);
 /* Sample writer for user integrands: points[N, dim] = a + U*(b-a), the same
  * stream that tq_mc_integrate draws. */
 int tq_mc_points(const tq_mc_desc* m, const void* domain, void* points, void* stream);

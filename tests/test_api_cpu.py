@@ -1,5 +1,6 @@
 """Host logic and ABI checks that need no GPU."""
 import ctypes
+import math
 import re
 import warnings
 from pathlib import Path
@@ -337,3 +338,79 @@ def test_noalloc_rejects_grids_beyond_its_scratch():
     g.rule = 2  # Boole needs n = 1 + 4k
     g.n = 10
     assert L.tq_grid_integrate_noalloc(ctypes.byref(g), ctypes.byref(f), fake, fake, None, None) == -1
+
+
+def test_gaussian_rule_classification():
+    """Which Gaussian rules run on the kernels: hooks classified on probes
+    (gaussian.py:13-25, tutorial.rst:1198-1236)."""
+    import scipy.special
+
+    from torchquad_amd.integration import gaussian as gm
+
+    assert tq.GaussLegendre()._tq_mode() == gm.AFFINE
+    assert tq.GaussHermite()._tq_mode() == gm.UNIT
+    assert tq.Gaussian()._tq_mode() is None  # identity map with the (b-a)/2 scale
+
+    class UserHermite(tq.Gaussian):  # the tutorial's subclass, verbatim hooks
+        def __init__(self):
+            super().__init__()
+            self._root_fn = scipy.special.roots_hermite
+
+        @staticmethod
+        def _apply_composite_rule(cur_dim_areas, dim, hs, domain):
+            for _ in range(dim):
+                cur_dim_areas = torch.sum(cur_dim_areas, dim=len(cur_dim_areas.shape) - 1)
+            return cur_dim_areas
+
+    class Lobatto(tq.GaussLegendre):  # another root function, Gauss-Legendre hooks
+        def __init__(self):
+            super().__init__()
+            self._root_fn = lambda n: np.polynomial.legendre.leggauss(n)
+
+    class Doubled(tq.GaussLegendre):  # a composite rule the kernels do not implement
+        @staticmethod
+        def _apply_composite_rule(cur_dim_areas, dim, hs, domain):
+            return 2.0 * torch.sum(cur_dim_areas.reshape(*cur_dim_areas.shape[:-dim], -1), dim=-1)
+
+    class OwnRoots(tq.GaussLegendre):
+        def _roots(self, N, backend, requires_grad=False):
+            return super()._roots(N, backend, requires_grad)
+
+    class Broken(tq.GaussLegendre):  # a hook that fails on the probes runs its own code
+        def _resize_roots(self, integration_domain, roots):
+            raise RuntimeError("only on the GPU")
+
+    assert UserHermite()._tq_mode() == gm.UNIT
+    assert Lobatto()._tq_mode() == gm.AFFINE
+    assert Doubled()._tq_mode() is None
+    assert OwnRoots()._tq_mode() is None
+    assert Broken()._tq_mode() is None
+    unit = UserHermite()._tq_domain(torch.tensor([[0.0, 3.0], [2.0, 5.0]], dtype=torch.float64))
+    assert unit.tolist() == [[-1.0, 1.0], [-1.0, 1.0]] and unit.dtype == torch.float64
+
+
+def test_param_batch_integrands_host_side():
+    """Parameter batches: shapes, descriptors and the [N, *S] torch definition."""
+    a = np.arange(1.0, 7.0).reshape(3, 2)
+    f = tq.GaussianPeak(a, [0.1, 0.2])
+    assert f.batch_shape == (3,) and f.batch_size == 3 and f.dim == 2
+    ds = f.descriptors()
+    assert [d.a[1] for d in ds] == [2.0, 4.0, 6.0] and all(d.u[0] == 0.1 for d in ds)
+    with pytest.raises(ValueError, match="parameter batch"):
+        f.descriptor()
+    x = torch.rand(11, 2, dtype=torch.float64)
+    y = f(x)
+    assert y.shape == (11, 3)
+    for s in range(3):
+        ref = tq.GaussianPeak(a[s], [0.1, 0.2])(x)
+        assert torch.allclose(y[:, s], ref, rtol=1e-15, atol=0)
+    g = tq.Oscillatory(np.ones((2, 3, 1)), w=np.array([[0.1, 0.2, 0.3], [0.4, 0.5, 0.6]]), dim=1)
+    assert g.batch_shape == (2, 3) and g(torch.rand(5, 1)).shape == (5, 2, 3)
+    assert [d.c1 for d in g.descriptors()] == [0.1, 0.2, 0.3, 0.4, 0.5, 0.6]
+    h = tq.DoubleGaussian(3, width=[0.1, 0.2])
+    ds = h.descriptors()
+    assert ds[1].c3 == pytest.approx(25.0) and ds[0].c0 == pytest.approx(0.5 * (0.1 * math.sqrt(math.pi)) ** -3)
+    assert tq.GaussianPeak([1.0, 2.0], [0.0, 0.0]).batch_shape == ()
+    assert f.key() != tq.GaussianPeak(a + 1e-9, [0.1, 0.2]).key()
+    with pytest.raises(ValueError, match="broadcast"):
+        tq.GaussianPeak(np.ones((3, 2)), np.zeros((4, 2)))

@@ -45,7 +45,7 @@ def scale_of(it, n, ref):
     integral of |f| by a 2e6-point trapezoid sum (a scale, not a reference)."""
     if it["family"] != "oscillatory":
         return abs(ref)
-    rule = tq.batch.RULE_NAMES[it["rule"]]
+    rule = it["rule"]
     if rule == "gauss_legendre" and n > 1000:
         x = np.linspace(0.0, 1.0, 2_000_001)
         y = np.abs(np.cos(2 * np.pi * it["w"] + it["a"][0] * x))

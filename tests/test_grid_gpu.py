@@ -351,7 +351,7 @@ def test_gauss_hermite_rule(dev):
     ref = math.sqrt(math.pi) * math.exp(1.0 / 16.0)
     assert abs(float(gh.integrate(tq.ExpLinear(1, [0.5]), 1, N=40, integration_domain=dom1)) - ref) <= 1e-12 * ref
 
-    class UserHermite(tq.Gaussian):  # the tutorial's subclass, run on the reference layout
+    class UserHermite(tq.Gaussian):  # the tutorial's subclass: classified, then run on the kernels
         def __init__(self):
             super().__init__()
             import scipy.special
@@ -364,10 +364,58 @@ def test_gauss_hermite_rule(dev):
                 cur_dim_areas = torch.sum(cur_dim_areas, dim=len(cur_dim_areas.shape) - 1)
             return cur_dim_areas
 
+    from torchquad_amd.integration.gaussian import UNIT
+
+    uh = UserHermite()
+    assert uh._tq_mode() == UNIT
     f = tq.GaussianPeak([0.7, 1.3], [0.2, -0.1])
     a = float(gh.integrate(f, 2, N=24**2, integration_domain=dom2))
-    b = float(UserHermite().integrate(f, 2, N=24**2, integration_domain=dom2))
-    assert abs(a - b) <= 1e-12 * abs(b)
+    b = float(uh.integrate(f, 2, N=24**2, integration_domain=dom2))
+    # same tables, same kernels: identical
+    assert a == b
+    # against the rule's own FP64 sum on the host (the reference layout's arithmetic)
+    import scipy.special
+
+    r, w = scipy.special.roots_hermite(24)
+    X, Y = np.meshgrid(r, r, indexing="ij")
+    ref = float(np.sum(np.outer(w, w) * np.exp(-(0.7 * (X - 0.2)) ** 2 - (1.3 * (Y + 0.1)) ** 2)))
+    assert abs(b - ref) <= 1e-12 * abs(ref)
+    # user callables on the point/reduce kernels, the domain's values ignored as in the reference
+    c = float(uh.integrate(lambda x: f(x), 2, N=24**2, integration_domain=dom2 * 3.0))
+    assert abs(c - ref) <= 1e-12 * abs(ref)
+    # the split API: points are the unresized roots, the result a plain sum
+    pts, hs, n = uh.calculate_grid(24**2, dom2 * 5.0)
+    assert n == 24 and abs(float(pts[0, 0]) - r[0]) <= 1e-15
+    vals, _ = uh.evaluate_integrand(lambda x: f(x), pts, weights=uh._weights(n, 2, "torch"))
+    assert abs(float(uh.calculate_result(vals, 2, n, hs, dom2 * 5.0)) - ref) <= 1e-12 * abs(ref)
+
+
+def test_custom_root_fn_rule_fused(dev, parity):
+    """A rule that only swaps _root_fn (gaussian.py:13-25): its roots and
+    weights run on the fused kernels with Gauss-Legendre's map and scale.
+    Here the nodes come from scipy.special.roots_legendre instead of numpy's
+    leggauss; against the FP64 oracle, and through the compiled graph."""
+    import scipy.special
+
+    class ScipyLegendre(tq.GaussLegendre):
+        def __init__(self):
+            super().__init__()
+            self._root_fn = scipy.special.roots_legendre
+
+    from torchquad_amd.integration.gaussian import AFFINE
+
+    g = ScipyLegendre()
+    assert g._tq_mode() == AFFINE
+    f = tq.GaussianPeak([2.0, 3.0, 1.5], [0.1, -0.2, 0.3])
+    dom = [[-1.0, 0.5], [0.0, 1.0], [-0.5, 0.7]]
+    for dtype in (torch.float32, torch.float64):
+        val = float(g.integrate(f, 3, N=20**3, integration_domain=torch.tensor(dom, dtype=dtype, device=dev)))
+        ref, scale = oracle_value("gauss_legendre", "gaussian_peak", 3, 20, as_seen(dom, dtype), dict(a=f.a, u=f.u))
+        check(val, ref, dtype, scale, parity)
+    # graph-replayed compiled integrate on the same custom rule
+    comp = g.get_jit_compiled_integrate(3, N=20**3, integration_domain=dom)
+    d32 = torch.tensor(dom, dtype=torch.float32, device=dev)
+    assert float(comp(f, d32)) == float(g.integrate(f, 3, N=20**3, integration_domain=d32))
 
 
 SPLIT_N = {
@@ -466,3 +514,63 @@ def test_compiled_graphs_own_their_workspaces(dev, parity):
     again = float(comp(f1, d32))
     assert again == first
     check(again, ref, torch.float32, scale, parity)
+
+
+@pytest.mark.parametrize("S", [(1,), (7,), (200,), (2, 3)])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_param_batch_fused_vs_oracle(S, dtype, dev, parity):
+    """Built-in families holding a parameter batch S (the reference's [N, *S]
+    vectorised integrands, base_integrator.py:77-89, tutorial.rst:741-774):
+    one batched launch over the grid, every parameter set against the FP64
+    oracle (error <= rtol * sum |w f|), the result shaped [*S] (a [N, 1]
+    batch is squeezed like a scalar integrand), and equal to the same object
+    called as a torch [N, *S] integrand on the point/reduce path."""
+    rng = np.random.default_rng(seed_of("param_batch", S, dtype))
+    cases = [
+        # (rule, dim, N, family, oracle family, parameter builder)
+        ("simpson", 1, 10001, "gaussian_peak"),
+        ("boole", 3, 17**3, "product_peak"),
+        ("gauss_legendre", 2, 24**2, "oscillatory"),
+        ("trapezoid", 2, 33**2, "exp_linear"),
+    ]
+    for rule, d, N, fname in cases:
+        a = rng.uniform(1.0, 5.0, S + (d,))
+        u = rng.uniform(-0.5, 0.5, S + (d,))
+        w = rng.uniform(0.0, 1.0, S)
+        if fname == "gaussian_peak":
+            f = tq.GaussianPeak(a, u)
+            kw = [dict(a=a.reshape(-1, d)[i], u=u.reshape(-1, d)[i]) for i in range(int(np.prod(S)))]
+        elif fname == "product_peak":
+            f = tq.ProductPeak(a * 0.4, u)
+            kw = [dict(a=0.4 * a.reshape(-1, d)[i], u=u.reshape(-1, d)[i]) for i in range(int(np.prod(S)))]
+        elif fname == "oscillatory":
+            f = tq.Oscillatory(a, w)
+            kw = [dict(a=a.reshape(-1, d)[i], c1=float(w.reshape(-1)[i])) for i in range(int(np.prod(S)))]
+        else:
+            f = tq.ExpLinear(d, a=a * 0.3, c=w)
+            kw = [dict(a=0.3 * a.reshape(-1, d)[i], c1=float(w.reshape(-1)[i])) for i in range(int(np.prod(S)))]
+        assert f.batch_shape == S
+        dom = [[-1.0, 0.5 + 0.1 * k] for k in range(d)]
+        domain = torch.tensor(dom, dtype=dtype, device=dev)
+        integ = RULES[rule]()
+        val = integ.integrate(f, d, N=N, integration_domain=domain)
+        n = integ._grid_n(N, d)
+        assert val.dtype == dtype
+        assert val.shape == (() if S == (1,) else S)
+        flat = val.reshape(-1).cpu().numpy()
+        for i in range(flat.size):
+            ref, scale = oracle_value(rule, fname, d, n, as_seen(dom, dtype), kw[i])
+            check(float(flat[i]), ref, dtype, scale, parity)
+        # the same object as a plain [N, *S] torch integrand (point writer -> fn -> reduce).
+        # In FP32 that path scales Newton-Cotes sums by the reference's h = x_1 - x_0 of the
+        # FP32 grid (grid_integrator.py:57-91), whose rounding (up to 4e-4 relative at
+        # n = 10001) the fused kernels, with h = (b - a)/(n - 1) in FP64, do not have: the
+        # values are compared in FP64.
+        user = RULES[rule]().integrate(lambda x: f(x), d, N=N, integration_domain=domain)
+        assert user.shape == val.shape
+        if dtype == torch.float64:
+            np.testing.assert_allclose(user.reshape(-1).cpu().numpy(), flat, rtol=1e-12,
+                                       atol=1e-12 * float(np.max(np.abs(flat))))
+        # a repeated call reuses the plan and gives the same values
+        again = integ.integrate(f, d, N=N, integration_domain=domain)
+        assert torch.equal(again, val)

@@ -150,6 +150,10 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.POINTER(McDesc), ctypes.POINTER(Integrand), _P, _P, _SZ, _P, _P, _P],
     ),
+    "tq_mc_integrate_dev_offset": (
+        ctypes.c_int,
+        [ctypes.POINTER(McDesc), ctypes.POINTER(Integrand), _P, _P, _P, _SZ, _P, _P, _P],
+    ),
     "tq_mc_points": (ctypes.c_int, [ctypes.POINTER(McDesc), _P, _P, _P]),
     "tq_mc_reduce_workspace_size": (_SZ, [_I64, _I64]),
     "tq_mc_reduce": (ctypes.c_int, [_I32, _P, _I64, _I64, _P, _I32, _P, _SZ, _P, _P]),

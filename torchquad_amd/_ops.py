@@ -179,24 +179,35 @@ def mc_desc(dim, N, dtype, seed, offset, samples_per_thread=0):
     return m
 
 
-def mc_integrate(integrand, dim, N, domain, seed, offset=0, stats=None, samples_per_thread=0):
+def mc_workspace(dim, N, domain):
+    """A new workspace tensor for mc_integrate (for callers that own theirs)."""
+    m = mc_desc(dim, N, domain.dtype, 0, 0)
+    return torch.empty(max(int(lib().tq_mc_workspace_size(ctypes.byref(m))), 16), dtype=torch.uint8,
+                       device=domain.device)
+
+
+def mc_integrate(integrand, dim, N, domain, seed, offset=0, stats=None, samples_per_thread=0, desc=None,
+                 offset_dev=None, ws=None, out=None):
+    """Fused Monte Carlo; 0-dim result.  desc: a C descriptor instead of
+    integrand's; offset_dev: int64 device tensor holding the first Philox
+    counter (then `offset` is ignored); ws / out: caller-owned buffers."""
     require_device(domain)
     m = mc_desc(dim, N, domain.dtype, seed, offset, samples_per_thread)
-    desc = integrand.descriptor(dim)
+    if desc is None:
+        desc = integrand.descriptor(dim)
     nbytes = lib().tq_mc_workspace_size(ctypes.byref(m))
-    ws = workspace.get(max(nbytes, 16), domain.device, tag="mc")
-    out = torch.empty((), dtype=domain.dtype, device=domain.device)
-    call(
-        "tq_mc_integrate",
-        ctypes.byref(m),
-        ctypes.byref(desc),
-        ptr(domain),
-        ptr(ws),
-        ws.numel(),
-        ptr(out),
-        ptr(stats),
-        stream_ptr(domain.device),
-    )
+    if ws is None:
+        ws = workspace.get(max(nbytes, 16), domain.device, tag="mc")
+    elif ws.numel() < nbytes:
+        raise ValueError(f"workspace of {ws.numel()} bytes, {nbytes} needed")
+    if out is None:
+        out = torch.empty((), dtype=domain.dtype, device=domain.device)
+    args = (ctypes.byref(m), ctypes.byref(desc), ptr(domain))
+    tail = (ptr(ws), ws.numel(), ptr(out), ptr(stats), stream_ptr(domain.device))
+    if offset_dev is not None:
+        call("tq_mc_integrate_dev_offset", *args, ptr(offset_dev), *tail)
+    else:
+        call("tq_mc_integrate", *args, *tail)
     return out
 
 

@@ -80,9 +80,12 @@ class Batch:
         dtype: torch.float32 or torch.float64 (points and integrand).
         tile_points: points per tile (None: autotuned / default).
         blocks: persistent workgroups (None: autotuned / default).
+        gl_tables: callable (n, device) -> (nodes, weights), FP64 device
+            tensors on [-1, 1], for the Gaussian items (default: the
+            Gauss-Legendre generator); a custom Gaussian rule's tables.
     """
 
-    def __init__(self, items, dtype=torch.float32, device=None, tile_points=None, blocks=None):
+    def __init__(self, items, dtype=torch.float32, device=None, tile_points=None, blocks=None, gl_tables=None):
         if not torch.cuda.is_available():
             raise _lib.NativeUnavailableError("the batched form runs on a ROCm GPU only")
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
@@ -100,7 +103,7 @@ class Batch:
             rule = it["rule"]
             rule = RULE_NAMES[rule.lower()] if isinstance(rule, str) else int(rule)
             fn = it["fn"]
-            if not isinstance(fn, BuiltinIntegrand):
+            if not isinstance(fn, (BuiltinIntegrand, _OneSet)):
                 raise TypeError("batched integrals need built-in integrands (torchquad_amd.integrands)")
             dim = int(it["dim"])
             n = int(it["n"]) if "n" in it else n_per_axis(rule, dim, int(it["N"]))
@@ -133,7 +136,7 @@ class Batch:
             for _, n in gl_ns:
                 if n not in offsets:
                     offsets[n] = off
-                    x, w = _gl_table(n, self.device)
+                    x, w = (gl_tables or _gl_table)(n, self.device)
                     parts_x.append(x)
                     parts_w.append(w)
                     off += n
@@ -201,6 +204,52 @@ class Batch:
         with torch.cuda.graph(g):
             self.run()
         return g
+
+
+class ParamBatchRunner:
+    """Integrals of a built-in family's parameter batch over one grid.
+
+    Each of the prod(S) parameter sets of ``fn`` (``fn.batch_shape == S``)
+    is one integral of a :class:`Batch` with the same rule, n, dimension and
+    domain; plans are cached per (rule, n, parameters, domain, dtype,
+    device), so a repeated call is one launch pair and no host work beyond
+    it.  The reference's counterpart is a vectorised integrand returning
+    ``[N, *S]`` (base_integrator.py:77-89).
+    """
+
+    def __init__(self, max_plans=16):
+        self._plans = {}
+        self._max = int(max_plans)
+
+    def run(self, rule, fn, dim, n, domain, domain_host, gl_tables=None, tables_key=None):
+        """[*S] integrals (domain dtype); domain_host: float64 numpy [dim, 2];
+        tables_key: identifies what gl_tables returns (part of the plan key)."""
+        key = (int(rule), int(n), fn.key(), np.ascontiguousarray(domain_host).tobytes(), domain.dtype,
+               str(domain.device), tables_key)
+        b = self._plans.get(key)
+        if b is None:
+            items = [{"rule": int(rule), "fn": f1, "dim": int(dim), "n": int(n), "domain": domain_host}
+                     for f1 in _split(fn)]
+            b = Batch(items, dtype=domain.dtype, device=domain.device, gl_tables=gl_tables)
+            if len(self._plans) >= self._max:
+                self._plans.pop(next(iter(self._plans)))
+            self._plans[key] = b
+        out, _ = b.run()
+        return out.clone().reshape(fn.batch_shape)
+
+
+class _OneSet:
+    """One parameter set of a batched family, as a scalar descriptor source."""
+
+    def __init__(self, desc):
+        self._desc = desc
+
+    def descriptor(self, dim=None):
+        return self._desc
+
+
+def _split(fn):
+    return [_OneSet(d) for d in fn.descriptors(fn.dim)]
 
 
 def integrate_batch(items, dtype=torch.float32, device=None, tile_points=None, blocks=None):

@@ -13,6 +13,7 @@ struct McArgs {
     long long N;
     long long spt;  // samples per thread
     uint64_t seed, offset;
+    const uint64_t* offset_dev;  // when set, the first counter is read from here
     const T* domain;
     double* partials;  // [blocks][2]
     Params<T> p;
@@ -35,6 +36,7 @@ __global__ __launch_bounds__(kBlock) void k_mc(McArgs<T> A) {
             sz[k] = T(0);
         }
     }
+    const uint64_t base = A.offset_dev ? *A.offset_dev : A.offset;
     const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long s0 = gt * A.spt;
     const long long s1 = (s0 + A.spt < A.N) ? (s0 + A.spt) : A.N;
@@ -44,7 +46,7 @@ __global__ __launch_bounds__(kBlock) void k_mc(McArgs<T> A) {
         T fa = T(0), fa2 = T(0);
         for (long long s = sb; s < se; ++s) {
             T x[MAXD];
-            Uniform<T>::template draw<MAXD>(A.seed, A.offset + (uint64_t)s, d, x);
+            Uniform<T>::template draw<MAXD>(A.seed, base + (uint64_t)s, d, x);
 #pragma unroll
             for (int k = 0; k < MAXD; ++k)
                 if (k < d) x[k] = fma(x[k], sz[k], lo[k]);
@@ -223,7 +225,7 @@ static void colsum_shape(long long N, long long S, long long* nb, int* bd, long 
 
 template <typename T>
 static int launch_mc(const tq_mc_desc* m, const tq_integrand* f, const void* domain, void* ws, size_t wsb,
-                     void* out, double* stats, hipStream_t st) {
+                     void* out, double* stats, hipStream_t st, const uint64_t* offset_dev = nullptr) {
     const long long nb = mc_blocks(m);
     if (!ws || wsb < (size_t)nb * 2 * sizeof(double)) return fail(TQ_E_WORKSPACE, "mc workspace too small");
     McArgs<T> A;
@@ -232,6 +234,7 @@ static int launch_mc(const tq_mc_desc* m, const tq_integrand* f, const void* dom
     A.spt = mc_spt(m);
     A.seed = m->seed;
     A.offset = m->offset;
+    A.offset_dev = offset_dev;
     A.domain = (const T*)domain;
     A.partials = (double*)ws;
     A.p = make_params<T>(*f);
@@ -293,6 +296,20 @@ int tq_mc_integrate(const tq_mc_desc* m, const tq_integrand* f, const void* doma
     if (m->dtype == TQ_F32)
         return launch_mc<float>(m, f, domain, workspace, workspace_bytes, out, stats, as_stream(stream));
     return launch_mc<double>(m, f, domain, workspace, workspace_bytes, out, stats, as_stream(stream));
+}
+
+int tq_mc_integrate_dev_offset(const tq_mc_desc* m, const tq_integrand* f, const void* domain,
+                               const uint64_t* offset_dev, void* workspace, size_t workspace_bytes, void* out,
+                               double* stats, void* stream) {
+    int rc = check_mc(m);
+    if (rc) return rc;
+    rc = check_integrand(f, m->dim);
+    if (rc) return rc;
+    if (!domain || !offset_dev) return fail(TQ_E_NULL, "domain/offset_dev is NULL");
+    if (!out && !stats) return fail(TQ_E_NULL, "no output pointer");
+    if (m->dtype == TQ_F32)
+        return launch_mc<float>(m, f, domain, workspace, workspace_bytes, out, stats, as_stream(stream), offset_dev);
+    return launch_mc<double>(m, f, domain, workspace, workspace_bytes, out, stats, as_stream(stream), offset_dev);
 }
 
 int tq_mc_points(const tq_mc_desc* m, const void* domain, void* points, void* stream) {

@@ -6,7 +6,7 @@ callable on a ``[N, dim]`` tensor (its mathematical definition in torch), so
 it can be handed to any code that expects a torchquad integrand, and the
 generic user-integrand path can be compared against the fused one.
 
-Families (parameters are per-axis vectors of length ``dim``):
+Families (per-axis parameters are vectors of length ``dim``):
 
 * ``GaussianPeak(a, u)``      f = exp(-sum a_k^2 (x_k - u_k)^2)       (Genz "Gaussian")
 * ``Oscillatory(c, w)``       f = cos(2 pi w + sum c_k x_k)           (Genz "oscillatory")
@@ -16,6 +16,14 @@ Families (parameters are per-axis vectors of length ``dim``):
 * ``DoubleGaussian(width, centers)``
   f = 1/2 sum_{c in centers} (width sqrt(pi))^-dim exp(-sum (x_k - c)^2 / width^2)
 * ``Constant(value)``         f = value
+
+Parameter batches.  Any parameter may carry leading batch dimensions S:
+per-axis parameters of shape ``[*S, dim]``, scalar ones of shape ``[*S]``
+(all broadcast together).  Such an object is the vectorised integrand of the
+reference (base_integrator.py:77-89, tutorial.rst:741-774): called on
+``[N, dim]`` points it returns ``[N, *S]``, and an integrator returns the
+``[*S]`` integrals.  On the fused path each of the prod(S) parameter sets is
+one integral of a single batched launch over the same grid.
 """
 import math
 
@@ -25,21 +33,39 @@ import torch
 from . import _lib
 
 
-def _vec(v, dim, default):
+def _array(v, default):
     if v is None:
-        return np.full(dim, float(default), dtype=np.float64)
+        return np.asarray(float(default), dtype=np.float64)
     if isinstance(v, torch.Tensor):
         v = v.detach().to("cpu", torch.float64).numpy()
-    arr = np.asarray(v, dtype=np.float64).reshape(-1)
-    if arr.size == 1 and dim > 1:
-        arr = np.full(dim, float(arr[0]))
-    if arr.size != dim:
-        raise ValueError(f"parameter has {arr.size} entries, expected dim={dim}")
+    return np.asarray(v, dtype=np.float64)
+
+
+def _axis_param(v, dim, default, name):
+    """[*S, dim] array of a per-axis parameter (a scalar or a length-1 last
+    axis is repeated over the axes)."""
+    arr = _array(v, default)
+    if arr.ndim == 0:
+        return np.full(dim, float(arr))
+    if arr.shape[-1] == 1 and dim > 1:
+        arr = np.repeat(arr, dim, axis=-1)
+    if arr.shape[-1] != dim:
+        raise ValueError(f"parameter {name} has {arr.shape[-1]} entries per set, expected dim={dim}")
     return arr
 
 
+def _dim_of(v):
+    arr = _array(v, 1.0)
+    return 1 if arr.ndim == 0 else int(arr.shape[-1])
+
+
 class BuiltinIntegrand:
-    """Base class: a family id, a dimension and float64 parameters."""
+    """Base class: a family id, a dimension and float64 parameters.
+
+    Subclasses register their parameters with ``_set_params``: per-axis ones
+    (``[*S, dim]``) and scalar ones (``[*S]``); ``batch_shape`` is the
+    broadcast S (``()`` for a single integrand).
+    """
 
     family = None
     name = "builtin"
@@ -48,24 +74,79 @@ class BuiltinIntegrand:
         self.dim = int(dim)
         if self.dim < 1:
             raise ValueError("dim must be >= 1")
-        self.scale = float(scale)
+        self._axis_names = ()
+        self._scalar_names = ("scale",)
+        self.scale = _array(scale, 1.0)
+        self.batch_shape = tuple(self.scale.shape)
 
-    # -- fused-kernel descriptor ------------------------------------------
-    def _fill(self, d):  # pragma: no cover - overridden
+    def _set_params(self, axis=None, scalar=None):
+        axis = dict(axis or {})
+        scalar = dict(scalar or {})
+        scalar.setdefault("scale", self.scale)
+        shapes = [np.shape(v)[:-1] for v in axis.values()] + [np.shape(v) for v in scalar.values()]
+        try:
+            S = tuple(np.broadcast_shapes(*shapes)) if shapes else ()
+        except ValueError as e:
+            raise ValueError(f"{self.name}: parameter batch shapes {shapes} do not broadcast") from e
+        for k, v in axis.items():
+            setattr(self, k, np.array(np.broadcast_to(v, S + (self.dim,))))
+        for k, v in scalar.items():
+            v = np.broadcast_to(v, S)
+            setattr(self, k, np.array(v) if S else float(v))
+        self._axis_names = tuple(axis)
+        self._scalar_names = tuple(scalar)
+        self.batch_shape = S
+
+    @property
+    def batch_size(self):
+        return int(np.prod(self.batch_shape)) if self.batch_shape else 1
+
+    # -- fused-kernel descriptors ------------------------------------------
+    def _fill(self, d, i):  # pragma: no cover - overridden
         raise NotImplementedError
 
-    def descriptor(self, dim=None):
+    def _ax(self, name, i):
+        v = getattr(self, name)
+        return v if not self.batch_shape else v.reshape(-1, self.dim)[i]
+
+    def _sc(self, name, i):
+        v = getattr(self, name)
+        return float(v) if not self.batch_shape else float(np.reshape(v, -1)[i])
+
+    def _check_dim(self, dim):
         dim = self.dim if dim is None else int(dim)
         if dim != self.dim:
             raise ValueError(f"{self.name} was built for dim={self.dim}, called with dim={dim}")
         if dim > _lib.TQ_MAX_DIM:
             raise ValueError(f"fused integrands support dim <= {_lib.TQ_MAX_DIM}")
+        return dim
+
+    def _descriptor(self, dim, i):
         d = _lib.Integrand()
         d.family = self.family
         d.dim = dim
-        d.c0 = self.scale
-        self._fill(d)
+        d.c0 = self._sc("scale", i)
+        self._fill(d, i)
         return d
+
+    def descriptor(self, dim=None):
+        """The C descriptor of a single integrand (no parameter batch)."""
+        dim = self._check_dim(dim)
+        if self.batch_shape:
+            raise ValueError(f"{self.name} holds a parameter batch of shape {self.batch_shape}; use descriptors()")
+        return self._descriptor(dim, 0)
+
+    def descriptors(self, dim=None):
+        """C descriptors of every parameter set, in C order of the batch shape."""
+        dim = self._check_dim(dim)
+        return [self._descriptor(dim, i) for i in range(self.batch_size)]
+
+    def key(self):
+        """Bytes identifying the family, dimension and every parameter value."""
+        parts = [str((self.family, self.dim, self.batch_shape)).encode()]
+        for name in self._axis_names + self._scalar_names:
+            parts.append(np.ascontiguousarray(getattr(self, name), dtype=np.float64).tobytes())
+        return b"|".join(parts)
 
     # -- torch definition -------------------------------------------------
     def torch_eval(self, x):  # pragma: no cover - overridden
@@ -74,13 +155,17 @@ class BuiltinIntegrand:
     def __call__(self, x):
         if x.shape[-1] != self.dim:
             raise ValueError(f"{self.name} expects points of dim {self.dim}, got {x.shape[-1]}")
-        return self.scale * self.torch_eval(x)
+        if self.batch_shape:
+            # [N, dim] -> [N, 1, ..., 1, dim] against parameters [*S, dim]: [N, *S]
+            x = x.reshape((x.shape[0],) + (1,) * len(self.batch_shape) + (self.dim,))
+        return self._t(self.scale, x) * self.torch_eval(x)
 
     def _t(self, arr, x):
         return torch.as_tensor(arr, dtype=x.dtype, device=x.device)
 
     def __repr__(self):
-        return f"{type(self).__name__}(dim={self.dim})"
+        s = f", batch_shape={self.batch_shape}" if self.batch_shape else ""
+        return f"{type(self).__name__}(dim={self.dim}{s})"
 
 
 class GaussianPeak(BuiltinIntegrand):
@@ -88,16 +173,14 @@ class GaussianPeak(BuiltinIntegrand):
     name = "GaussianPeak"
 
     def __init__(self, a, u, dim=None, scale=1.0):
-        a_arr = np.asarray(a.detach().cpu() if isinstance(a, torch.Tensor) else a, dtype=np.float64).reshape(-1)
-        dim = a_arr.size if dim is None else dim
-        super().__init__(dim, scale)
-        self.a = _vec(a, self.dim, 1.0)
-        self.u = _vec(u, self.dim, 0.0)
+        super().__init__(_dim_of(a) if dim is None else dim, scale)
+        self._set_params(axis={"a": _axis_param(a, self.dim, 1.0, "a"), "u": _axis_param(u, self.dim, 0.0, "u")})
 
-    def _fill(self, d):
+    def _fill(self, d, i):
+        a, u = self._ax("a", i), self._ax("u", i)
         for k in range(self.dim):
-            d.a[k] = self.a[k]
-            d.u[k] = self.u[k]
+            d.a[k] = a[k]
+            d.u[k] = u[k]
 
     def torch_eval(self, x):
         a, u = self._t(self.a, x), self._t(self.u, x)
@@ -109,20 +192,18 @@ class Oscillatory(BuiltinIntegrand):
     name = "Oscillatory"
 
     def __init__(self, c, w=0.0, dim=None, scale=1.0):
-        c_arr = np.asarray(c.detach().cpu() if isinstance(c, torch.Tensor) else c, dtype=np.float64).reshape(-1)
-        dim = c_arr.size if dim is None else dim
-        super().__init__(dim, scale)
-        self.c = _vec(c, self.dim, 1.0)
-        self.w = float(w)
+        super().__init__(_dim_of(c) if dim is None else dim, scale)
+        self._set_params(axis={"c": _axis_param(c, self.dim, 1.0, "c")}, scalar={"w": _array(w, 0.0)})
 
-    def _fill(self, d):
-        d.c1 = self.w
+    def _fill(self, d, i):
+        d.c1 = self._sc("w", i)
+        c = self._ax("c", i)
         for k in range(self.dim):
-            d.a[k] = self.c[k]
+            d.a[k] = c[k]
 
     def torch_eval(self, x):
         c = self._t(self.c, x)
-        return torch.cos(2.0 * math.pi * self.w + x @ c)
+        return torch.cos(2.0 * math.pi * self._t(self.w, x) + torch.sum(x * c, dim=-1))
 
 
 class ProductPeak(BuiltinIntegrand):
@@ -130,18 +211,16 @@ class ProductPeak(BuiltinIntegrand):
     name = "ProductPeak"
 
     def __init__(self, a, u, dim=None, scale=1.0):
-        a_arr = np.asarray(a.detach().cpu() if isinstance(a, torch.Tensor) else a, dtype=np.float64).reshape(-1)
-        dim = a_arr.size if dim is None else dim
-        super().__init__(dim, scale)
-        self.a = _vec(a, self.dim, 1.0)
-        self.u = _vec(u, self.dim, 0.0)
+        super().__init__(_dim_of(a) if dim is None else dim, scale)
+        self._set_params(axis={"a": _axis_param(a, self.dim, 1.0, "a"), "u": _axis_param(u, self.dim, 0.0, "u")})
         if np.any(self.a == 0.0):
             raise ValueError("ProductPeak needs a_k != 0")
 
-    def _fill(self, d):
+    def _fill(self, d, i):
+        a, u = self._ax("a", i), self._ax("u", i)
         for k in range(self.dim):
-            d.a[k] = self.a[k]
-            d.u[k] = self.u[k]
+            d.a[k] = a[k]
+            d.u[k] = u[k]
 
     def torch_eval(self, x):
         a, u = self._t(self.a, x), self._t(self.u, x)
@@ -154,16 +233,16 @@ class ExpLinear(BuiltinIntegrand):
 
     def __init__(self, dim, a=None, c=0.0, scale=1.0):
         super().__init__(dim, scale)
-        self.a = _vec(a, self.dim, 1.0)
-        self.c = float(c)
+        self._set_params(axis={"a": _axis_param(a, self.dim, 1.0, "a")}, scalar={"c": _array(c, 0.0)})
 
-    def _fill(self, d):
-        d.c1 = self.c
+    def _fill(self, d, i):
+        d.c1 = self._sc("c", i)
+        a = self._ax("a", i)
         for k in range(self.dim):
-            d.a[k] = self.a[k]
+            d.a[k] = a[k]
 
     def torch_eval(self, x):
-        return torch.exp(self.c + x @ self._t(self.a, x))
+        return torch.exp(self._t(self.c, x) + torch.sum(x * self._t(self.a, x), dim=-1))
 
 
 class SinSum(BuiltinIntegrand):
@@ -172,13 +251,13 @@ class SinSum(BuiltinIntegrand):
 
     def __init__(self, dim, a=None, u=None, scale=1.0):
         super().__init__(dim, scale)
-        self.a = _vec(a, self.dim, 1.0)
-        self.u = _vec(u, self.dim, 0.0)
+        self._set_params(axis={"a": _axis_param(a, self.dim, 1.0, "a"), "u": _axis_param(u, self.dim, 0.0, "u")})
 
-    def _fill(self, d):
+    def _fill(self, d, i):
+        a, u = self._ax("a", i), self._ax("u", i)
         for k in range(self.dim):
-            d.a[k] = self.a[k]
-            d.u[k] = self.u[k]
+            d.a[k] = a[k]
+            d.u[k] = u[k]
 
     def torch_eval(self, x):
         return torch.sum(torch.sin(x * self._t(self.a, x) + self._t(self.u, x)), dim=-1)
@@ -192,24 +271,22 @@ class DoubleGaussian(BuiltinIntegrand):
 
     def __init__(self, dim, width=0.1, centers=(1.0 / 3.0, 2.0 / 3.0), scale=1.0):
         super().__init__(dim, scale)
-        self.width = float(width)
         self.centers = (float(centers[0]), float(centers[1]))
+        self._set_params(scalar={"width": _array(width, 0.1)})
         self.norm = 0.5 * (self.width * math.sqrt(math.pi)) ** (-self.dim)
 
-    def descriptor(self, dim=None):
-        d = super().descriptor(dim)
-        d.c0 = self.scale * self.norm
-        return d
-
-    def _fill(self, d):
+    def _fill(self, d, i):
+        w = self._sc("width", i)
+        d.c0 = self._sc("scale", i) * 0.5 * (w * math.sqrt(math.pi)) ** (-self.dim)
         d.c1, d.c2 = self.centers
-        d.c3 = 1.0 / (self.width * self.width)
+        d.c3 = 1.0 / (w * w)
 
     def torch_eval(self, x):
-        inv = 1.0 / (self.width * self.width)
+        w = self._t(self.width, x)
+        inv = 1.0 / (w * w)
         s1 = torch.sum((x - self.centers[0]) ** 2, dim=-1)
         s2 = torch.sum((x - self.centers[1]) ** 2, dim=-1)
-        return self.norm * (torch.exp(-inv * s1) + torch.exp(-inv * s2))
+        return self._t(self.norm, x) * (torch.exp(-inv * s1) + torch.exp(-inv * s2))
 
 
 class Constant(BuiltinIntegrand):
@@ -218,8 +295,9 @@ class Constant(BuiltinIntegrand):
 
     def __init__(self, dim, value=1.0):
         super().__init__(dim, scale=value)
+        self._set_params()
 
-    def _fill(self, d):
+    def _fill(self, d, i):
         pass
 
     def torch_eval(self, x):
@@ -235,6 +313,11 @@ def is_builtin(fn, dim=None):
     return fn.dim <= _lib.TQ_MAX_DIM
 
 
+def is_batched(fn):
+    """True when `fn` is a built-in family holding a parameter batch."""
+    return isinstance(fn, BuiltinIntegrand) and bool(fn.batch_shape)
+
+
 __all__ = [
     "BuiltinIntegrand",
     "GaussianPeak",
@@ -245,4 +328,5 @@ __all__ = [
     "DoubleGaussian",
     "Constant",
     "is_builtin",
+    "is_batched",
 ]

@@ -1,32 +1,90 @@
 """Gaussian quadrature (reference: torchquad/integration/gaussian.py:6-162).
 
-``GaussLegendre`` with the default root function runs on the fused HIP
-kernels; its node/weight tables live on the device, cached per (n, device).
-Subclasses that change ``_root_fn``, ``_resize_roots`` or the composite rule
-(e.g. Gauss-Hermite, tutorial.rst:1198-1236) run their own code on the
-reference's data layout.
+Any Gaussian rule -- the built-in ``GaussLegendre``, ``GaussHermite`` or a
+user subclass that sets ``_root_fn`` (and possibly overrides
+``_resize_roots`` / ``_apply_composite_rule``, as the tutorial's
+Gauss-Hermite does, tutorial.rst:1198-1236) -- runs on the fused and
+point/reduce HIP kernels with its node/weight tables uploaded once per
+(n, device, root function).  The kernels implement
 
-Nodes and weights: ``numpy.polynomial.legendre.leggauss`` (the reference's
-``_root_fn``) for n <= ``LEGGAUSS_MAX_N``; above that the package's O(n)
-device generator (Newton on asymptotic/recurrence evaluations of P_n), since
-leggauss builds a dense n x n companion matrix.
+    x_k = (b_k - a_k)/2 r + (a_k + b_k)/2,   I = prod_k (b_k - a_k)/2 sum W f,
+
+and the two hooks that shape a rule are classified once per class by
+evaluating them on small probe tensors:
+
+* ``_resize_roots``: affine (Gauss-Legendre's map to [a, b]) or identity;
+* ``_apply_composite_rule``: the sum scaled by (b - a)/2 per axis (the base
+  class's rule) or the plain sum.
+
+(affine, scaled) is the kernels' rule on the given domain; (identity, plain
+sum) is the kernels' rule on the unit box [-1, 1]^dim, where both the map
+and the scale are the identity.  Other combinations, and classes that
+override ``_roots``, ``_weights``, ``_grid_func`` or ``calculate_result``,
+run their own code on the reference's data layout.
+
+Nodes and weights come from ``_cached_points_and_weights`` (the reference's
+cache around ``_root_fn``), except Gauss-Legendre above ``LEGGAUSS_MAX_N``
+nodes, where the package's O(n) device generator replaces
+``numpy.polynomial.legendre.leggauss`` (a dense n x n eigenproblem).
 """
 import numpy
 import torch
 
 from .. import _ops
-from .grid_integrator import GridIntegrator, _composite_by_kernel, _mark_owner
+from .grid_integrator import GridIntegrator
 
 LEGGAUSS_MAX_N = 256
 
+AFFINE = "affine"  # kernels' map and (b-a)/2 scale on the given domain
+UNIT = "unit"      # identity map and plain sum: the kernels on [-1, 1]^dim
+
+_mode_cache = {}
+
+
+def _classify_resize(resize):
+    """'identity', 'affine' or None for a _resize_roots bound method."""
+    r = torch.tensor([-0.7, 0.1, 0.55], dtype=torch.float64)
+    seen = set()
+    for a, b in ((0.3, 1.7), (-2.0, -0.5)):
+        dom = torch.tensor([a, b], dtype=torch.float64)
+        out = torch.as_tensor(resize(dom, r.clone()), dtype=torch.float64)
+        if tuple(out.shape) != tuple(r.shape):
+            return None
+        if torch.allclose(out, r, rtol=1e-14, atol=0.0):
+            seen.add("identity")
+        elif torch.allclose(out, 0.5 * (b - a) * r + 0.5 * (a + b), rtol=1e-14, atol=1e-15):
+            seen.add("affine")
+        else:
+            return None
+    return seen.pop() if len(seen) == 1 else None
+
+
+def _classify_composite(rule):
+    """'sum', 'scaled' or None for an _apply_composite_rule callable."""
+    g = torch.Generator().manual_seed(7)
+    vals = torch.rand((2, 3, 3), generator=g, dtype=torch.float64) + 0.5
+    dom = torch.tensor([[0.3, 1.7], [-1.0, 2.0]], dtype=torch.float64)
+    hs = torch.tensor([0.1, 0.2], dtype=torch.float64)
+    out = torch.as_tensor(rule(vals.clone(), 2, hs, dom), dtype=torch.float64)
+    plain = vals.sum(dim=(-1, -2))
+    if tuple(out.shape) != tuple(plain.shape):
+        return None
+    if torch.allclose(out, plain, rtol=1e-13, atol=0.0):
+        return "sum"
+    if torch.allclose(out, 0.7 * 1.5 * plain, rtol=1e-13, atol=0.0):
+        return "scaled"
+    return None
+
 
 class Gaussian(GridIntegrator):
-    """Base class for Gaussian quadrature (Gauss-Legendre on [-1,1] by default).
+    """Base class for Gaussian quadrature (Gauss-Legendre roots on [-1,1] by default).
 
     The primary hooks are ``_root_fn`` (returns roots and weights like
     ``numpy.polynomial.legendre.leggauss``), ``_resize_roots`` and
     ``_apply_composite_rule``.
     """
+
+    _tq_rule = _ops.RULES["gauss_legendre"]
 
     def __init__(self):
         super().__init__()
@@ -40,6 +98,64 @@ class Gaussian(GridIntegrator):
         """Integrate fn with a Gaussian rule; N is the total number of points (gaussian.py:29-42)."""
         return super().integrate(fn, dim, N, integration_domain, backend)
 
+    # -- which kernels can run this rule --------------------------------------
+    def _tq_mode(self):
+        """AFFINE, UNIT or None (see the module docstring); cached per class."""
+        cls = type(self)
+        if (
+            cls._roots is not Gaussian._roots
+            or cls._weights is not Gaussian._weights
+            or cls._grid_func is not Gaussian._grid_func
+            or cls.calculate_result is not GridIntegrator.calculate_result
+            or cls._cached_points_and_weights is not Gaussian._cached_points_and_weights
+        ):
+            return None
+        key = (cls, cls.__dict__.get("_resize_roots"), cls.__dict__.get("_apply_composite_rule"))
+        if key in _mode_cache:
+            return _mode_cache[key]
+        try:
+            resize = _classify_resize(self._resize_roots)
+            comp = _classify_composite(self._apply_composite_rule)
+        except Exception:  # a user hook that does not accept the probes: its own code runs
+            resize = comp = None
+        mode = {("affine", "scaled"): AFFINE, ("identity", "sum"): UNIT}.get((resize, comp))
+        _mode_cache[key] = mode
+        return mode
+
+    def _tq_standard(self):
+        return self._tq_mode() is not None
+
+    def _tq_domain(self, domain):
+        """The domain the kernels see: the unit box for identity-map, plain-sum rules."""
+        if self._tq_mode() != UNIT:
+            return domain
+        if isinstance(domain, torch.Tensor):
+            unit = torch.empty_like(domain)
+            unit[:, 0] = -1.0
+            unit[:, 1] = 1.0
+            return unit
+        return [[-1.0, 1.0] for _ in domain]
+
+    def _gl_tables(self, n, device):
+        """Device float64 (nodes, weights) of the n-point rule, cached per
+        (n, device, root function)."""
+        key = (int(n), str(device), id(self._root_fn), tuple(self._root_args))
+        t = self._device_cache.get(key)
+        if t is None:
+            if self._root_fn is numpy.polynomial.legendre.leggauss and not self._root_args and n > LEGGAUSS_MAX_N:
+                from ..gl_nodes import gauss_legendre
+
+                t = gauss_legendre(int(n), device=device)
+            else:
+                r, w = self._cached_points_and_weights(int(n))
+                t = (
+                    torch.tensor(numpy.asarray(r, dtype=numpy.float64), device=device),
+                    torch.tensor(numpy.asarray(w, dtype=numpy.float64), device=device),
+                )
+            self._device_cache[key] = t
+        return t
+
+    # -- the reference's hooks -----------------------------------------------
     def _weights(self, N, dim, backend, requires_grad=False):
         """Tensor-product weights [N^dim] (gaussian.py:44-69)."""
         weights = torch.tensor(self._cached_points_and_weights(N)[1])
@@ -67,6 +183,8 @@ class Gaussian(GridIntegrator):
         def f(integration_domain, N, requires_grad, backend=None):
             return self._resize_roots(integration_domain, self._roots(N, backend, requires_grad))
 
+        if self._tq_standard():
+            f._tq_spec = ("gauss_legendre", self._gl_tables)
         return f
 
     def _resize_roots(self, integration_domain, roots):
@@ -102,77 +220,27 @@ class Gaussian(GridIntegrator):
         return cur_dim_areas
 
 
-@_mark_owner
 class GaussLegendre(Gaussian):
     """Gauss-Legendre quadrature on any box: x = (b-a)/2 r + (a+b)/2 (gaussian.py:147-162)."""
 
-    _tq_rule = _ops.RULES["gauss_legendre"]
-
     def __init__(self):
         super().__init__()
-
-    def _tq_standard(self):
-        return (
-            super()._tq_standard()
-            and type(self)._resize_roots is GaussLegendre._resize_roots
-            and type(self)._roots is Gaussian._roots
-            and type(self)._cached_points_and_weights is Gaussian._cached_points_and_weights
-            and self._root_fn is numpy.polynomial.legendre.leggauss
-            and self._root_args == ()
-        )
-
-    def _gl_tables(self, n, device):
-        """Device float64 (nodes, weights) on [-1, 1], cached per (n, device)."""
-        key = (int(n), str(device))
-        t = self._device_cache.get(key)
-        if t is None:
-            if self._root_fn is numpy.polynomial.legendre.leggauss and n > LEGGAUSS_MAX_N:
-                from ..gl_nodes import gauss_legendre
-
-                t = gauss_legendre(int(n), device=device)
-            else:
-                r, w = self._cached_points_and_weights(int(n))
-                t = (
-                    torch.tensor(numpy.asarray(r, dtype=numpy.float64), device=device),
-                    torch.tensor(numpy.asarray(w, dtype=numpy.float64), device=device),
-                )
-            self._device_cache[key] = t
-        return t
-
-    @property
-    def _grid_func(self):
-        f = Gaussian._grid_func.fget(self)
-        if self._tq_standard():
-            f._tq_spec = ("gauss_legendre", self._gl_tables)
-        return f
 
     def _resize_roots(self, integration_domain, roots):
         a = integration_domain[0]
         b = integration_domain[1]
         return ((b - a) / 2) * roots + ((a + b) / 2)
 
-    @staticmethod
-    def _apply_composite_rule(cur_dim_areas, dim, hs, domain):
-        """Sum with (b-a)/2 per axis via the HIP reduction (gaussian.py:132-144)."""
-        return _composite_by_kernel(_ops.RULES["gauss_legendre"], cur_dim_areas, dim, hs, domain)
 
-
-@_mark_owner
 class GaussHermite(Gaussian):
     """Gauss-Hermite quadrature: integrals of exp(-|x|^2) f(x) over R^dim.
 
-    The reference shows this rule as a user subclass of Gaussian
-    (tutorial.rst:1198-1236): roots and weights from
-    ``scipy.special.roots_hermite``, identity ``_resize_roots`` and a plain
-    sum as composite rule, so the integration domain only sets dtype and
-    device.  Here it runs on the fused and point/reduce kernels of the
-    Gauss-Legendre path with the Hermite node/weight tables on the unit box
-    [-1, 1]^dim, where the affine map is the identity and the per-axis scale
-    (b - a)/2 is 1.  A subclass that overrides a hook runs its own code on
-    the reference's layout.
+    The tutorial's custom rule (tutorial.rst:1198-1236): roots and weights
+    from ``scipy.special.roots_hermite``, identity ``_resize_roots`` and a
+    plain sum as composite rule, so the integration domain only sets dtype
+    and device.  It runs on the kernels' unit box [-1, 1]^dim like any user
+    subclass written the same way.
     """
-
-    _tq_rule = _ops.RULES["gauss_legendre"]
 
     def __init__(self):
         super().__init__()
@@ -180,46 +248,6 @@ class GaussHermite(Gaussian):
 
         self.name = "Gauss-Hermite"
         self._root_fn = scipy.special.roots_hermite
-
-    def integrate(self, fn, dim, N=8, integration_domain=None, backend=None):
-        """Integrate exp(-|x|^2) fn(x) over R^dim with N total points."""
-        from .utils import _float_domain, _setup_integration_domain
-
-        dom = _float_domain(_setup_integration_domain(dim, integration_domain, backend))
-        unit = torch.tensor([[-1.0, 1.0]] * dim, dtype=dom.dtype, device=dom.device)
-        return super().integrate(fn, dim, N, unit, backend)
-
-    def _tq_standard(self):
-        import scipy.special
-
-        return (
-            super()._tq_standard()
-            and type(self)._resize_roots is Gaussian._resize_roots
-            and type(self)._roots is Gaussian._roots
-            and type(self)._cached_points_and_weights is Gaussian._cached_points_and_weights
-            and self._root_fn is scipy.special.roots_hermite
-            and self._root_args == ()
-        )
-
-    def _gl_tables(self, n, device):
-        """Device float64 (nodes, weights) of the n-point Hermite rule, cached per (n, device)."""
-        key = (int(n), str(device))
-        t = self._device_cache.get(key)
-        if t is None:
-            r, w = self._cached_points_and_weights(int(n))
-            t = (
-                torch.tensor(numpy.asarray(r, dtype=numpy.float64), device=device),
-                torch.tensor(numpy.asarray(w, dtype=numpy.float64), device=device),
-            )
-            self._device_cache[key] = t
-        return t
-
-    @property
-    def _grid_func(self):
-        f = Gaussian._grid_func.fget(self)
-        if self._tq_standard():
-            f._tq_spec = ("gauss_legendre", self._gl_tables)
-        return f
 
     @staticmethod
     def _apply_composite_rule(cur_dim_areas, dim, hs, domain):

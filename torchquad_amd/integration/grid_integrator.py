@@ -18,12 +18,13 @@ import warnings
 import torch
 
 from .. import _ops
-from ..integrands import is_builtin
+from ..integrands import is_batched, is_builtin
 from ..utils.set_log_level import logger
 from .base_integrator import BaseIntegrator
 from .integration_grid import IntegrationGrid, grid_func as _standard_grid_func
 from .utils import (
     _float_domain,
+    _host_values,
     _setup_integration_domain,
     _warn_scalar_deprecation,
     expand_func_values_and_squeeze_integral,
@@ -113,6 +114,11 @@ class GridIntegrator(BaseIntegrator):
     def _gl_tables(self, n, device):
         return None
 
+    def _tq_domain(self, domain):
+        """The domain the standard kernels integrate over (Gaussian rules with an
+        identity map use the unit box)."""
+        return domain
+
     def _grid_n(self, N, dim):
         """n per axis after _adjust_N and IntegrationGrid's checks (integration_grid.py:64, 105-124)."""
         N = self._adjust_N(dim=dim, N=N)
@@ -132,11 +138,15 @@ class GridIntegrator(BaseIntegrator):
         integration_domain = _setup_integration_domain(dim, integration_domain, backend)
         self._check_inputs(dim=dim, N=N, integration_domain=integration_domain)
         domain = _float_domain(integration_domain)
+        if self._tq_standard():
+            domain = self._tq_domain(domain)
 
         if self._tq_standard() and is_builtin(fn, dim) and not _needs_grad(domain):
             n = self._grid_n(N, dim)
-            gl = self._gl_tables(n, domain.device)
             self._nr_of_fevals = n**dim
+            if is_batched(fn):
+                return self._param_batch_integrate(fn, dim, n, domain)
+            gl = self._gl_tables(n, domain.device)
             _warn_scalar_deprecation()
             return _ops.grid_integrate(self._tq_rule, fn, dim, n, domain, gl)
 
@@ -156,6 +166,25 @@ class GridIntegrator(BaseIntegrator):
         self._nr_of_fevals = num_points
         return self.calculate_result(function_values, dim, n_per_dim, hs, integration_domain)
 
+    def _param_batch_integrate(self, fn, dim, n, domain):
+        """[*S] integrals of a built-in family's parameter batch: one batched
+        launch over the grid (the fused form of an [N, *S] integrand,
+        base_integrator.py:77-89)."""
+        from ..batch import ParamBatchRunner
+
+        runner = self.__dict__.get("_tq_param_batches")
+        if runner is None:
+            runner = self._tq_param_batches = ParamBatchRunner()
+        gauss = self._tq_rule == _ops.RULES["gauss_legendre"]
+        gl = self._gl_tables if gauss else None
+        tables_key = (id(self._root_fn), tuple(self._root_args)) if gauss else None
+        out = runner.run(self._tq_rule, fn, dim, n, domain, _host_values(domain), gl, tables_key)
+        if fn.batch_shape == (1,):
+            # [N, 1] values are squeezed like [N] ones (utils.py:235-277)
+            _warn_scalar_deprecation()
+            return torch.squeeze(out)
+        return out
+
     @expand_func_values_and_squeeze_integral
     def _standard_result(self, function_values, dim, n_per_dim, hs, integration_domain, unit=False):
         return self._kernel_result(function_values, dim, n_per_dim, hs, integration_domain, unit)
@@ -167,7 +196,7 @@ class GridIntegrator(BaseIntegrator):
                 f"reshaping produced shape {list(function_values.shape)}, expected {expected} points "
                 f"({n_per_dim}^{dim})"
             )
-        domain = _float_domain(integration_domain)
+        domain = self._tq_domain(_float_domain(integration_domain))
         gl = self._gl_tables(n_per_dim, domain.device)
         gauss = self._tq_rule == _ops.RULES["gauss_legendre"]
         return kernel_reduce(
@@ -196,6 +225,8 @@ class GridIntegrator(BaseIntegrator):
     def calculate_grid(self, N, integration_domain, disable_integration_domain_check=False):
         """Grid points, mesh widths and points per axis (grid_integrator.py:93-127)."""
         N = self._adjust_N(dim=integration_domain.shape[0], N=N)
+        if self._tq_standard():
+            integration_domain = self._tq_domain(integration_domain)
         grid = IntegrationGrid(N, integration_domain, self._grid_func, disable_integration_domain_check)
         return grid.points, grid.h, grid._N
 
@@ -230,10 +261,12 @@ class GridIntegrator(BaseIntegrator):
 
         def compiled_integrate(fn, integration_domain):
             dom = _float_domain(_setup_integration_domain(dim, integration_domain, backend))
-            if not (self._tq_standard() and is_builtin(fn, dim) and dom.is_cuda and not _needs_grad(dom)):
+            if not (self._tq_standard() and is_builtin(fn, dim) and dom.is_cuda and not _needs_grad(dom)) \
+                    or is_batched(fn):
                 return self.integrate(fn, dim, N, dom, backend)
             if tuple(dom.shape) != (dim, 2):
                 raise ValueError(f"integration_domain must have shape ({dim}, 2)")
+            dom = self._tq_domain(dom)
             key = (bytes(fn.descriptor(dim)), dom.dtype, dom.device)
             entry = graphs.get(key)
             if entry is None:

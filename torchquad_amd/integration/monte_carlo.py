@@ -7,10 +7,13 @@ reference returns no error estimate).  Any other callable gets the same
 sample stream as ``[N, dim]`` points from the HIP point writer, and the mean
 is taken by a HIP reduction.
 """
+import warnings
+
+import numpy as np
 import torch
 
 from .. import _ops
-from ..integrands import is_builtin
+from ..integrands import is_batched, is_builtin
 from ..utils.set_log_level import logger
 from .base_integrator import BaseIntegrator
 from .rng import RNG
@@ -78,11 +81,13 @@ class MonteCarlo(BaseIntegrator):
             elif seed is not None:
                 raise ValueError("seed and rng cannot both be passed")
             first = rng.reserve(N)
+            self.last_std_error = None
+            self._nr_of_fevals = N
+            if is_batched(fn):
+                return self._param_batch_integrate(fn, dim, N, domain, rng.key, first)
             stats = torch.empty(3, dtype=torch.float64, device=domain.device)
             out = _ops.mc_integrate(fn, dim, N, domain, rng.key, first, stats=stats)
             self._last_stats = stats
-            self.last_std_error = None
-            self._nr_of_fevals = N
             _warn_scalar_deprecation()
             return out
         sample_points = self.calculate_sample_points(N, integration_domain, seed, rng)
@@ -90,14 +95,36 @@ class MonteCarlo(BaseIntegrator):
         function_values, self._nr_of_fevals = self.evaluate_integrand(fn, sample_points)
         return self.calculate_result(function_values, integration_domain)
 
+    def _param_batch_integrate(self, fn, dim, N, domain, key, first):
+        """[*S] integrals of a built-in family's parameter batch, every set on
+        the same samples (as an [N, *S] integrand sees one set of points,
+        monte_carlo.py:60-106): one fused launch per set."""
+        descs = fn.descriptors(dim)
+        out = torch.empty(len(descs), dtype=domain.dtype, device=domain.device)
+        stats = torch.empty((len(descs), 3), dtype=torch.float64, device=domain.device)
+        for i, d in enumerate(descs):
+            _ops.mc_integrate(None, dim, N, domain, key, first, stats=stats[i], desc=d, out=out[i])
+        self._last_stats = stats
+        self._last_shape = fn.batch_shape
+        if fn.batch_shape == (1,):
+            _warn_scalar_deprecation()
+            return torch.squeeze(out)
+        return out.reshape(fn.batch_shape)
+
     def std_error(self):
-        """Standard error of the last fused integral (one device read)."""
+        """Standard error of the last fused integral (one device read); an
+        array of the batch shape for a parameter batch."""
         if self._last_stats is None:
             return None
-        s1, s2, n = self._last_stats.tolist()
+        st = self._last_stats.double().reshape(-1, 3).cpu().numpy()
+        s1, s2, n = st[:, 0], st[:, 1], st[:, 2]
         mean = s1 / n
-        var = max(s2 / n - mean * mean, 0.0)
-        self.last_std_error = (var / max(n - 1.0, 1.0)) ** 0.5
+        var = np.maximum(s2 / n - mean * mean, 0.0)
+        err = np.sqrt(var / np.maximum(n - 1.0, 1.0))
+        if self._last_stats.dim() == 1:
+            self.last_std_error = float(err[0])
+        else:
+            self.last_std_error = err.reshape(self._last_shape)
         return self.last_std_error
 
     @expand_func_values_and_squeeze_integral
@@ -126,12 +153,63 @@ class MonteCarlo(BaseIntegrator):
         return _ops.mc_points(dim, N, domain, rng.key, first)
 
     def get_jit_compiled_integrate(self, dim, N=1000, integration_domain=None, seed=None, backend=None):
-        """Integrate function with dim and N fixed (monte_carlo.py:108-225)."""
+        """Integrate function with dim and N fixed (monte_carlo.py:108-225).
+
+        As in the reference, every call draws new sample points from one
+        random stream started at ``seed``.  For a built-in integrand the fused
+        step (sampling, integrand, reduction, finalize) is captured into a HIP
+        graph on the first call per (integrand, domain dtype and device); the
+        graph reads the domain and the stream position from device buffers,
+        so a call copies the domain, stores the position and replays, with no
+        other host work.  Other callables (and domains that require grad)
+        run the eager path on the same stream.
+
+        Returns:
+            function(fn, integration_domain)
+        """
         integration_domain = _setup_integration_domain(dim, integration_domain, backend)
         self._check_inputs(dim=dim, N=N, integration_domain=integration_domain)
         rng = RNG(backend="torch", seed=seed)
+        graphs = {}
 
         def compiled_integrate(fn, integration_domain):
-            return self.integrate(fn, dim, N, integration_domain, rng=rng, backend=backend)
+            dom = _float_domain(_setup_integration_domain(dim, integration_domain, backend))
+            fused = (is_builtin(fn, dim) and not is_batched(fn) and dom.is_cuda
+                     and not (torch.is_grad_enabled() and dom.requires_grad))
+            if not fused:
+                return self.integrate(fn, dim, N, dom, rng=rng, backend=backend)
+            key = (bytes(fn.descriptor(dim)), dom.dtype, dom.device)
+            entry = graphs.get(key)
+            if entry is None:
+                entry = self._capture_step(fn, dim, N, dom, rng.key)
+                graphs[key] = entry
+            graph, sdom, soff, sout, sstats, _ws = entry
+            sdom.copy_(dom)
+            soff.fill_(rng.reserve(N))
+            graph.replay()
+            self._nr_of_fevals = N
+            self._last_stats = sstats.clone()
+            self.last_std_error = None
+            return sout.clone()
 
         return compiled_integrate
+
+    @staticmethod
+    def _capture_step(fn, dim, N, domain, key):
+        """(graph, static domain, stream position, result, stats, workspace) of one built-in
+        integrand's fused Monte Carlo step; the entry owns every buffer the graph uses."""
+        dev = domain.device
+        sdom = domain.detach().clone()
+        soff = torch.zeros((), dtype=torch.int64, device=dev)
+        sout = torch.empty((), dtype=sdom.dtype, device=dev)
+        sstats = torch.empty(3, dtype=torch.float64, device=dev)
+        ws = _ops.mc_workspace(dim, N, sdom)
+        # eager warm-up (lazy initialisation outside the capture)
+        _ops.mc_integrate(fn, dim, N, sdom, key, stats=sstats, offset_dev=soff, ws=ws, out=sout)
+        torch.cuda.synchronize(dev)
+        graph = torch.cuda.CUDAGraph()
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            with torch.cuda.graph(graph):
+                _ops.mc_integrate(fn, dim, N, sdom, key, stats=sstats, offset_dev=soff, ws=ws, out=sout)
+        return (graph, sdom, soff, sout, sstats, ws)

@@ -71,8 +71,11 @@ def _setup_integration_domain(dim, integration_domain, backend):
         dtype_name = _get_precision("torch")
         dtype = {"float32": torch.float32, "float64": torch.float64}.get(dtype_name, torch.get_default_dtype())
         _check_list_domain(integration_domain)
+        host = integration_domain
         integration_domain = torch.tensor(integration_domain, dtype=dtype, device=_default_device())
         _mark_checked(integration_domain)
+        # the values as the tensor holds them (rounded to its dtype)
+        _remember_host(integration_domain, torch.tensor(host, dtype=dtype).to(torch.float64).numpy())
 
     if tuple(integration_domain.shape) != (dim, 2):
         raise ValueError(
@@ -184,6 +187,34 @@ def _check_integration_domain(integration_domain):
         raise ValueError("integration_domain has invalid boundary values")
     _mark_checked(integration_domain)
     return dim
+
+
+# Host copies of device domains, keyed like _checked: the batched kernels
+# take domains by value, and a domain tensor reused across calls is read to
+# the host once.
+_host_copies = {}
+
+
+def _remember_host(t, values):
+    key = id(t)
+    try:
+        ref = weakref.ref(t, lambda _r, k=key: _host_copies.pop(k, None))
+    except TypeError:
+        return
+    _host_copies[key] = (ref, t.data_ptr(), t._version, values)
+
+
+def _host_values(t):
+    """float64 numpy copy of a domain tensor (one device read per tensor and
+    in-place version)."""
+    e = _host_copies.get(id(t))
+    if e is not None:
+        ref, p, v, values = e
+        if ref() is t and p == t.data_ptr() and v == t._version:
+            return values
+    values = t.detach().to("cpu", torch.float64).numpy().copy()
+    _remember_host(t, values)
+    return values
 
 
 def _float_domain(integration_domain):
