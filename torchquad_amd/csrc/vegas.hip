@@ -10,10 +10,11 @@
 //   k_vegas_strat   per-cube JF/JF2 (segmented, fixed order), ih, sigma^2, dh
 //   k_vegas_strat_finalize   iteration result and dh normaliser
 //   k_vegas_map_reduce + k_vegas_map_smooth + k_vegas_map_edges   update_map
-// Map histograms are accumulated per wave in LDS by looping over the distinct
-// bins present in the wave (ballot + fixed-order wave reduction), combined in
-// wave order per block and reduced over blocks in block order: the whole
-// iteration is deterministic.  Integer-valued and rounding-sensitive steps
+// Map histograms that fit in LDS are exact per block (integer atomics on a
+// per-bin fixed-point sum relative to the bin's largest exponent, see
+// HistSlots) and reduced over blocks in block order: the whole iteration is
+// deterministic.  Larger maps aggregate each wave's equal bins (sorted keys,
+// fixed-order scan) and add them with FP64 atomics.  Integer-valued and rounding-sensitive steps
 // (bin ids, y, nh) are written operation by operation as the reference does
 // them, with floating-point contraction disabled in this file.
 #pragma clang fp contract(off)
@@ -289,15 +290,84 @@ __device__ inline BinSum wave_bin_sums(int id, double v) {
     return r;
 }
 
-// Adds v into bin `id` of axis k of the wave-private LDS slot for every lane
-// with id >= 0.
-__device__ inline void wave_hist_add(double* hw, uint32_t* hc, int k, int ni, int id, double v) {
-    if (!__ballot(id >= 0)) return;
-    const BinSum r = wave_bin_sums(id, v);
-    if (r.head) {
-        hw[k * ni + r.bin] += r.sum;
-        hc[k * ni + r.bin] += (uint32_t)r.count;
+// Exact block histograms (maps that fit in LDS).  Each bin keeps a count, the
+// largest binary exponent among its values and a 64-bit fixed-point sum of
+// the values relative to that exponent.  All three are updated with integer
+// LDS atomics, which commute, so the block's sums do not depend on the order
+// in which lanes and waves arrive: results are reproducible run to run.
+//
+// A value v >= 0 (a square) is m 2^(e - 1075) with m < 2^53 and e its biased
+// exponent field (e = 1 with no hidden bit for subnormals).  Pass 1 takes
+// the largest e per bin (E); pass 2 adds floor(m 2^(e - E)) in units of
+// 2^(E - 1075).  A bin receives at most kSB values per block (one per sample
+// and axis), so the sum stays below 2^63; each value loses under one unit,
+// i.e. under 2^-52 of the bin's largest value.  Non-finite values set
+// E = 0x7ff and mark the sum (1: +inf seen, 2: NaN seen) instead.
+constexpr uint32_t kExpNonFinite = 0x7ffu;
+
+struct HistSlots {
+    unsigned long long* sum;  // [W]
+    uint32_t* exp;            // [W]
+    uint32_t* cnt;            // [W]
+};
+
+__device__ inline size_t hist_slot_bytes(int W) { return (size_t)W * (8 + 4 + 4); }
+
+__device__ inline HistSlots hist_slots(unsigned char* base, int W) {
+    HistSlots h;
+    h.sum = reinterpret_cast<unsigned long long*>(base);
+    h.exp = reinterpret_cast<uint32_t*>(base + (size_t)W * 8);
+    h.cnt = reinterpret_cast<uint32_t*>(base + (size_t)W * 12);
+    return h;
+}
+
+__device__ inline void hist_zero(const HistSlots& h, int W) {
+    for (int i = threadIdx.x; i < W; i += blockDim.x) {
+        h.sum[i] = 0ull;
+        h.exp[i] = 0u;
+        h.cnt[i] = 0u;
     }
+}
+
+// biased exponent field and integer significand of v >= 0
+__device__ inline uint32_t f64_exp(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const uint32_t e = (uint32_t)(b >> 52) & 0x7ffu;
+    return e == 0u ? 1u : e;
+}
+__device__ inline unsigned long long f64_sig(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned long long f = b & ((1ull << 52) - 1ull);
+    return ((b >> 52) & 0x7ffull) ? (f | (1ull << 52)) : f;
+}
+
+// pass 1 for one (bin, value): count and largest exponent
+__device__ inline void hist_pass1(const HistSlots& h, int bin, double v) {
+    atomicAdd(&h.cnt[bin], 1u);
+    if (v != 0.0) atomicMax(&h.exp[bin], f64_exp(v));
+}
+
+// pass 2 for one (bin, value), after every pass 1 of the block
+__device__ inline void hist_pass2(const HistSlots& h, int bin, double v) {
+    if (v == 0.0) return;
+    const uint32_t E = h.exp[bin];
+    const uint32_t e = f64_exp(v);
+    if (e == kExpNonFinite) {
+        atomicOr(&h.sum[bin], (v != v) ? 2ull : 1ull);
+        return;
+    }
+    if (E == kExpNonFinite) return;
+    const uint32_t sh = E - e;
+    if (sh < 64u) atomicAdd(&h.sum[bin], f64_sig(v) >> sh);
+}
+
+// the bin's FP64 value after pass 2
+__device__ inline double hist_value(const HistSlots& h, int bin) {
+    const uint32_t E = h.exp[bin];
+    const unsigned long long s = h.sum[bin];
+    if (E == kExpNonFinite) return (s & 2ull) ? __longlong_as_double(0x7ff8000000000000ll)
+                                              : __longlong_as_double(0x7ff0000000000000ll);
+    return ldexp((double)s, (int)E - 1075);
 }
 
 // Same aggregation, written with FP64 atomics to the global accumulators
@@ -354,18 +424,21 @@ __device__ inline double uniform_coord<double>(uint64_t seed, uint64_t s, int k,
 // d <= 8 so the unrolled per-axis work is not predicated over 16 axes.
 template <typename T, int FAM, int MODE, bool LDS_HIST, int MAXD = TQ_MAX_DIM>
 __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
+    static_assert(MAXD % 2 == 0, "bin ids are packed in pairs");
     using F = Family<FAM, T>;
     __shared__ int s_off[kOffLds];  // cube offsets relative to the block's first sample, clamped
     __shared__ double s_map[kMapLds];
-    // wave-private histograms in dynamic LDS sized to the map (hist_lds_bytes)
+    // exact block histogram in dynamic LDS sized to the map (hist_slot_bytes)
     extern __shared__ __align__(16) unsigned char s_hist_raw[];
-    double* s_hw = reinterpret_cast<double*>(s_hist_raw);
     __shared__ long long s_hrange[2];
     const tq_vegas_state& st = A.st;
-    const int tid = threadIdx.x, wave = tid >> 6;
+    const int tid = threadIdx.x;
     const int d = st.dim, ni = st.n_intervals;
     const int W = d * ni;
-    uint32_t* s_hc = reinterpret_cast<uint32_t*>(s_hist_raw + (size_t)kWavesPerBlock * W * sizeof(double));
+    const HistSlots hs = hist_slots(s_hist_raw, W);
+    // the block's histogram values and packed bin ids, kept for pass 2
+    double r_jf2[kSPT];
+    uint32_t r_ids[kSPT][(MAXD + 1) / 2];
 
     long long total;
     if (A.warmup) {
@@ -388,12 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
         xe = s_map;
         dxe = s_map + d * (ni + 1);
     }
-    if (MODE == MODE_FUSED && LDS_HIST && A.hist) {
-        for (int i = tid; i < kWavesPerBlock * W; i += kBlock) {
-            s_hw[(i / W) * W + (i % W)] = 0.0;
-            s_hc[(i / W) * W + (i % W)] = 0u;
-        }
-    }
+    if (MODE == MODE_FUSED && LDS_HIST && A.hist) hist_zero(hs, W);
     // cube range of this block
     long long h_lo = 0, n_off = 0;
     bool off_lds = false;
@@ -451,6 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
                            (unsigned long long)st.n_strat);
     }
 
+#pragma unroll
     for (int j = 0; j < kSPT; ++j) {
         const long long s = s0 + (long long)j * kBlock + tid;
         const bool active = s < total;
@@ -540,29 +609,43 @@ __global__ __launch_bounds__(kBlock) void k_vegas_sample(SampleArgs<T> A) {
         if (A.hist) {
             const double jf2 = jf * jf;
             // unrolled with a uniform bound check, so ids[] stays in registers
+            if (LDS_HIST) {
+                r_jf2[j] = active ? jf2 : 0.0;
 #pragma unroll
-            for (int k = 0; k < MAXD; ++k) {
-                if (k < d) {
-                    const int id = active ? ids[k] : -1;
-                    if (LDS_HIST)
-                        wave_hist_add(s_hw + wave * W, s_hc + wave * W, k, ni, id, jf2);
-                    else
+                for (int k = 0; k < MAXD; k += 2) {
+                    const uint32_t lo = (k < d) ? (uint32_t)ids[k] : 0u;
+                    const uint32_t hi = (k + 1 < d) ? (uint32_t)ids[k + 1] : 0u;
+                    r_ids[j][k >> 1] = lo | (hi << 16);
+                }
+                if (active) {
+#pragma unroll
+                    for (int k = 0; k < MAXD; ++k)
+                        if (k < d) hist_pass1(hs, k * ni + ids[k], jf2);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < MAXD; ++k) {
+                    if (k < d) {
+                        const int id = active ? ids[k] : -1;
                         wave_hist_add_global(st.weights, st.counts, k, ni, id, jf2);
+                    }
                 }
             }
         }
     }
     if (MODE == MODE_FUSED && LDS_HIST && A.hist) {
         __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kSPT; ++j) {
+            if (r_jf2[j] == 0.0) continue;
+#pragma unroll
+            for (int k = 0; k < MAXD; ++k)
+                if (k < d) hist_pass2(hs, k * ni + (int)((r_ids[j][k >> 1] >> (16 * (k & 1))) & 0xffffu), r_jf2[j]);
+        }
+        __syncthreads();
         for (int e = tid; e < W; e += kBlock) {
-            double w = 0.0;
-            uint32_t c = 0;
-            for (int wv = 0; wv < kWavesPerBlock; ++wv) {
-                w += s_hw[wv * W + e];
-                c += s_hc[wv * W + e];
-            }
-            st.hist_w[(long long)blockIdx.x * W + e] = w;
-            st.hist_c[(long long)blockIdx.x * W + e] = c;
+            st.hist_w[(long long)blockIdx.x * W + e] = hist_value(hs, e);
+            st.hist_c[(long long)blockIdx.x * W + e] = hs.cnt[e];
         }
     }
 }
@@ -572,54 +655,56 @@ template <typename T, bool LDS_HIST>
 __global__ __launch_bounds__(kBlock) void k_vegas_consume(tq_vegas_state st, const T* fv, const T* domain,
                                                           const double* jacs, const int32_t* idsg, int warmup,
                                                           long long n_warm, int hist) {
-    __shared__ double s_hw[LDS_HIST ? kWavesPerBlock * kHistLds : 1];
-    __shared__ uint32_t s_hc[LDS_HIST ? kWavesPerBlock * kHistLds : 1];
-    const int tid = threadIdx.x, wave = tid >> 6;
+    __shared__ unsigned long long s_sum[LDS_HIST ? kHistLds : 1];
+    __shared__ uint32_t s_exp[LDS_HIST ? kHistLds : 1];
+    __shared__ uint32_t s_cnt[LDS_HIST ? kHistLds : 1];
+    const HistSlots hs{s_sum, s_exp, s_cnt};
+    const int tid = threadIdx.x;
     const int d = st.dim, ni = st.n_intervals, W = d * ni;
     long long total = warmup ? n_warm : st.offsets[st.n_cubes];
     if (!warmup && total > st.max_samples) total = st.max_samples;
     const long long s0 = (long long)blockIdx.x * kSB;
     if (s0 >= total) return;
-    if (LDS_HIST && hist)
-        for (int i = tid; i < kWavesPerBlock * W; i += kBlock) {
-            s_hw[(i / W) * kHistLds + (i % W)] = 0.0;
-            s_hc[(i / W) * kHistLds + (i % W)] = 0u;
-        }
+    if (LDS_HIST && hist) hist_zero(hs, W);
     T vol = T(1);
     for (int k = 0; k < d; ++k) vol = vol * (domain[2 * k + 1] - domain[2 * k]);
     __syncthreads();
+    auto jf_of = [&](long long s) { return (double)(fv[s] * vol) * jacs[s]; };
+    auto id_of = [&](long long s, int k) {
+        const int id = idsg[s * d + k];
+        return id >= ni ? ni - 1 : id;
+    };
     for (int j = 0; j < kSPT; ++j) {
         const long long s = s0 + (long long)j * kBlock + tid;
         const bool active = s < total;
         double jf = 0.0;
         if (active) {
-            const T f = fv[s] * vol;
-            jf = (double)f * jacs[s];
+            jf = jf_of(s);
             if (!warmup) st.jf[s] = jf;
         }
         if (hist) {
             const double jf2 = jf * jf;
             for (int k = 0; k < d; ++k) {
-                int id = active ? idsg[s * d + k] : -1;
-                if (id >= ni) id = ni - 1;
-                if (LDS_HIST)
-                    wave_hist_add(s_hw + wave * kHistLds, s_hc + wave * kHistLds, k, ni, id, jf2);
-                else
-                    wave_hist_add_global(st.weights, st.counts, k, ni, id, jf2);
+                if (LDS_HIST) {
+                    if (active) hist_pass1(hs, k * ni + id_of(s, k), jf2);
+                } else {
+                    wave_hist_add_global(st.weights, st.counts, k, ni, active ? id_of(s, k) : -1, jf2);
+                }
             }
         }
     }
     if (LDS_HIST && hist) {
         __syncthreads();
+        for (int j = 0; j < kSPT; ++j) {
+            const long long s = s0 + (long long)j * kBlock + tid;
+            if (s >= total) break;
+            const double jf = jf_of(s);
+            for (int k = 0; k < d; ++k) hist_pass2(hs, k * ni + id_of(s, k), jf * jf);
+        }
+        __syncthreads();
         for (int e = tid; e < W; e += kBlock) {
-            double w = 0.0;
-            uint32_t c = 0;
-            for (int wv = 0; wv < kWavesPerBlock; ++wv) {
-                w += s_hw[wv * kHistLds + e];
-                c += s_hc[wv * kHistLds + e];
-            }
-            st.hist_w[(long long)blockIdx.x * W + e] = w;
-            st.hist_c[(long long)blockIdx.x * W + e] = c;
+            st.hist_w[(long long)blockIdx.x * W + e] = hist_value(hs, e);
+            st.hist_c[(long long)blockIdx.x * W + e] = hs.cnt[e];
         }
     }
 }
@@ -628,40 +713,35 @@ __global__ __launch_bounds__(kBlock) void k_vegas_consume(tq_vegas_state st, con
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_vegas_accum_given(tq_vegas_state st, const T* y, const T* jf2v,
                                                               long long n) {
-    __shared__ double s_hw[kWavesPerBlock * kHistLds];
-    __shared__ uint32_t s_hc[kWavesPerBlock * kHistLds];
-    const int tid = threadIdx.x, wave = tid >> 6;
+    __shared__ unsigned long long s_sum[kHistLds];
+    __shared__ uint32_t s_exp[kHistLds];
+    __shared__ uint32_t s_cnt[kHistLds];
+    const HistSlots hs{s_sum, s_exp, s_cnt};
+    const int tid = threadIdx.x;
     const int d = st.dim, ni = st.n_intervals, W = d * ni;
     const long long s0 = (long long)blockIdx.x * kSB;
-    for (int i = tid; i < kWavesPerBlock * W; i += kBlock) {
-        s_hw[(i / W) * kHistLds + (i % W)] = 0.0;
-        s_hc[(i / W) * kHistLds + (i % W)] = 0u;
-    }
+    hist_zero(hs, W);
     __syncthreads();
     const T tni = (T)ni;
-    for (int j = 0; j < kSPT; ++j) {
-        const long long s = s0 + (long long)j * kBlock + tid;
-        const bool active = s < n;
-        const double v = active ? (double)jf2v[s] : 0.0;
-        for (int k = 0; k < d; ++k) {
-            int id = -1;
-            if (active) {
-                id = (int)floor(y[s * d + k] * tni);
-                id = id < 0 ? 0 : (id >= ni ? ni - 1 : id);
+    auto id_of = [&](long long s, int k) {
+        int id = (int)floor(y[s * d + k] * tni);
+        return id < 0 ? 0 : (id >= ni ? ni - 1 : id);
+    };
+    for (int pass = 1; pass <= 2; ++pass) {
+        for (int j = 0; j < kSPT; ++j) {
+            const long long s = s0 + (long long)j * kBlock + tid;
+            if (s >= n) break;
+            const double v = (double)jf2v[s];
+            for (int k = 0; k < d; ++k) {
+                if (pass == 1) hist_pass1(hs, k * ni + id_of(s, k), v);
+                else hist_pass2(hs, k * ni + id_of(s, k), v);
             }
-            wave_hist_add(s_hw + wave * kHistLds, s_hc + wave * kHistLds, k, ni, id, v);
         }
+        __syncthreads();
     }
-    __syncthreads();
     for (int e = tid; e < W; e += kBlock) {
-        double w = 0.0;
-        uint32_t c = 0;
-        for (int wv = 0; wv < kWavesPerBlock; ++wv) {
-            w += s_hw[wv * kHistLds + e];
-            c += s_hc[wv * kHistLds + e];
-        }
-        st.hist_w[(long long)blockIdx.x * W + e] = w;
-        st.hist_c[(long long)blockIdx.x * W + e] = c;
+        st.hist_w[(long long)blockIdx.x * W + e] = hist_value(hs, e);
+        st.hist_c[(long long)blockIdx.x * W + e] = hs.cnt[e];
     }
 }
 
@@ -1310,8 +1390,8 @@ static int vegas_iterate(const tq_vegas_state* st, const tq_integrand* f, const 
     A.p = make_params<T>(*f);
     A.u_in = nullptr;
     const long long nb = sample_blocks(st, warmup, n_warm);
-    // wave-private histograms: (FP64 weight + uint32 count) per bin and wave
-    const size_t hist_lds = (size_t)kWavesPerBlock * W * (sizeof(double) + sizeof(uint32_t));
+    // exact block histogram: 64-bit sum, exponent and count per bin
+    const size_t hist_lds = (size_t)W * (8 + 4 + 4);
     TQ_DISPATCH_FAMILY(f->family, {
         if (lds_hist && st->dim <= 8)
             hipLaunchKernelGGL((k_vegas_sample<T, FAM, MODE_FUSED, true, 8>), dim3((unsigned)nb), dim3(kBlock),
