@@ -90,7 +90,12 @@ int tq_device_count(void);
  * _apply_composite_rule collapse (trapezoid.py:28-37, simpson.py:30-45,
  * boole.py:30-48, gaussian.py:132-144) for built-in integrands.
  * domain: device [dim][2] of dtype.  out: device scalar of dtype.
- * out_f64 (optional, may be NULL): device scalar receiving the FP64 value. */
+ * out_f64 (optional, may be NULL): device scalar receiving the FP64 value.
+ * workspace: tq_grid_workspace_size() bytes; its first 256 bytes are a
+ * completion counter (the last workgroup to finish sums the partials, so one
+ * launch does the whole integral) that must be zero before the first call
+ * with that workspace -- clear it once after allocating -- and that every
+ * call leaves zero. */
 size_t tq_grid_workspace_size(const tq_grid_desc* g);
 int tq_grid_integrate(const tq_grid_desc* g, const tq_integrand* f, const void* domain,
                       void* workspace, size_t workspace_bytes, void* out, double* out_f64,
@@ -98,11 +103,12 @@ int tq_grid_integrate(const tq_grid_desc* g, const tq_integrand* f, const void* 
 
 /* tq_grid_integrate without a workspace, for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops, harnesses
- * that forbid hipMalloc).  No counterpart in the reference.  The block
- * partials live in a static device array of 1 MiB per device, each device's
+ * that forbid hipMalloc).  No counterpart in the reference.  The
+ * completion counter and block partials live in a static device array of
+ * 1 MiB per device (zero from load; every call leaves the counter zero), each device's
  * address looked up once on that device's first call; the call allocates
- * nothing.  Grids that need more than 1 MiB of partials (more than 131072
- * workgroups, i.e. tq_grid_workspace_size() > 1 MiB) return TQ_E_WORKSPACE.
+ * nothing.  Grids with tq_grid_workspace_size() > 1 MiB (more than 131040
+ * workgroups) return TQ_E_WORKSPACE.
  * Results equal tq_grid_integrate's with a workspace of
  * tq_grid_workspace_size() bytes.  Every call on a device shares that array:
  * calls that overlap on one device (different streams, or host threads)
@@ -247,14 +253,16 @@ int tq_batch_plan(const tq_batch_item* items, int64_t count, int32_t dtype, int6
  * the general kernel.  Each class's tiles form 16 interleaved queues (tile
  * t in queue t mod 16); a wave drains its home queue, then the others.
  * gl_nodes / gl_weights: every Gauss-Legendre item's table at its
- * gl_offset; gl_pairs32 (optional, F32 batches only, else ignored): the
- * same tables as interleaved FP32 (node, weight) pairs, pair k at
- * [2 (gl_offset + k)], read by one-axis items instead of the FP64 tables. */
+ * gl_offset, gl_len entries each; gl_pairs32 (optional, F32 batches only,
+ * else ignored): the same tables as interleaved FP32 (node, weight) pairs,
+ * pair k at [2 (gl_offset + k)], read by one-axis items instead of the FP64
+ * tables.  A Gauss-Legendre item whose gl_offset + n exceeds gl_len (or with
+ * NULL tables) reads no table and its result is NaN. */
 int tq_batch_run(int32_t dtype, const void* desc, int64_t count, const tq_batch_tile* tiles, int64_t n_tiles,
                  const int64_t* tile_begin, const int64_t* tile_count, const double* gl_nodes,
-                 const double* gl_weights, const float* gl_pairs32, double* tile_partials, unsigned int* counter,
-                 int64_t counter_words, void* out, double* out64, int32_t blocks, const int64_t* class_offsets,
-                 void* stream);
+                 const double* gl_weights, const float* gl_pairs32, int64_t gl_len, double* tile_partials,
+                 unsigned int* counter, int64_t counter_words, void* out, double* out64, int32_t blocks,
+                 const int64_t* class_offsets, void* stream);
 
 /* ---- differentiation ----------------------------------------------------
  * Parameter sweep of the Gaussian peak f = exp(-sum a_k^2 (x_k-u_k)^2) over B

@@ -117,6 +117,26 @@ def test_fp32_gl_pairs_vs_fp64_tables(dev):
             assert abs(v - ref) <= 1e-6 * abs(ref), (it["n"], v, ref)
 
 
+def test_gl_tables_shorter_than_an_item_give_nan(dev):
+    """A Gauss-Legendre item whose n nodes at gl_offset do not all lie in the
+    gl_len entries passed to tq_batch_run reads no table and gives NaN; the
+    other items are unchanged.  (The tables really hold 150 entries, so no
+    read leaves the allocation whatever the kernel does.)"""
+    f = tq.GaussianPeak([3.0], [0.4])
+    items = [{"rule": "gauss_legendre", "fn": f, "dim": 1, "n": 100, "domain": [[0.0, 1.0]]},
+             {"rule": "gauss_legendre", "fn": f, "dim": 1, "n": 50, "domain": [[0.0, 1.0]]},
+             {"rule": "trapezoid", "fn": f, "dim": 1, "n": 333, "domain": [[0.0, 1.0]]}]
+    for dtype in (torch.float32, torch.float64):
+        b = Batch(items, dtype=dtype)
+        assert b.gl_x.numel() == 150
+        full = b.run()[1].clone()
+        assert torch.isfinite(full).all()
+        b.gl_x, b.gl_w = b.gl_x[:120], b.gl_w[:120]  # the second table (offset 100, n 50) ends at 150
+        short = b.run()[1].clone()
+        assert float(short[0]) == float(full[0]) and float(short[2]) == float(full[2])
+        assert math.isnan(float(short[1]))
+
+
 def test_batch_graph_replay(dev):
     items = genz_batch(128, seed=4)
     b = Batch(items, dtype=torch.float32)

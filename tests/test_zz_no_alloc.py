@@ -30,7 +30,7 @@ def _run(L, entry, g, desc, dom, dev):
                                          _lib.ptr(out64), stream)
     else:
         nbytes = L.tq_grid_workspace_size(ctypes.byref(g))
-        ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        ws = torch.zeros(nbytes, dtype=torch.uint8, device=dev)  # the completion counter starts at zero
         rc = L.tq_grid_integrate(ctypes.byref(g), ctypes.byref(desc), _lib.ptr(dom), _lib.ptr(ws), nbytes,
                                  _lib.ptr(out), _lib.ptr(out64), stream)
     assert rc == 0, L.tq_last_error()

@@ -143,8 +143,8 @@ SIGNATURES = {
     ),
     "tq_batch_plan_sizes": (ctypes.c_int, [_P, _I64, _I32, _I64, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "tq_batch_plan": (ctypes.c_int, [_P, _I64, _I32, _I64, _P, _P, _P, _P, _P]),
-    "tq_batch_run": (ctypes.c_int, [_I32, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _I32, _P,
-                                    _P]),
+    "tq_batch_run": (ctypes.c_int, [_I32, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _I64, _P, _P, _I64, _P, _P, _I32,
+                                    _P, _P]),
     "tq_mc_workspace_size": (_SZ, [ctypes.POINTER(McDesc)]),
     "tq_mc_integrate": (
         ctypes.c_int,
@@ -290,7 +290,9 @@ def dtype_code(dtype):
 
 
 class _Workspace:
-    """Per (device, stream) scratch buffers grown on demand.
+    """Per (device, stream) scratch buffers grown on demand, zeroed when
+    allocated (the fused grid kernel keeps a completion counter in the first
+    bytes that must start at zero; every launch leaves it zero).
 
     The kernels of one call finish reading a workspace before any later call
     on the same stream writes it, so reuse within a stream is safe.
@@ -308,7 +310,7 @@ class _Workspace:
             if buf is None or buf.numel() < nbytes:
                 size = max(int(nbytes), 256)
                 size = 1 << (size - 1).bit_length()
-                buf = torch.empty(size, dtype=torch.uint8, device=device)
+                buf = torch.zeros(size, dtype=torch.uint8, device=device)
                 self._bufs[key] = buf
             return buf
 

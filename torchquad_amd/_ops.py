@@ -46,7 +46,7 @@ def grid_workspace(rule, integrand, dim, n, domain, gl=None, lane_axes=0, items_
     (for callers that must own theirs, e.g. captured graphs)."""
     g, _ = _fused_grid_desc(rule, integrand, dim, n, domain.dtype, gl, lane_axes, items_per_wave)
     nbytes = lib().tq_grid_workspace_size(ctypes.byref(g))
-    return torch.empty(max(int(nbytes), 8), dtype=torch.uint8, device=domain.device)
+    return torch.zeros(max(int(nbytes), 8), dtype=torch.uint8, device=domain.device)
 
 
 def grid_integrate(rule, integrand, dim, n, domain, gl=None, lane_axes=0, items_per_wave=0, out_f64=None, ws=None):

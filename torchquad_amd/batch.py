@@ -188,7 +188,7 @@ class Batch:
         """Launch the batch on torch's current stream; returns (out, out64)."""
         call("tq_batch_run", self.dt, ptr(self.desc), self.count, ptr(self.tiles), self.n_tiles,
              ptr(self.tile_begin), ptr(self.tile_count), ptr(self.gl_x), ptr(self.gl_w), ptr(self.gl_p32),
-             ptr(self.tile_part),
+             self.gl_x.numel() if self.gl_x is not None else 0, ptr(self.tile_part),
              ptr(self.counter), self.counter.numel(), ptr(self.out), ptr(self.out64), self.blocks,
              self.class_offsets.ctypes.data_as(ctypes.c_void_p), stream_ptr(self.device))
         return self.out, self.out64

@@ -135,7 +135,8 @@ __device__ __forceinline__ double batch_tile(const BatchDev<T>& D, const Axis* a
 template <typename T, int REGION>
 __global__ __launch_bounds__(kBlock) void k_batch(const BatchDev<T>* descs, const tq_batch_tile* tiles,
                                                   long long n_tiles, const double* glx, const double* glw,
-                                                  const float* glp32, double* tile_part, unsigned int* counter) {
+                                                  const float* glp32, long long gl_len, double* tile_part,
+                                                  unsigned int* counter) {
     constexpr int NT = BatchTab<T>::N;
     constexpr int RB = REGION > 0 ? REGION : (int)(2 * NT * sizeof(T));
     __shared__ Axis s_ax[kWavesPerBlock][TQ_MAX_DIM];
@@ -162,18 +163,30 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchDev<T>* descs, cons
         const long long t = q + (long long)i * kQueues;
         const tq_batch_tile tile = tiles[t];
         const BatchDev<T>& D = descs[tile.item];
-        const double* gx = glx ? glx + D.gl_offset : nullptr;
-        const double* gw = glw ? glw + D.gl_offset : nullptr;
-        // FP32 (node, weight) pairs for one-axis Gauss-Legendre rows of FP32 batches
-        const float* gp = (glp32 && sizeof(T) == 4 && D.m == 0) ? glp32 + 2 * D.gl_offset : nullptr;
-        for (int k = lane; k < D.d; k += kWave) s_ax[wave][k] = D.ax[k];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        const bool table = REGION == 0 && D.m > 0 && D.L >= 0 && D.nl * tab_stride(D.fam) <= 2 * NT;
-        double acc = batch_tile<T, REGION>(D, s_ax[wave], region, table, gx, gw, gp, tile.lane_wave,
-                                           tile.lane_wave_end, tile.t0, tile.t1);
-        acc = wave_sum(acc);
+        // A Gauss-Legendre integral whose n nodes at gl_offset are not all
+        // inside the gl_len-entry tables reads nothing and yields NaN
+        // (tq_batch_plan checks items without the tables; this is where
+        // their length is known).  The skip is a uniform branch around the
+        // work that rejoins the common store and wave barrier below: no
+        // early continue, whose divergent lane-0 store let the compiler
+        // run the next claim without lane 0, the lane that claims.
+        const bool gl_ok = D.rule != TQ_RULE_GAUSS_LEGENDRE ||
+                           (glx && glw && D.gl_offset >= 0 && D.gl_offset + D.n <= gl_len);
+        double acc = __longlong_as_double(0x7ff8000000000000ll);
+        if (gl_ok) {
+            const double* gx = glx ? glx + D.gl_offset : nullptr;
+            const double* gw = glw ? glw + D.gl_offset : nullptr;
+            // FP32 (node, weight) pairs for one-axis Gauss-Legendre rows of FP32 batches
+            const float* gp = (glp32 && sizeof(T) == 4 && D.m == 0) ? glp32 + 2 * D.gl_offset : nullptr;
+            for (int k = lane; k < D.d; k += kWave) s_ax[wave][k] = D.ax[k];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const bool table = REGION == 0 && D.m > 0 && D.L >= 0 && D.nl * tab_stride(D.fam) <= 2 * NT;
+            acc = batch_tile<T, REGION>(D, s_ax[wave], region, table, gx, gw, gp, tile.lane_wave,
+                                        tile.lane_wave_end, tile.t0, tile.t1);
+            acc = wave_sum(acc);
+        }
         if (lane == 0) tile_part[t] = acc;
         // all lanes are done with this tile's LDS before the next one writes it
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -371,21 +384,21 @@ static int plan_batch(const tq_batch_item* items, long long count, long long til
 
 template <typename T, int REGION>
 static int launch_class(const void* desc, const tq_batch_tile* tiles, long long n, const double* glx,
-                        const double* glw, const float* glp32, double* tile_part, unsigned int* counter, int blocks,
-                        hipStream_t s) {
+                        const double* glw, const float* glp32, long long gl_len, double* tile_part,
+                        unsigned int* counter, int blocks, hipStream_t s) {
     if (n <= 0) return TQ_OK;
     long long need = (n + kWavesPerBlock - 1) / kWavesPerBlock;
     if (blocks > need) blocks = (int)need;
     hipLaunchKernelGGL((k_batch<T, REGION>), dim3(blocks), dim3(kBlock), 0, s, (const BatchDev<T>*)desc, tiles, n,
-                       glx, glw, glp32, tile_part, counter);
+                       glx, glw, glp32, gl_len, tile_part, counter);
     return check_launch("k_batch");
 }
 
 template <typename T>
 static int run_batch(const void* desc, long long count, const tq_batch_tile* tiles, long long n_tiles,
                      const int64_t* tile_begin, const int64_t* tile_count, const double* glx, const double* glw,
-                     const float* glp32, double* tile_part, unsigned int* counter, void* out, double* out64, int blocks,
-                     const int64_t* class_offsets, hipStream_t s) {
+                     const float* glp32, long long gl_len, double* tile_part, unsigned int* counter, void* out,
+                     double* out64, int blocks, const int64_t* class_offsets, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(kBatchClasses * kQueues), 0, s, counter);
     int rc = check_launch("k_batch_reset");
     if (rc) return rc;
@@ -397,13 +410,13 @@ static int run_batch(const void* desc, long long count, const tq_batch_tile* til
             if (off[c] < 0 || off[c] > off[c + 1] || off[kBatchClasses] != n_tiles)
                 return fail(TQ_E_ARG, "class_offsets do not partition the tiles");
     }
-    rc = launch_class<T, kRegionSmall>(desc, tiles + off[0], off[1] - off[0], glx, glw, glp32, tile_part + off[0],
-                                       counter, blocks, s);
+    rc = launch_class<T, kRegionSmall>(desc, tiles + off[0], off[1] - off[0], glx, glw, glp32, gl_len,
+                                       tile_part + off[0], counter, blocks, s);
     if (!rc)
-        rc = launch_class<T, kRegionBig>(desc, tiles + off[1], off[2] - off[1], glx, glw, glp32, tile_part + off[1],
-                                         counter + kQueues * kQueueStride, blocks, s);
+        rc = launch_class<T, kRegionBig>(desc, tiles + off[1], off[2] - off[1], glx, glw, glp32, gl_len,
+                                         tile_part + off[1], counter + kQueues * kQueueStride, blocks, s);
     if (!rc)
-        rc = launch_class<T, 0>(desc, tiles + off[2], off[3] - off[2], glx, glw, glp32, tile_part + off[2],
+        rc = launch_class<T, 0>(desc, tiles + off[2], off[3] - off[2], glx, glw, glp32, gl_len, tile_part + off[2],
                                 counter + 2 * kQueues * kQueueStride, blocks, s);
     if (rc) return rc;
     const long long fb = (count + kWavesPerBlock - 1) / kWavesPerBlock;
@@ -448,20 +461,24 @@ int tq_batch_plan(const tq_batch_item* items, int64_t count, int32_t dtype, int6
 
 int tq_batch_run(int32_t dtype, const void* desc, int64_t count, const tq_batch_tile* tiles, int64_t n_tiles,
                  const int64_t* tile_begin, const int64_t* tile_count, const double* gl_nodes,
-                 const double* gl_weights, const float* gl_pairs32, double* tile_partials, unsigned int* counter,
-                 int64_t counter_words, void* out, double* out64, int32_t blocks, const int64_t* class_offsets,
-                 void* stream) {
+                 const double* gl_weights, const float* gl_pairs32, int64_t gl_len, double* tile_partials,
+                 unsigned int* counter, int64_t counter_words, void* out, double* out64, int32_t blocks,
+                 const int64_t* class_offsets, void* stream) {
     if (!desc || !tiles || !tile_begin || !tile_count || !tile_partials || !counter)
         return fail(TQ_E_NULL, "NULL argument");
+    if (gl_len < 0) return fail(TQ_E_ARG, "gl_len must be >= 0");
+    if (gl_len > 0 && (!gl_nodes || !gl_weights)) return fail(TQ_E_NULL, "gl_len > 0 with NULL node/weight tables");
     if (counter_words < TQ_BATCH_COUNTER_WORDS) return fail(TQ_E_WORKSPACE, "counter needs TQ_BATCH_COUNTER_WORDS words");
     if (count < 0 || n_tiles < 0 || n_tiles > 0xffffffffLL) return fail(TQ_E_ARG, "count/n_tiles out of range");
     if (!out && !out64) return fail(TQ_E_NULL, "no output pointer");
     if (dtype == TQ_F32)
         return run_batch<float>(desc, count, tiles, n_tiles, tile_begin, tile_count, gl_nodes, gl_weights,
-                                gl_pairs32, tile_partials, counter, out, out64, blocks, class_offsets, as_stream(stream));
+                                gl_pairs32, gl_len, tile_partials, counter, out, out64, blocks, class_offsets,
+                                as_stream(stream));
     if (dtype == TQ_F64)
         return run_batch<double>(desc, count, tiles, n_tiles, tile_begin, tile_count, gl_nodes, gl_weights,
-                                 nullptr, tile_partials, counter, out, out64, blocks, class_offsets, as_stream(stream));
+                                 nullptr, gl_len, tile_partials, counter, out, out64, blocks, class_offsets,
+                                 as_stream(stream));
     return fail(TQ_E_ARG, "dtype must be TQ_F32 or TQ_F64");
 }
 

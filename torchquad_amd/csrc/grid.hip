@@ -29,8 +29,17 @@ struct GridArgs {
     const double* glx;
     const double* glw;
     double* partials;
+    unsigned int* done;  // blocks finished; zero between launches
+    T* out;
+    double* out64;
+    double c0;
     Params<T> p;
 };
+
+// Workspace of the fused kernel: a completion counter in the first
+// kWsHeader bytes (zero before the first launch, and every launch leaves it
+// zero), then one FP64 partial per workgroup.
+constexpr size_t kWsHeader = 256;
 
 static int check_grid_desc(const tq_grid_desc* g, int max_dim) {
     if (!g) return fail(TQ_E_NULL, "grid descriptor is NULL");
@@ -132,25 +141,33 @@ __global__ __launch_bounds__(BW * kWave) void k_grid_fused(GridArgs<T> A) {
                                                A.glw, lw, t0, t1, s_rt, s_rw);
     }
     const double bs = block_sum(acc, s_red);
-    if (tid == 0) A.partials[blockIdx.x] = bs;
-}
-
-// Deterministic final sum of block partials and the per-axis scale factors.
-template <typename T>
-__global__ __launch_bounds__(1024) void k_grid_finalize(const double* partials, long long nparts,
-                                                         const T* domain, int d, int n, int rule,
-                                                         double c0, T* out, double* out64) {
-    __shared__ double s_red[1024 / kWave];
+    // The last block to finish sums every block's partial in a fixed order
+    // (independent of which block is last) and applies the per-axis scale:
+    // partial store, device-scope release, counter; the last block acquires
+    // before reading the partials and resets the counter for the next launch.
+    // Release writes this workgroup's partial back to device scope (no L2
+    // invalidate, which a full fence in every workgroup would add); only
+    // the last workgroup acquires.
+    __shared__ bool s_last;
+    if (tid == 0) {
+        A.partials[blockIdx.x] = bs;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        s_last = __hip_atomic_fetch_add(A.done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     double v = 0.0;
-    for (long long i = threadIdx.x; i < nparts; i += blockDim.x) v += partials[i];
+    for (unsigned i = tid; i < gridDim.x; i += NT) v += A.partials[i];
     v = block_sum(v, s_red);
-    if (threadIdx.x == 0) {
-        double scale = c0;
+    if (tid == 0) {
+        double scale = A.c0;
         for (int k = 0; k < d; ++k)
-            scale *= axis_scale((double)domain[2 * k], (double)domain[2 * k + 1], n, rule);
+            scale *= axis_scale((double)A.domain[2 * k], (double)A.domain[2 * k + 1], n, rule);
         const double r = v * scale;
-        if (out) *out = (T)r;
-        if (out64) *out64 = r;
+        if (A.out) *A.out = (T)r;
+        if (A.out64) *A.out64 = r;
+        *A.done = 0u;
     }
 }
 
@@ -236,17 +253,21 @@ static int launch_grid(const tq_grid_desc* gd, const tq_integrand* f, const void
     GridGeom G;
     int rc = make_geom(gd, G, fused_merge_cap(f->family, (int)sizeof(T)));
     if (rc) return rc;
-    const size_t need = (size_t)G.blocks * sizeof(double);
+    const size_t need = kWsHeader + (size_t)G.blocks * sizeof(double);
     if (!ws || ws_bytes < need) return fail(TQ_E_WORKSPACE, "grid workspace too small");
     GridArgs<T> A;
     A.g = G;
     A.domain = (const T*)domain;
     A.glx = gd->gl_nodes;
     A.glw = gd->gl_weights;
-    A.partials = (double*)ws;
+    A.done = (unsigned int*)ws;
+    A.partials = (double*)((unsigned char*)ws + kWsHeader);
+    A.out = (T*)out;
+    A.out64 = out64;
+    A.c0 = f->c0;
     A.p = make_params<T>(*f);
     const bool auto_ipw = gd->items_per_wave <= 0;
-    const long long max_blocks = (long long)(ws_bytes / sizeof(double));
+    const long long max_blocks = (long long)((ws_bytes - kWsHeader) / sizeof(double));
     TQ_DISPATCH_FAMILY(f->family, {
         const bool rowtab = G.use_table && G.L >= 1 && (fused_lds_bytes<T, FAM>(G, true)) <= kFusedLdsMax;
         const size_t shmem = (fused_lds_bytes<T, FAM>(G, rowtab));
@@ -262,11 +283,7 @@ static int launch_grid(const tq_grid_desc* gd, const tq_integrand* f, const void
         else
             (launch_fused<T, FAM, false, false, kWavesPerBlock>)(G, A, 0, auto_ipw, max_blocks, st);
     });
-    rc = check_launch("k_grid_fused");
-    if (rc) return rc;
-    hipLaunchKernelGGL((k_grid_finalize<T>), dim3(1), dim3(1024), 0, st, (const double*)ws,
-                       (long long)G.blocks, (const T*)domain, G.d, G.n, G.rule, f->c0, (T*)out, out64);
-    return check_launch("k_grid_finalize");
+    return check_launch("k_grid_fused");
 }
 
 // ---------------------------------------------------------------------------
@@ -544,9 +561,10 @@ static void points_backward_shape(int d, long long P, int* bd, long long* nb) {
 // of the array; host code reaches it only through the address that
 // hipGetSymbolAddress returns with that device current (the array's name is
 // a host-side stand-in, not a device address).  Every block writes its
-// partial before the finalize reads it, so nothing needs to start at zero.
+// partial before the last block reads it; the completion counter in the
+// first bytes starts at zero (static storage) and every launch leaves it zero.
 // ---------------------------------------------------------------------------
-constexpr size_t kNoAllocScratchBytes = (size_t)1 << 20;  // 131072 block partials
+constexpr size_t kNoAllocScratchBytes = (size_t)1 << 20;  // a counter and 131040 block partials
 constexpr int kMaxDevices = 64;
 
 static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
@@ -589,7 +607,7 @@ size_t tq_grid_workspace_size(const tq_grid_desc* g) {
     GridGeom G;
     if (make_geom(g, G)) return 0;
     const long long blocks = G.blocks > kFusedMinBlocks ? G.blocks : kFusedMinBlocks;
-    return (size_t)blocks * sizeof(double);
+    return kWsHeader + (size_t)blocks * sizeof(double);
 }
 
 int tq_grid_integrate(const tq_grid_desc* g, const tq_integrand* f, const void* domain, void* workspace,
