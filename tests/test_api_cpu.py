@@ -340,6 +340,25 @@ def test_noalloc_rejects_grids_beyond_its_scratch():
     assert L.tq_grid_integrate_noalloc(ctypes.byref(g), ctypes.byref(f), fake, fake, None, None) == -1
 
 
+def test_batch_run_rejects_bad_gl_tables_on_the_host():
+    """tq_batch_run checks the Gauss-Legendre table arguments before any
+    device call: a negative gl_len is TQ_E_ARG, a positive one with NULL
+    node/weight tables TQ_E_NULL."""
+    L = _lib.load_library()
+    fake = ctypes.c_void_p(256)
+    offs = np.zeros(4, dtype=np.int64)
+
+    def run(gx, gw, gl_len):
+        return L.tq_batch_run(0, fake, 1, fake, 1, fake, fake, gx, gw, None, gl_len, fake, fake,
+                              _lib.BATCH_COUNTER_WORDS, fake, fake, 0, offs.ctypes.data_as(ctypes.c_void_p), None)
+
+    assert run(fake, fake, -1) == -1
+    assert b"gl_len" in L.tq_last_error()
+    assert run(None, fake, 10) == -2
+    assert run(fake, None, 10) == -2
+    assert b"gl_len > 0" in L.tq_last_error()
+
+
 def test_gaussian_rule_classification():
     """Which Gaussian rules run on the kernels: hooks classified on probes
     (gaussian.py:13-25, tutorial.rst:1198-1236)."""
@@ -414,3 +433,76 @@ def test_param_batch_integrands_host_side():
     assert f.key() != tq.GaussianPeak(a + 1e-9, [0.1, 0.2]).key()
     with pytest.raises(ValueError, match="broadcast"):
         tq.GaussianPeak(np.ones((3, 2)), np.zeros((4, 2)))
+
+
+def test_serving_item_records_match_the_c_layout():
+    """numpy records of submit_many have tq_batch_item's layout; vectorised N
+    rounding equals the per-integral one; the Genz arrays carry the same
+    descriptors as the Genz items."""
+    from torchquad_amd.batch import n_per_axis
+    from torchquad_amd.serve import ITEM_DTYPE, n_per_axis_array
+    from torchquad_amd.workloads import genz_arrays, genz_batch
+
+    assert ITEM_DTYPE.itemsize == ctypes.sizeof(_lib.BatchItem)
+    for name, _ in _lib.BatchItem._fields_:
+        assert ITEM_DTYPE.fields[name][1] == getattr(_lib.BatchItem, name).offset, name
+    rng = np.random.default_rng(0)
+    rule = rng.integers(0, 4, 500)
+    dim = rng.integers(1, 7, 500)
+    N = np.maximum((10 ** rng.uniform(1.5, 6.0, 500)).astype(np.int64), 2**dim)
+    vec = n_per_axis_array(rule, dim, N)
+    assert vec.tolist() == [n_per_axis(int(r), int(d), int(m)) for r, d, m in zip(rule, dim, N)]
+    items = genz_batch(50, seed=3)
+    arr = genz_arrays(50, seed=3)
+    for i, it in enumerate(items):
+        d = it["fn"].descriptor(it["dim"])
+        assert arr["family"][i] == d.family and arr["c"][i].tolist() == [d.c0, d.c1, d.c2, d.c3]
+        assert arr["a"][i].tolist() == list(d.a) and arr["u"][i].tolist() == list(d.u)
+
+
+class _HostOnlyServer:
+    """The submission half of IntegrationServer without a device: requests
+    are packed and kept, nothing is launched."""
+
+    def __new__(cls):
+        from torchquad_amd.serve import IntegrationServer
+
+        class _S(IntegrationServer):
+            def __init__(self):
+                self.dt, self.tile_points, self.hbm_budget, self.queued = _lib.TQ_F32, 4096, 1 << 40, []
+
+            def _enqueue(self, r):
+                self.queued.append(r)
+
+        return _S()
+
+
+def test_submit_packing_on_the_host():
+    """submit_grid's record equals the field-by-field tq_batch_item; submit_many
+    takes its count from any per-integral argument (here only n) and packs
+    the same records as one submit_grid per integral."""
+    from torchquad_amd.serve import ITEM_DTYPE
+
+    s = _HostOnlyServer()
+    f = tq.GaussianPeak([3.0, 2.0, 1.0], [0.4, 0.5, 0.6])
+    dom = [[0.0, 1.0], [0.1, 0.9], [0.2, 0.7]]
+    s.submit_grid("simpson", f, 3, N=100000, domain=dom)
+    ref = np.zeros(1, ITEM_DTYPE)
+    d = f.descriptor(3)
+    ref["rule"], ref["family"], ref["dim"], ref["n"] = _lib.RULE_SIMPSON, d.family, 3, 45
+    ref["domain"][0, :6] = np.asarray(dom).reshape(-1)
+    ref["c0"], ref["c1"], ref["c2"], ref["c3"] = d.c0, d.c1, d.c2, d.c3
+    ref["a"][0] = np.ctypeslib.as_array(d.a)
+    ref["u"][0] = np.ctypeslib.as_array(d.u)
+    assert s.queued[0].items.tobytes() == ref.tobytes()
+    assert s.queued[0].gl_n.tolist() == [0]
+
+    ns = [33, 2000, 33, 2500]
+    g = tq.GaussianPeak([3.0], [0.4])
+    s.submit_many(_lib.RULE_GAUSS_LEGENDRE, _lib.FAM_GAUSSIAN_PEAK, 1, [[0.0, 1.0]], [3.0], [0.4], n=ns)
+    many = s.queued[-1]
+    assert len(many.items) == 4 and many.gl_n.tolist() == ns and many.gl_unique.tolist() == [33, 2000, 2500]
+    for n in ns:
+        s.submit_grid("gauss_legendre", g, 1, n=n, domain=[[0.0, 1.0]])
+    singles = np.concatenate([r.items for r in s.queued[-4:]])
+    assert many.items.tobytes() == singles.tobytes()

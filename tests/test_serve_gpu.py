@@ -7,6 +7,7 @@ import pytest
 import torch
 
 import torchquad_amd as tq
+from torchquad_amd import _lib
 from torchquad_amd.batch import integrate_batch
 from torchquad_amd.serve import AdmissionError, CancelledError, IntegrationServer, latency_stats, serve_open_loop
 from torchquad_amd.workloads import genz_batch
@@ -141,16 +142,54 @@ def test_threaded_clients(dev):
     assert st["count"] == len(items) and st["p99_ms"] >= st["p50_ms"] > 0
 
 
-def test_gl_table_cache_eviction(dev):
-    """Tables evicted while a batch is packed (tiny cache) still serve every request."""
+def test_gl_table_arena_clears(dev):
+    """A table arena too small for the tables of one round (3000 entries for
+    6733) is cleared and refilled between batches and still serves every
+    request with the whole-batch values."""
     ns = [33, 2000, 33, 2500, 1500, 33, 2000, 700, 33]
     f = tq.GaussianPeak([3.0], [0.4])
     items = [{"rule": "gauss_legendre", "fn": f, "dim": 1, "n": n, "domain": [[0.0, 1.0]]} for n in ns]
-    server = IntegrationServer(dev, tile_points=TILE)
-    server.gl_cache_limit = 24 * 3000  # holds about one large table (FP64 nodes, weights + FP32 pairs)
+    # FP64 nodes and weights + FP32 pairs: 24 B per entry
+    server = IntegrationServer(dev, tile_points=TILE, gl_cache_limit=24 * 3000)
+    ref = integrate_batch(items, dtype=torch.float32, device=dev, tile_points=TILE).tolist()
     for rounds in range(3):
         tickets = [server.submit_grid(it["rule"], it["fn"], 1, n=it["n"], domain=it["domain"]) for it in items]
         server.run_until_idle(timeout=60)
-        ref = integrate_batch(items, dtype=torch.float32, device=dev, tile_points=TILE).tolist()
         assert [t.result() for t in tickets] == ref
-    assert server._gl_cache_bytes <= max(server.gl_cache_limit, 24 * 2500)
+        assert server._arena.used <= 3000 and server._arena.cap <= 3000
+    many = server.submit_many(_lib.RULE_GAUSS_LEGENDRE, _lib.FAM_GAUSSIAN_PEAK, 1, [[0.0, 1.0]], [3.0], [0.4], n=ns)
+    server.run_until_idle(timeout=60)
+    assert many.result().tolist() == ref
+
+
+def test_submit_many_matches_single_requests(dev):
+    """Packed arrays (submit_many) give the same values as one request per
+    integral and as a whole-batch run, bit for bit, also split into several
+    arrays that share batches with single requests."""
+    from torchquad_amd.workloads import genz_arrays
+
+    count = 300
+    items = genz_batch(count, seed=13)
+    arr = genz_arrays(count, seed=13)
+    ref = integrate_batch(items, dtype=torch.float32, device=dev, tile_points=TILE).tolist()
+    server = IntegrationServer(dev, dtype=torch.float32, tile_points=TILE)
+    t_all = server.submit_many(arr["rule"], arr["family"], arr["dim"], arr["domain"], arr["a"], arr["u"], arr["c"],
+                               N=arr["N"])
+    server.run_until_idle(timeout=60)
+    vals = t_all.result()
+    assert isinstance(vals, np.ndarray) and vals.shape == (count,)
+    assert vals.tolist() == ref
+    # three arrays and a few single requests in flight together
+    parts = [slice(0, 100), slice(100, 180), slice(180, 300)]
+    tickets = []
+    for k, sl in enumerate(parts):
+        tickets.append(server.submit_many(arr["rule"][sl], arr["family"][sl], arr["dim"][sl], arr["domain"][sl],
+                                          arr["a"][sl], arr["u"][sl], arr["c"][sl], N=arr["N"][sl]))
+        tickets.append(grid_submitter(items[7 * k])(server, None))
+    server.run_until_idle(timeout=60)
+    got = np.concatenate([tickets[0].result(), tickets[2].result(), tickets[4].result()])
+    assert got.tolist() == ref
+    assert [tickets[1].result(), tickets[3].result(), tickets[5].result()] == [ref[0], ref[7], ref[14]]
+    st = latency_stats([t_all])
+    assert st["count"] == count
+    assert server.reserved == 0

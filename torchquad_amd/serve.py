@@ -15,11 +15,21 @@ any thread).  Each scheduling round (:meth:`IntegrationServer.step`):
    persistent kernel plus the per-integral reduction (``tq_batch_run``),
    asynchronous download of the results -- on one of ``max_inflight``
    streams, so a batch is planned and launched while earlier ones still
-   run;
-4. advances each resident VEGAS request by one iteration on its own
-   stream.  Map and stratification state stay on the device between
-   iterations; the every-fifth-iteration abort check runs only once the
-   request's event shows the device has finished those iterations.
+   run.  Gauss-Legendre tables live at fixed offsets of one server-wide
+   arena, generated once per n on the table stream; a request waits in the
+   admitted set until the host has seen its tables' event, so batches carry
+   no per-table stream dependencies;
+4. advances each resident VEGAS request by one iteration (all VEGAS
+   requests share one stream; with the two batch streams and the table
+   stream the server stays within the device's four hardware queues).  Map
+   and stratification state stay on the device between iterations; the
+   every-fifth-iteration abort check runs only once the request's event
+   shows the device has finished those iterations.
+
+Requests come one at a time (:meth:`submit_grid`) or as packed arrays
+(:meth:`submit_many`: numpy records in the layout of ``tq_batch_item``, so a
+set of integrals costs one admission, one host plan and one launch, with no
+per-integral Python work).
 
 A grid integral's value depends only on the integral and the server's
 ``tile_points`` (its tiles and the fixed-order sum over them do not depend
@@ -39,9 +49,10 @@ simpson.py:14, boole.py:14, gaussian.py:29) and VEGAS.integrate
 """
 import ctypes
 import itertools
+import struct
 import threading
 import time
-from collections import OrderedDict, deque
+from collections import deque
 
 import numpy as np
 import torch
@@ -49,11 +60,36 @@ import torch
 from . import _lib
 from ._lib import NativeUnavailableError, call, lib, ptr
 from .batch import RULE_NAMES, n_per_axis
-from .gl_nodes import fp32_pairs, gauss_legendre
 from .integrands import BuiltinIntegrand
 
 ITEM_BYTES = ctypes.sizeof(_lib.BatchItem)
 TILE_BYTES = ctypes.sizeof(_lib.BatchTile)
+
+# tq_batch_item as a numpy record: requests are packed, concatenated and
+# uploaded as arrays, so a batch costs no per-request Python work.
+ITEM_DTYPE = np.dtype([
+    ("rule", np.int32), ("family", np.int32), ("dim", np.int32), ("n", np.int32), ("gl_offset", np.int64),
+    ("domain", np.float64, (2 * _lib.TQ_MAX_DIM,)), ("c0", np.float64), ("c1", np.float64), ("c2", np.float64),
+    ("c3", np.float64), ("a", np.float64, (_lib.TQ_MAX_DIM,)), ("u", np.float64, (_lib.TQ_MAX_DIM,)),
+])
+assert ITEM_DTYPE.itemsize == ITEM_BYTES
+_ITEM_HEAD = struct.Struct("<iiiiq")
+_ZERO_DOMAIN = bytes(16 * _lib.TQ_MAX_DIM)
+_INTEGRAND_PARAMS = _lib.Integrand.c0.offset  # c0..c3, a, u: the tail of both structs
+assert ctypes.sizeof(_lib.Integrand) - _INTEGRAND_PARAMS == ITEM_BYTES - ITEM_DTYPE.fields["c0"][1]
+
+
+def n_per_axis_array(rule, dim, N):
+    """Vectorised n_per_axis (the reference's N -> n rounding per rule)."""
+    rule, dim, N = (np.asarray(v, dtype=np.int64) for v in (rule, dim, N))
+    n = np.floor(N.astype(np.float64) ** (1.0 / dim) + 1e-8).astype(np.int64)
+    simpson = rule == _lib.RULE_SIMPSON
+    n = np.where(simpson & (n < 3), 3, np.where(simpson & (n % 2 != 1), n - 1, n))
+    boole = rule == _lib.RULE_BOOLE
+    n = np.where(boole & (n < 5), 5, np.where(boole & ((n - 1) % 4 != 0), n - ((n - 1) % 4), n))
+    if np.any((n < 2) & (rule != _lib.RULE_GAUSS_LEGENDRE)):
+        raise ValueError("N has to be > 1.")
+    return np.maximum(n, 1)
 
 
 class AdmissionError(RuntimeError):
@@ -65,12 +101,14 @@ class CancelledError(RuntimeError):
 
 
 class Ticket:
-    """Handle of one request: wait for its result, or cancel it."""
+    """Handle of one request: wait for its result, or cancel it.  A ticket of
+    ``submit_many`` covers ``count`` integrals and its result is an array."""
 
     _ids = itertools.count()
 
-    def __init__(self, kind, nbytes, evals):
+    def __init__(self, kind, nbytes, evals, count=1):
         self.id = next(Ticket._ids)
+        self.count = int(count)
         self.kind = kind
         self.nbytes = int(nbytes)
         self.evals = int(evals)
@@ -83,10 +121,13 @@ class Ticket:
         self.cancelled = False
         self.state = None  # VEGAS requests: the run's state_dict(), for a later warm start
         self.rule = None  # grid requests: rule code
-        self._event = threading.Event()
+        self._done = False
+        self._event = None  # made by the first waiter (most tickets are read once done)
+
+    _lock = threading.Lock()
 
     def done(self):
-        return self._event.is_set()
+        return self._done
 
     @property
     def latency(self):
@@ -95,7 +136,11 @@ class Ticket:
 
     def result(self, timeout=None):
         """The integral (float64 Python float); raises the request's error."""
-        if not self._event.wait(timeout):
+        with Ticket._lock:
+            if not self._done and self._event is None:
+                self._event = threading.Event()
+            ev = self._event
+        if not self._done and not ev.wait(timeout):
             raise TimeoutError(f"request {self.id} not finished within {timeout} s")
         if self.cancelled:
             raise CancelledError(f"request {self.id} was cancelled")
@@ -107,17 +152,119 @@ class Ticket:
         self.value = value
         self.error = error
         self.t_done = time.perf_counter()
-        self._event.set()
+        with Ticket._lock:
+            self._done = True
+            ev = self._event
+        if ev is not None:
+            ev.set()
 
 
 class _GridRequest:
-    __slots__ = ("ticket", "item", "gl_n", "gl_entry")
+    """One ticket's integrals: packed items and their Gauss-Legendre n (0 if none)."""
 
-    def __init__(self, ticket, item, gl_n):
+    __slots__ = ("ticket", "items", "gl_n", "gl_unique", "gl_seq", "single")
+
+    def __init__(self, ticket, items, gl_n, single):
         self.ticket = ticket
-        self.item = item
+        self.items = items
         self.gl_n = gl_n
-        self.gl_entry = None  # (nodes, weights, event, FP32 pairs) once its table is started
+        self.gl_unique = np.unique(gl_n[gl_n > 0]) if len(gl_n) > 1 else gl_n[gl_n > 0]
+        self.gl_seq = None  # table generation its tables wait for, once placed in the arena
+        self.single = single
+
+
+class _GlArena:
+    """Gauss-Legendre tables of every n the server has used, at fixed offsets
+    of one device buffer (float64 nodes and weights, plus FP32 (node, weight)
+    pairs for a float32 server).
+
+    A table is generated once, in place, on the arena's stream
+    (``tq_gauss_legendre``); each round's new tables are followed by one event,
+    and a request is launched only once the host has seen the event of its
+    tables fire, so batches read the arena with no stream dependency.  Tables
+    are never moved while batches may read them: the arena grows into new
+    buffers (the old ones stay referenced by the batches in flight) and is
+    cleared only while no batch is in flight.
+    """
+
+    def __init__(self, device, fp32, limit_bytes):
+        self.device = device
+        self.fp32 = fp32
+        self.per = 24 if fp32 else 16
+        self.limit = max(int(limit_bytes) // self.per, 1)
+        self.cap = 0
+        self.x = self.w = self.p = None
+        self.offset = {}
+        self.used = 0
+        self.seq = 0  # last generation issued
+        self.ready_seq = 0  # last generation seen complete
+        self.floor_seq = 0  # generation every table needs (the last regrowth copy)
+        self.pending = deque()
+        self.stream = torch.cuda.Stream(device=device)
+
+    def poll(self):
+        while self.pending and self.pending[0][0].query():
+            self.ready_seq = self.pending.popleft()[1]
+
+    def idle(self):
+        self.poll()
+        return not self.pending
+
+    def clear(self):
+        """Forget every table (only while no batch reads the arena and no
+        generation is in flight)."""
+        self.offset.clear()
+        self.used = 0
+
+    def _grow(self, need):
+        """Move to buffers that hold need more entries; the copy of the
+        resident tables is ordered before the next generation's event."""
+        cap = min(max(2 * self.cap, self.used + need, 1 << 16), max(self.limit, self.used + need))
+        x = torch.empty(cap, dtype=torch.float64, device=self.device)
+        w = torch.empty(cap, dtype=torch.float64, device=self.device)
+        p = torch.empty(2 * cap, dtype=torch.float32, device=self.device) if self.fp32 else None
+        self.stream.wait_stream(torch.cuda.current_stream(self.device))  # the allocations
+        if self.used:
+            with torch.cuda.stream(self.stream):
+                for old, new, k in ((self.x, x, 1), (self.w, w, 1), (self.p, p, 2)):
+                    if new is not None:
+                        new[: k * self.used].copy_(old[: k * self.used])
+                        old.record_stream(self.stream)
+            self.floor_seq = self.seq + 1
+        self.x, self.w, self.p, self.cap = x, w, p, cap
+
+    def place(self, ns):
+        """Start the tables of ns (distinct, > 0) that are not resident; returns
+        the generation the caller must see complete, or None if the tables
+        do not fit before the arena is cleared."""
+        new = [int(n) for n in ns if int(n) not in self.offset]
+        if new:
+            need = sum(new)
+            if self.used + need > self.cap:
+                if self.used + need > self.limit and self.used:
+                    return None
+                self._grow(need)
+            self.seq += 1
+            with torch.cuda.stream(self.stream):
+                for n in new:
+                    o = self.used
+                    call("tq_gauss_legendre", n, ptr(self.x[o:]), ptr(self.w[o:]), self.stream.cuda_stream)
+                    if self.p is not None:
+                        pairs = self.p[2 * o: 2 * (o + n)].view(n, 2)
+                        pairs[:, 0].copy_(self.x[o: o + n])
+                        pairs[:, 1].copy_(self.w[o: o + n])
+                    self.offset[n] = (o, self.seq)
+                    self.used += n
+                ev = torch.cuda.Event()
+                ev.record(self.stream)
+            self.pending.append((ev, self.seq))
+        return max([self.floor_seq] + [self.offset[int(n)][1] for n in ns])
+
+    def offsets(self, gl_n):
+        """Arena offsets of the tables of gl_n (entries > 0)."""
+        u, inv = np.unique(gl_n, return_inverse=True)
+        offs = np.fromiter((self.offset[int(n)][0] for n in u), dtype=np.int64, count=len(u))
+        return offs[inv]
 
 
 class _VegasRequest:
@@ -199,10 +346,16 @@ class IntegrationServer:
         max_inflight: grid batches in flight at once (one stream each).
         max_vegas: VEGAS requests resident at once.
         max_batch: grid requests per batch.
+        gl_cache_limit: bytes the Gauss-Legendre table arena may grow to
+            (tables of every n used so far, 24 B per node for a float32
+            server; cleared when full while nothing reads it).  The default
+            4 GiB holds about 1.8e8 nodes: a mix of one-axis integrals up
+            to n = 1e6 keeps every table resident, as the integrators'
+            own per-n caches do.
     """
 
-    def __init__(self, device=None, dtype=torch.float32, hbm_budget=None, tile_points=8192, max_inflight=3,
-                 max_vegas=8, max_batch=8192):
+    def __init__(self, device=None, dtype=torch.float32, hbm_budget=None, tile_points=8192, max_inflight=2,
+                 max_vegas=8, max_batch=8192, gl_cache_limit=4 << 30):
         if not torch.cuda.is_available():
             raise NativeUnavailableError("the serving loop runs on a ROCm GPU only")
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
@@ -223,11 +376,9 @@ class IntegrationServer:
         self._vegas = []
         self._slots = [_BatchSlot(self.device, dtype) for _ in range(int(max_inflight))]
         self._next_slot = 0
-        self._gl_cache = OrderedDict()
-        self._gl_cache_bytes = 0
-        self._gl_streams = []  # tables of new n are generated round-robin on these
-        self._gl_next = 0
-        self.gl_cache_limit = 256 << 20
+        self.gl_cache_limit = int(gl_cache_limit)
+        self._arena = _GlArena(self.device, self.dt == _lib.TQ_F32, self.gl_cache_limit)
+        self._vegas_shared_stream = None
         self._thread = None
         self._stop = threading.Event()
         self.completed = []
@@ -254,29 +405,83 @@ class IntegrationServer:
         if np.any(dom[:, 1] < dom[:, 0]):
             raise ValueError("integration_domain has invalid boundary values")
         d = fn.descriptor(dim)
-        item = _lib.BatchItem()
-        item.rule, item.family, item.dim, item.n = code, d.family, dim, int(n)
-        item.gl_offset = 0
-        for k in range(dim):
-            item.domain[2 * k] = dom[k, 0]
-            item.domain[2 * k + 1] = dom[k, 1]
-        item.c0, item.c1, item.c2, item.c3 = d.c0, d.c1, d.c2, d.c3
-        for k in range(_lib.TQ_MAX_DIM):
-            item.a[k] = d.a[k]
-            item.u[k] = d.u[k]
-        desc_bytes = ctypes.c_int64()
-        n_tiles = ctypes.c_int64()
-        call("tq_batch_plan_sizes", ctypes.byref(item), 1, self.dt, self.tile_points, ctypes.byref(desc_bytes),
-             ctypes.byref(n_tiles))
-        gl_n = int(n) if code == _lib.RULE_GAUSS_LEGENDRE else 0
-        gl_bytes = (24 if self.dt == _lib.TQ_F32 else 16) * gl_n  # FP64 table (+ FP32 pairs) in the batch
-        nbytes = (desc_bytes.value + n_tiles.value * (TILE_BYTES + 8) + 16 + 16 + ITEM_BYTES + gl_bytes)
-        t = Ticket("grid", nbytes, int(n) ** dim)
+        # the record's bytes: header, zero-padded domain, then c0..c3, a, u,
+        # which tq_integrand holds in the same layout after family and dim
+        items = np.frombuffer(_ITEM_HEAD.pack(code, d.family, dim, int(n), 0) + dom.tobytes()
+                              + _ZERO_DOMAIN[16 * dim:] + bytes(d)[_INTEGRAND_PARAMS:], dtype=ITEM_DTYPE)
+        gl_n = np.array([int(n) if code == _lib.RULE_GAUSS_LEGENDRE else 0], dtype=np.int64)
+        t = Ticket("grid", self._grid_bytes(items, gl_n), int(n) ** dim)
         t.rule = code
         if arrival is not None:
             t.t_arrival = arrival
-        self._enqueue(_GridRequest(t, item, gl_n))
+        self._enqueue(_GridRequest(t, items, gl_n, True))
         return t
+
+    def submit_many(self, rule, family, dim, domain, a, u, c=None, n=None, N=None, arrival=None):
+        """Queue many tensor-product integrals at once; returns one Ticket whose
+        result() is the float64 array of their values.
+
+        Arrays of length R (scalars broadcast): rule codes, family codes
+        (torchquad_amd._lib.FAM_*), dim, and n (points per axis) or N (total
+        points, rounded per rule like the integrators); domain [R, dim_max, 2],
+        a and u [R, k <= 16] and c [R, 4] (c0..c3) are the built-in family
+        parameters as in tq_integrand.  Items are packed with numpy and
+        admitted, planned and launched as one set.
+        """
+        rule = np.atleast_1d(np.asarray(rule, dtype=np.int64))
+        # R from every argument that may carry it (per-integral scalars, or
+        # the leading axis of domain / a / u / c)
+        lead = [rule.shape, np.shape(np.atleast_1d(family)), np.shape(np.atleast_1d(dim)),
+                np.shape(domain)[:-2], np.shape(a)[:-1], np.shape(u)[:-1]]
+        lead += [np.shape(np.atleast_1d(v)) for v in (n, N) if v is not None]
+        if c is not None:
+            lead.append(np.shape(c)[:-1])
+        R = int(np.broadcast_shapes(*[sh if len(sh) else (1,) for sh in lead])[0])
+        rule = np.broadcast_to(rule, (R,))
+        family = np.broadcast_to(np.asarray(family, dtype=np.int64), (R,))
+        dim = np.broadcast_to(np.asarray(dim, dtype=np.int64), (R,))
+        if np.any((dim < 1) | (dim > _lib.TQ_MAX_DIM)):
+            raise ValueError(f"dim must be in 1..{_lib.TQ_MAX_DIM}")
+        if np.any((family < 0) | (family >= 7)) or np.any((rule < 0) | (rule > 3)):
+            raise ValueError("unknown rule or family code")
+        if n is None:
+            if N is None:
+                raise ValueError("give N (total points) or n (points per axis)")
+            n = n_per_axis_array(rule, dim, np.broadcast_to(np.asarray(N, dtype=np.int64), (R,)))
+        n = np.broadcast_to(np.asarray(n, dtype=np.int64), (R,))
+        dom = np.asarray(domain, dtype=np.float64)
+        dom = np.broadcast_to(dom, (R,) + dom.shape[-2:])
+        if dom.shape[1] > _lib.TQ_MAX_DIM or dom.shape[2] != 2:
+            raise ValueError("domain must be [R, <=16, 2]")
+        axis_ok = np.arange(dom.shape[1])[None, :] < dim[:, None]
+        if np.any(axis_ok & (dom[:, :, 1] < dom[:, :, 0])):
+            raise ValueError("integration_domain has invalid boundary values")
+        items = np.zeros(R, dtype=ITEM_DTYPE)
+        items["rule"], items["family"], items["dim"], items["n"] = rule, family, dim, n
+        items["domain"][:, : 2 * dom.shape[1]] = np.where(axis_ok[:, :, None], dom, 0.0).reshape(R, -1)
+        cc = np.broadcast_to(np.asarray(c if c is not None else [1.0, 0.0, 0.0, 0.0], dtype=np.float64), (R, 4))
+        items["c0"], items["c1"], items["c2"], items["c3"] = cc[:, 0], cc[:, 1], cc[:, 2], cc[:, 3]
+        for name, v in (("a", a), ("u", u)):
+            v = np.asarray(v, dtype=np.float64)
+            v = np.broadcast_to(v, (R,) + v.shape[-1:])
+            items[name][:, : v.shape[1]] = v
+        gl_n = np.where(rule == _lib.RULE_GAUSS_LEGENDRE, n, 0).astype(np.int64)
+        evals = int(np.sum(n.astype(np.float64) ** dim))
+        t = Ticket("grid", self._grid_bytes(items, gl_n), evals, count=R)
+        if arrival is not None:
+            t.t_arrival = arrival
+        self._enqueue(_GridRequest(t, items, gl_n, False))
+        return t
+
+    def _grid_bytes(self, items, gl_n):
+        """Device memory a set of grid items reserves: descriptors, tiles,
+        partials and outputs.  (Gauss-Legendre tables live in the server's
+        arena, at most gl_cache_limit bytes beside the budget.)"""
+        desc_bytes = ctypes.c_int64()
+        n_tiles = ctypes.c_int64()
+        call("tq_batch_plan_sizes", ctypes.c_void_p(items.ctypes.data), len(items), self.dt, self.tile_points,
+             ctypes.byref(desc_bytes), ctypes.byref(n_tiles))
+        return int(desc_bytes.value + n_tiles.value * (TILE_BYTES + 8) + len(items) * (32 + ITEM_BYTES))
 
     def submit_vegas(self, fn, dim, N, domain=None, seed=0, N_intervals=None, max_iterations=20, eps_rel=0,
                      eps_abs=0, use_grid_improve=True, use_warmup=True, arrival=None, N_strat=None,
@@ -348,13 +553,17 @@ class IntegrationServer:
         finished = 0
         for slot in self._slots:
             if slot.busy() and slot.event.query():
-                vals = slot.host_out[: len(slot.requests)].tolist()
-                for r, v in zip(slot.requests, vals):
+                total = sum(len(r.items) for r in slot.requests)
+                vals = slot.host_out[:total].numpy()
+                pos = 0
+                for r in slot.requests:
+                    k = len(r.items)
                     self._release(r.ticket)
                     if not r.ticket.cancelled:
-                        r.ticket._finish(value=v)
+                        r.ticket._finish(value=float(vals[pos]) if r.single else vals[pos:pos + k].copy())
                     else:
                         r.ticket._finish()
+                    pos += k
                     self.completed.append(r.ticket)
                 finished += len(slot.requests)
                 slot.requests = None
@@ -363,6 +572,7 @@ class IntegrationServer:
         return finished
 
     def _admit(self):
+        admitted = sum(len(g.items) for g in self._admitted_grid)
         with self._lock:
             while self._waiting:
                 r = self._waiting[0]
@@ -370,7 +580,7 @@ class IntegrationServer:
                     break
                 if isinstance(r, _VegasRequest) and len(self._vegas) >= self.max_vegas:
                     break
-                if isinstance(r, _GridRequest) and len(self._admitted_grid) >= self.max_batch:
+                if isinstance(r, _GridRequest) and admitted and admitted + len(r.items) > self.max_batch:
                     break
                 self._waiting.popleft()
                 self.reserved += r.ticket.nbytes
@@ -378,48 +588,25 @@ class IntegrationServer:
                 r.ticket.t_admit = time.perf_counter()
                 if isinstance(r, _GridRequest):
                     self._admitted_grid.append(r)
+                    admitted += len(r.items)
                 else:
                     self._vegas.append(r)
-
-    def _gl_table(self, n):
-        """The n-point Gauss-Legendre table entry (nodes, weights, ready event,
-        FP32 (node, weight) pairs or None for a float64 server).
-
-        A new n starts its table on a side stream, so a request whose table
-        is still being generated waits for a later batch instead of delaying
-        the requests packed with it.  Requests keep a reference to their
-        entry, so evicting it from the cache (least recently used first,
-        beyond gl_cache_limit bytes) never takes a table from a request that
-        still needs it.
-        """
-        e = self._gl_cache.get(n)
-        if e is not None:
-            self._gl_cache.move_to_end(n)
-            return e
-        if not self._gl_streams:
-            self._gl_streams = [torch.cuda.Stream(device=self.device) for _ in range(4)]
-        stream = self._gl_streams[self._gl_next % len(self._gl_streams)]
-        self._gl_next += 1
-        with torch.cuda.stream(stream):
-            x, w = gauss_legendre(n, device=self.device)
-            p32 = fp32_pairs(x, w) if self.dt == _lib.TQ_F32 else None
-            ev = torch.cuda.Event()
-            ev.record(stream)
-        e = (x, w, ev, p32)
-        per = 24 if p32 is not None else 16
-        self._gl_cache[n] = e
-        self._gl_cache_bytes += per * n
-        while self._gl_cache_bytes > self.gl_cache_limit and len(self._gl_cache) > 1:
-            old_n, _ = self._gl_cache.popitem(last=False)
-            self._gl_cache_bytes -= per * old_n
-        return e
 
     def _launch_grid(self):
         if not self._admitted_grid:
             return
+        arena = self._arena
+        arena.poll()
+        idle = not any(s.busy() for s in self._slots)
         for r in self._admitted_grid:  # start tables even while every slot is busy
-            if r.gl_n and r.gl_entry is None:
-                r.gl_entry = self._gl_table(r.gl_n)
+            if r.gl_seq is None and len(r.gl_unique):
+                r.gl_seq = arena.place(r.gl_unique)
+                if r.gl_seq is None and idle and arena.idle():
+                    # over the arena's limit with nothing reading it: start afresh
+                    arena.clear()
+                    for q in self._admitted_grid:
+                        q.gl_seq = None
+                    r.gl_seq = arena.place(r.gl_unique)
         slot = None
         for k in range(len(self._slots)):
             s = self._slots[(self._next_slot + k) % len(self._slots)]
@@ -429,45 +616,28 @@ class IntegrationServer:
                 break
         if slot is None:
             return
-        reqs, later, ready = [], [], {}
+        reqs, later = [], []
         for r in self._admitted_grid:
-            if r.gl_n:
-                if not r.gl_entry[2].query():
-                    later.append(r)
-                    continue
-                ready.setdefault(r.gl_n, r.gl_entry)
-            reqs.append(r)
+            if len(r.gl_unique) and (r.gl_seq is None or max(r.gl_seq, arena.floor_seq) > arena.ready_seq):
+                later.append(r)
+            else:
+                reqs.append(r)
         self._admitted_grid = later
         if not reqs:
             return
-        count = len(reqs)
-        # Gauss-Legendre tables of the batch (ready), concatenated on the slot's stream
+        items = np.concatenate([r.items for r in reqs]) if len(reqs) > 1 else reqs[0].items.copy()
+        gl_n = np.concatenate([r.gl_n for r in reqs]) if len(reqs) > 1 else reqs[0].gl_n
+        count = len(items)
+        is_gl = gl_n > 0
+        gl_len = 0
         gl_x = gl_w = gl_p = None
-        ns = sorted({r.gl_n for r in reqs if r.gl_n})
-        offsets, tabs, off = {}, [], 0
-        for n in ns:
-            offsets[n] = off
-            tabs.append(ready[n])
-            off += n
+        if np.any(is_gl):
+            items["gl_offset"][is_gl] = arena.offsets(gl_n[is_gl])
+            gl_x, gl_w, gl_p, gl_len = arena.x, arena.w, arena.p, arena.used
         with torch.cuda.stream(slot.stream):
-            if ns:
-                for x, w, ev, p32 in tabs:
-                    slot.stream.wait_event(ev)
-                    x.record_stream(slot.stream)
-                    w.record_stream(slot.stream)
-                    if p32 is not None:
-                        p32.record_stream(slot.stream)
-                gl_x = torch.cat([t[0] for t in tabs])
-                gl_w = torch.cat([t[1] for t in tabs])
-                if self.dt == _lib.TQ_F32:
-                    gl_p = torch.cat([t[3] for t in tabs])
-                for r in reqs:
-                    if r.gl_n:
-                        r.item.gl_offset = offsets[r.gl_n]
             ih, idev = slot.items.reserve(count * ITEM_BYTES)
+            ih[: count * ITEM_BYTES].numpy()[:] = items.view(np.uint8)
             base = ih.data_ptr()
-            for i, r in enumerate(reqs):
-                ctypes.memmove(base + i * ITEM_BYTES, ctypes.addressof(r.item), ITEM_BYTES)
             desc_bytes = ctypes.c_int64()
             n_tiles = ctypes.c_int64()
             call("tq_batch_plan_sizes", ctypes.c_void_p(base), count, self.dt, self.tile_points,
@@ -485,8 +655,8 @@ class IntegrationServer:
             cd[: count * 8].copy_(ch[: count * 8], non_blocking=True)
             slot.ensure_outputs(count, nt)
             call("tq_batch_run", self.dt, ptr(dd), count, ptr(td), int(n_tiles.value), ptr(bd), ptr(cd), ptr(gl_x),
-                 ptr(gl_w), ptr(gl_p), ptr(slot.part), ptr(slot.counter), slot.counter.numel(), ptr(slot.out), ptr(slot.out64), 0,
-                 slot.class_offsets.ctypes.data_as(ctypes.c_void_p), slot.stream.cuda_stream)
+                 ptr(gl_w), ptr(gl_p), gl_len, ptr(slot.part), ptr(slot.counter), slot.counter.numel(), ptr(slot.out),
+                 ptr(slot.out64), 0, slot.class_offsets.ctypes.data_as(ctypes.c_void_p), slot.stream.cuda_stream)
             slot.host_out[:count].copy_(slot.out64[:count], non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(slot.stream)
@@ -545,8 +715,12 @@ class IntegrationServer:
         return finished
 
     def _vegas_stream(self, r):
-        if r.stream is None:
-            r.stream = torch.cuda.Stream(device=self.device)
+        # every VEGAS request shares one stream: with the batch slots' and the
+        # table stream that keeps the server within the device's 4 hardware
+        # queues (more streams would share queues and add false dependencies)
+        if self._vegas_shared_stream is None:
+            self._vegas_shared_stream = torch.cuda.Stream(device=self.device)
+        r.stream = self._vegas_shared_stream
         return r.stream
 
     # -- driving ------------------------------------------------------------
@@ -593,8 +767,10 @@ class IntegrationServer:
 
 
 def latency_stats(tickets):
-    """p50 / p99 / mean latency in ms over finished tickets."""
-    lat = sorted(t.latency * 1e3 for t in tickets if t.latency is not None and t.error is None)
+    """p50 / p99 / mean latency in ms over the finished integrals (a ticket of
+    submit_many counts once per integral, all with its latency)."""
+    lat = sorted(x for t in tickets if t.latency is not None and t.error is None
+                 for x in [t.latency * 1e3] * t.count)
     if not lat:
         return {"count": 0}
     return {
