@@ -41,14 +41,18 @@ sys.path.insert(0, str(ROOT))
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (packed counted)
 PEAK_HBM_GBS = 8000.0
+CLOCK_GHZ = 2.4  # MI355X peak engine clock
 # Algorithmic FLOPs per evaluation of the dim-5 Gaussian peak
 # exp(-sum_i a_i^2 (x_i - u_i)^2) with its quadrature weight: 4d+3 arithmetic
-# (+, -, x = 1, FMA = 2) plus 1 exp = 24.  The fused kernel executes fewer
-# per point because row and lane partial sums are shared across points.
+# (+, -, x = 1, FMA = 2) plus 1 exp = 24.  The fused kernel executes far
+# fewer per point (row and lane partial sums are shared across points), so
+# this count is reported only as an "algorithmic-equivalent" rate; the
+# roofline uses the executed FP32 operations from the committed counter pass.
 DIM_C2A = 5
 FLOPS_PER_EVAL_GRID_GAUSSIAN = 4 * DIM_C2A + 3 + 1
 COPY_BYTES = 256 << 20
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_grid_fused_boole_d5_n65.json"
+VALU_FILE = ROOT / "profiles" / "valu_grid_fused_boole_d5_n65.json"
 SETTLE_STEPS = 200  # untimed steps before tuning (about 35 ms of GPU work at n=65)
 
 
@@ -59,6 +63,20 @@ def load_traffic():
         return t["traffic_bytes_per_launch"]
     except (OSError, ValueError, KeyError):
         return None
+
+
+def load_valu():
+    """Executed VALU work per evaluation of the timed kernel (committed counter pass), or None."""
+    try:
+        return json.loads(VALU_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+
+
+def simd_count():
+    import torch
+
+    return 4 * torch.cuda.get_device_properties(0).multi_processor_count
 
 
 def cpu_model():
@@ -77,6 +95,38 @@ def _events():
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
 
+def bench_mc(c2b, dev, steps):
+    """Monte Carlo dim 8, 1e8 samples, FP32 oscillatory (HIP events on the launch stream)."""
+    import torch
+
+    import torchquad_amd as tq
+
+    mc = tq.MonteCarlo()
+    fo = tq.Oscillatory(c2b["c"], c2b["w"])
+    dom8 = torch.tensor(c2b["domain"], dtype=torch.float32, device=dev)
+    for _ in range(2):
+        mc.integrate(fo, 8, N=c2b["N"], integration_domain=dom8, seed=0)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = _events()
+    k = max(3, steps // 4)
+    e0.record(stream)
+    for i in range(k):
+        mval = mc.integrate(fo, 8, N=c2b["N"], integration_domain=dom8, seed=i)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    mc_ms = e0.elapsed_time(e1) / k
+    se = mc.std_error()
+    return {
+        "evals_per_s": c2b["N"] / (mc_ms / 1e3),
+        "ms_per_call": mc_ms,
+        "value": float(mval),
+        "analytic": c2b["analytic"],
+        "abs_err": abs(float(mval) - c2b["analytic"]),
+        "std_error": se,
+    }
+
+
 def bench_vegas(c3, dev):
     """VEGAS dim 6, 50 bins/axis, 20 iterations of about 1e6 evals."""
     import torch
@@ -87,10 +137,13 @@ def bench_vegas(c3, dev):
     dom = torch.tensor(c3["domain"], dtype=torch.float32, device=dev)
     N = c3["N_increment"] * (c3["iterations"] + 5)
     v = tq.VEGAS()
-    v.integrate(f, 6, N=N // 10, integration_domain=dom, seed=1, N_intervals=c3["N_intervals"])
+    # untimed run of the same shape: the integrator keeps its device state
+    # between calls of one shape, so the timed calls measure the iterations
+    v.integrate(f, 6, N=N, integration_domain=dom, seed=99, N_intervals=c3["N_intervals"],
+                max_iterations=c3["iterations"])
     torch.cuda.synchronize()
     times, vals = [], []
-    for seed in range(3):
+    for seed in range(5):
         t0 = time.perf_counter()
         val = float(v.integrate(f, 6, N=N, integration_domain=dom, seed=seed, N_intervals=c3["N_intervals"],
                                 max_iterations=c3["iterations"]))
@@ -101,8 +154,14 @@ def bench_vegas(c3, dev):
     return {
         "evals_per_s": v._nr_of_fevals / t,
         "ms_per_integrate": t * 1e3,
+        "ms_per_integrate_all": [round(x * 1e3, 3) for x in times],
+        "timing": "median of 5 integrates (seeds 0..4) after one untimed integrate of the same shape",
         "evals": int(v._nr_of_fevals),
         "iterations": int(v.it),
+        "schedule_note": f"N = {N} = 1e6 x (20 + 5), max_iterations=20, 50 bins: the reference's schedule "
+                         "(vegas.py:161-209) raises the evaluations per iteration by N/25 after every five "
+                         "stable iterations and stops when the next five would exceed N, here after "
+                         f"{int(v.it)} iterations",
         "value": vals[-1],
         "analytic": c3["analytic"],
         "std_error": float(v._get_error()),
@@ -121,6 +180,8 @@ def bench_batched(dev, steps):
     items = genz_batch(8192, seed=0)
     b = Batch(items, dtype=torch.float32, device=dev)
     (tile, blocks), _ = autotune(b, reps=3, use_cache=False)
+    b.run()
+    acc = batch_accuracy(items, b.out.double().cpu().numpy())
     g = b.graph()
     for _ in range(3):
         g.replay()
@@ -157,6 +218,84 @@ def bench_batched(dev, steps):
         "plain_loop_ms": ms_plain,
         "plain_loop_evals_per_s": b.evals / (ms_plain / 1e3),
         "speedup_vs_plain_loop": ms_plain / ms_batch,
+        **acc,
+    }
+
+
+def batch_accuracy(items, out):
+    """FP32 batch results against FP64 separable sums of the same rules (closed
+    forms for Gauss-Legendre beyond n = 1000, where the rule is exact to FP64)."""
+    import numpy as np
+
+    from oracle import families as fam
+    from oracle import grid as og
+    from torchquad_amd.batch import RULE_NAMES, n_per_axis
+
+    worst_pos = worst_osc = 0.0
+    for it, v in zip(items, out):
+        rule, fam_name, d = it["rule"], it["family"], it["dim"]
+        n = n_per_axis(RULE_NAMES[rule], d, it["N"])
+        a, u, w, dom = it["a"], it["u"], it["w"], it["domain"]
+        lo, hi = [0.0] * d, [1.0] * d
+        if rule == "gauss_legendre" and n > 1000:
+            ref = {"gaussian_peak": lambda: fam.gaussian_peak_integral(a, u, lo, hi),
+                   "oscillatory": lambda: fam.oscillatory_integral(a, w, lo, hi),
+                   "product_peak": lambda: fam.product_peak_integral(a, u, lo, hi)}[fam_name]()
+        elif fam_name == "gaussian_peak":
+            ref = og.separable_gaussian_peak(rule, a, u, dom, n)
+        elif fam_name == "oscillatory":
+            ref = og.separable_oscillatory(rule, a, w, dom, n)
+        else:
+            ref = og.separable_product(rule, [lambda x, ak=ak, uk=uk: 1.0 / (ak**-2.0 + (x - uk) ** 2)
+                                              for ak, uk in zip(a, u)], dom, n)
+        if fam_name == "oscillatory":
+            worst_osc = max(worst_osc, abs(v - ref))  # |f| <= 1 on a unit-volume box
+        else:
+            worst_pos = max(worst_pos, abs(v - ref) / abs(ref))
+    return {"max_rel_err_positive_families_vs_fp64_oracle": worst_pos,
+            "max_abs_err_oscillatory_vs_fp64_oracle": worst_osc,
+            "accuracy_note": "FP32 results of the graph-replayed batch against FP64 sums of the same rules; "
+                             "oscillatory integrals (|f| <= 1 on [0,1]^d) by absolute error"}
+
+
+def bench_param_batch(dev, steps):
+    """The reference's vectorised-integrand case shape (modular_benchmark.py:945-1011): Simpson,
+    dim 1, N=10001 on [0,1], 200 parameter values p = linspace(1, 5, 200); built-in family
+    cos(p x) (Oscillatory, w = 0).  Fused parameter batch vs a loop of 200 plain calls vs the
+    same object called as an [N, 200] torch integrand (point writer -> fn -> reduce)."""
+    import numpy as np
+    import torch
+
+    import torchquad_amd as tq
+
+    p = np.linspace(1.0, 5.0, 200)
+    fb = tq.Oscillatory(p[:, None], 0.0)
+    dom = torch.tensor([[0.0, 1.0]], dtype=torch.float32, device=dev)
+    simp = tq.Simpson()
+
+    def timed(fn, reps):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / reps, r
+
+    reps = max(20, steps)
+    ms_fused, vals = timed(lambda: simp.integrate(fb, 1, N=10001, integration_domain=dom), reps)
+    singles = [tq.Oscillatory([q], 0.0) for q in p]
+    ms_loop, _ = timed(lambda: [simp.integrate(f1, 1, N=10001, integration_domain=dom) for f1 in singles], 3)
+    ms_vec, uvals = timed(lambda: simp.integrate(lambda x: fb(x), 1, N=10001, integration_domain=dom), reps)
+    exact = np.sin(p) / p
+    return {
+        "workload": "Simpson dim=1 N=10001 [0,1] fp32, cos(p x) for 200 p in [1,5]",
+        "ms_fused_param_batch": ms_fused,
+        "ms_loop_of_200_calls": ms_loop,
+        "ms_torch_vectorised_integrand": ms_vec,
+        "speedup_fused_vs_loop": ms_loop / ms_fused,
+        "max_rel_err_fused_vs_analytic": float(np.max(np.abs(vals.double().cpu().numpy() - exact) / np.abs(exact))),
     }
 
 
@@ -187,28 +326,31 @@ def bench_c1(c1, dev):
     return out
 
 
-def cpu_ports(cfg, c2b):
-    """Oracle restatements timed on the host for bounded samples of the other configs."""
+def cpu_ports(cfg, c2b, seconds=3.0):
+    """The reference's algorithms on the host for the other configs: torch-CPU
+    on every core for C1 and the Monte Carlo config, the numpy VEGAS
+    restatement (one thread) for a bounded VEGAS sample."""
     import numpy as np
+    import torch
 
     from oracle import families as fam
-    from oracle import grid as og
-    from oracle import native
+    from oracle import torch_cpu
     from oracle import vegas as ov
 
     out = {}
+    cores = os.cpu_count()
+    before = torch.get_num_threads()
+    torch.set_num_threads(cores)
     c1 = cfg["C1"]
     for rule in ("trapezoid", "simpson"):
-        t0 = time.perf_counter()
-        og.integrate(rule, lambda x: fam.exp_linear(x, np.ones(3)), 3, c1["N"], c1["domain"])
-        dt = time.perf_counter() - t0
-        out[f"c1_{rule}"] = {"evals_per_s": c1["N"] / dt, "sample": "full C1 grid 101^3 fp64 (numpy)", "cores": 1}
-    n_mc = 2_000_000
-    t0 = time.perf_counter()
-    native.mc_sums("float32", "oscillatory", 8, n_mc, 0, 0, c2b["domain"], c1=c2b["w"], a=c2b["c"])
-    dt = time.perf_counter() - t0
-    out["mc_d8"] = {"evals_per_s": n_mc / dt, "sample": f"{n_mc} samples, C oracle (Philox + FP64 sums)",
-                    "cores": 1}
+        val, _, pts, dt = torch_cpu.grid_rule_slices(rule, torch_cpu.exp_sum_fn(), c1["domain"], 101,
+                                                     dtype=torch.float64)
+        out[f"c1_{rule}"] = {"evals_per_s": pts / dt, "value": val, "cores": cores,
+                             "sample": "full C1 grid 101^3 fp64 (torch-CPU restatement)"}
+    n_mc, dt, est = torch_cpu.monte_carlo_oscillatory(c2b["c"], c2b["w"], c2b["domain"], seconds)
+    out["mc_d8"] = {"evals_per_s": n_mc / dt, "estimate": est, "cores": cores,
+                    "sample": f"{n_mc} samples in chunks of 1e7 (torch.rand + cos, torch-CPU)"}
+    torch.set_num_threads(before)
     c3 = cfg["C3"]
     n_v = 2_500_000
     t0 = time.perf_counter()
@@ -222,6 +364,7 @@ def cpu_ports(cfg, c2b):
 
 def bench_serving(dev, rate=2000.0, count=4000):
     """Serving loop: Poisson arrivals of Genz integrals at `rate`/s; latency and throughput."""
+    import numpy as np
     import torch
 
     from torchquad_amd.serve import IntegrationServer, latency_stats, serve_open_loop
@@ -276,12 +419,48 @@ def bench_serving(dev, rate=2000.0, count=4000):
     st_gl = latency_stats([t for t in tickets if t.rule == gl_code])
     st_nc = latency_stats([t for t in tickets if t.rule != gl_code])
     evals = sum(t.evals for t in tickets)
-    # saturation: every request submitted at once
+    # saturation: every request submitted at once, one submit_grid call each
     server.completed.clear()
     t0 = time.perf_counter()
     tickets2 = [req(it)(server, None) for it in items]
     server.run_until_idle(timeout=120)
     wall2 = time.perf_counter() - t0
+    # saturation with the same requests as packed arrays of 512 (submit_many)
+    from torchquad_amd.workloads import genz_arrays
+
+    arr = genz_arrays(count, seed=1)
+    chunk = 512
+
+    def packed():
+        ts = [server.submit_many(arr["rule"][k:k + chunk], arr["family"][k:k + chunk], arr["dim"][k:k + chunk],
+                                 arr["domain"][k:k + chunk], arr["a"][k:k + chunk], arr["u"][k:k + chunk],
+                                 arr["c"][k:k + chunk], N=arr["N"][k:k + chunk]) for k in range(0, count, chunk)]
+        server.run_until_idle(timeout=120)
+        return ts
+
+    packed()  # warm-up
+    server.completed.clear()
+    walls = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        tickets3 = packed()
+        walls.append(time.perf_counter() - t0)
+    wall3 = statistics.median(walls)
+    same = all(np.array_equal(t.result(), np.array([x.result() for x in tickets2[k * chunk:(k + 1) * chunk]]))
+               for k, t in enumerate(tickets3))
+    # the plain per-call path on the same requests
+    import torchquad_amd as tq
+
+    integ = {"trapezoid": tq.Trapezoid(), "simpson": tq.Simpson(), "gauss_legendre": tq.GaussLegendre()}
+    doms = {d: torch.tensor([[0.0, 1.0]] * d, dtype=torch.float32, device=dev) for d in range(1, 7)}
+    for it in items:  # warm pass (Gauss-Legendre tables)
+        integ[it["rule"]].integrate(it["fn"], it["dim"], N=it["N"], integration_domain=doms[it["dim"]])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for it in items:
+        integ[it["rule"]].integrate(it["fn"], it["dim"], N=it["N"], integration_domain=doms[it["dim"]])
+    torch.cuda.synchronize()
+    wall_plain = time.perf_counter() - t0
     return {
         "arrival_rate_per_s": rate,
         "requests": count,
@@ -298,8 +477,14 @@ def bench_serving(dev, rate=2000.0, count=4000):
         "longest_round_ms": step_max[0] * 1e3,
         "evals_per_s_at_rate": evals / wall,
         "batches": server.batches_launched,
-        "saturated_evals_per_s": sum(t.evals for t in tickets2) / wall2,
-        "saturated_requests_per_s": count / wall2,
+        "saturated_single_submit_evals_per_s": sum(t.evals for t in tickets2) / wall2,
+        "saturated_single_submit_requests_per_s": count / wall2,
+        "saturated_evals_per_s": sum(t.evals for t in tickets3) / wall3,
+        "saturated_requests_per_s": count / wall3,
+        "saturated_note": f"the {count} requests as packed arrays of {chunk} (submit_many), submitted at once; "
+                          "median of 5 drains; values bit-identical to the single submissions: " + str(same),
+        "plain_loop_evals_per_s": sum(t.evals for t in tickets2) / wall_plain,
+        "saturated_vs_plain_loop": wall_plain / wall3,
         "workload": "Genz integrals (dim 1..6, N 1e3..1e6, trapezoid/simpson/gauss-legendre), numpy PCG64 seed 1",
     }
 
@@ -389,6 +574,8 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run only a 256 MiB elementwise copy (counter calibration) and the timed step, "
                     "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes")
+    ap.add_argument("--profile-section", choices=["mc", "vegas", "batched", "grad", "param", "serving"],
+                    help="run only this side section (for rocprofv3 counter passes) and print its result")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -416,6 +603,19 @@ def main():
     n = args.n
     N = n**5
     stream = torch.cuda.current_stream(dev)
+
+    if args.profile_section:
+        torch.cuda.init()
+        section = {
+            "mc": lambda: bench_mc(c2b, dev, args.steps),
+            "vegas": lambda: bench_vegas(cfg["C3"], dev),
+            "batched": lambda: bench_batched(dev, args.steps),
+            "grad": lambda: bench_grad(cfg["C5"], dev, args.steps),
+            "param": lambda: bench_param_batch(dev, args.steps),
+            "serving": lambda: bench_serving(dev),
+        }[args.profile_section]()
+        print(json.dumps({"profile_section": args.profile_section, **section}))
+        return
 
     if args.traffic_probe:
         # plain copy of known size: reads and writes COPY_BYTES each
@@ -491,80 +691,86 @@ def main():
     p99 = lat[int(len(lat) * 0.99) - 1]
 
     # Monte Carlo, dim 8, 1e8 samples
-    mc_info = None
-    if not args.no_mc:
-        mc = tq.MonteCarlo()
-        fo = tq.Oscillatory(c2b["c"], c2b["w"])
-        dom8 = torch.tensor(c2b["domain"], dtype=torch.float32, device=dev)
-        for _ in range(2):
-            mc.integrate(fo, 8, N=c2b["N"], integration_domain=dom8, seed=0)
-        torch.cuda.synchronize()
-        k = max(3, args.steps // 4)
-        e0.record(stream)
-        for i in range(k):
-            mval = mc.integrate(fo, 8, N=c2b["N"], integration_domain=dom8, seed=i)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        mc_ms = e0.elapsed_time(e1) / k
-        se = mc.std_error()
-        mc_info = {
-            "evals_per_s": c2b["N"] / (mc_ms / 1e3),
-            "ms_per_call": mc_ms,
-            "value": float(mval),
-            "analytic": c2b["analytic"],
-            "abs_err": abs(float(mval) - c2b["analytic"]),
-            "std_error": se,
-        }
+    mc_info = None if args.no_mc else bench_mc(c2b, dev, args.steps)
 
     extra = {}
     if not args.no_extra:
         extra["vegas_d6"] = bench_vegas(cfg["C3"], dev)
         extra["batched_c4"] = bench_batched(dev, args.steps)
+        extra["param_batch_simpson_1d"] = bench_param_batch(dev, args.steps)
         extra["grad_c5"] = bench_grad(cfg["C5"], dev, args.steps)
         extra["serving_c4"] = bench_serving(dev)
         extra["c1_fp64_d3_n101"] = bench_c1(cfg["C1"], dev)
         if not args.no_cpu:
             extra["cpu_port_other_configs"] = cpu_ports(cfg, c2b)
 
-    achieved_tflops = FLOPS_PER_EVAL_GRID_GAUSSIAN * evals_per_s / 1e12
+    # Roofline of the timed kernel.  One step is one launch of the fused kernel
+    # (the last workgroup finishes the sum), so the step time measured above
+    # with HIP events on the launch stream is the kernel's duration per launch.
+    # Executed FP32 operations per evaluation come from the committed counter
+    # pass (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F32 x 64 lanes, FMA = 2); the
+    # VALU issue fraction counts every wave instruction as 4 cycles of its
+    # SIMD and a transcendental as 8.
+    valu = load_valu()
+    kernel_s = ms_per_step / 1e3
+    alg_tflops = FLOPS_PER_EVAL_GRID_GAUSSIAN * evals_per_s / 1e12
     roofline = {
         "bound": "valu",
-        "kernel": "k_grid_fused<float, GAUSSIAN_PEAK, table>",
-        "achieved": achieved_tflops,
+        "kernel": "k_grid_fused<float, GAUSSIAN_PEAK, table, rowtab, 8 waves> (one launch per step)",
+        "achieved": None,
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
-        "frac": achieved_tflops / PEAK_FP32_TFLOPS,
-        "flops_per_eval": FLOPS_PER_EVAL_GRID_GAUSSIAN,
-        "flops_per_eval_note": "4d+3 arithmetic + 1 exp at d=5 (algorithmic count; the kernel shares row and "
-                               "lane partial sums, so it executes fewer and frac can exceed 1)",
-        "bound_note": "vector ALU: FP32 157.3 TFLOP/s, the same figure as dense FP32 MFMA; no HBM traffic "
-                      "or matrix product on this path",
+        "frac": None,
+        "bound_note": "vector ALU: FP32 157.3 TFLOP/s (packed FMA counted; packed FP32 is disabled in this "
+                      "build, so FMA-only code would reach 78.6); no HBM stream or matrix product on this path",
         "traffic": load_traffic(),
+        "algorithmic_equivalent": {
+            "flops_per_eval": FLOPS_PER_EVAL_GRID_GAUSSIAN,
+            "tflops": alg_tflops,
+            "note": "4d+3 arithmetic + 1 exp per evaluation at d=5 as if each point were evaluated from "
+                    "scratch; not a hardware rate (the kernel shares row and lane partial sums)",
+        },
     }
+    if valu:
+        flops_eval = valu["executed_fp32_flops_per_eval"]
+        achieved = flops_eval * evals_per_s / 1e12
+        issue_s = (valu["valu_insts_per_eval"] * 4 + valu["trans_f32_insts_per_eval"] * 4) * N / (
+            simd_count() * CLOCK_GHZ * 1e9)
+        roofline.update({
+            "achieved": achieved,
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "executed_fp32_flops_per_eval": flops_eval,
+            "valu_issue_frac": issue_s / kernel_s,
+            "valu_note": "executed work per evaluation from " + VALU_FILE.name + " (" + valu["command"] +
+                         "); valu_issue_frac = (VALU wave instructions x 4 cycles + transcendentals x 4 more) / "
+                         "(SIMDs x 2.4 GHz x measured time per launch)",
+        })
 
     cpu = None
     if not args.no_cpu:
-        # oracle restatement (materialised grid + per-axis collapse, numpy FP64)
-        # of the same rule/integrand on the 17^5 grid, repeated for ~cpu_seconds
-        from oracle import families as fam
+        # the reference's algorithm on torch CPU (linspace axes, ij meshgrid
+        # points, the integrand, the per-axis Boole collapse) for the same
+        # workload, FP32, on every host core; bounded to about cpu_seconds by
+        # running whole slices of the outermost axis
+        from oracle import torch_cpu
 
-        a = np.array(c2a["a"])
-        u = np.array(c2a["u"])
-        calls = 0
-        t0 = time.perf_counter()
-        while True:
-            og.integrate("boole", lambda x: fam.gaussian_peak(x, a, u), 5, N17, c2a["domain"])
-            calls += 1
-            if time.perf_counter() - t0 >= args.cpu_seconds:
-                break
-        dt = time.perf_counter() - t0
+        cores = os.cpu_count()
+        threads_before = torch.get_num_threads()
+        torch.set_num_threads(cores)
+        fn_cpu = torch_cpu.gaussian_peak_fn(c2a["a"], c2a["u"])
+        torch_cpu.grid_rule_slices("boole", fn_cpu, c2a["domain"], 17)  # warm-up
+        cval, slices, pts, dt = torch_cpu.grid_rule_slices("boole", fn_cpu, c2a["domain"], n,
+                                                           seconds=args.cpu_seconds)
+        torch.set_num_threads(threads_before)
         cpu = {
-            "value": calls * N17 / dt,
+            "value": pts / dt,
             "unit": "integrand evals/s",
-            "cores": 1,
+            "cores": cores,
             "kind": "port",
-            "sample": f"oracle.grid.integrate('boole', GaussianPeak d=5, N=17^5) x{calls} "
-            f"({dt:.1f} s, numpy FP64, {cpu_model()}, os.cpu_count()={os.cpu_count()})",
+            "sample": f"torch-CPU restatement of the reference path (oracle/torch_cpu.py): Boole d=5 n={n} "
+            f"FP32 Gaussian peak, {slices} of {n} outer-axis slices ({pts} evals, {dt:.1f} s), "
+            f"torch.set_num_threads({cores}), {cpu_model()}"
+            + (f", full-grid value {cval:.9g}" if cval is not None else ""),
         }
 
     out = {

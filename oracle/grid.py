@@ -16,7 +16,19 @@ Pinned by polynomial exactness (trapezoid_test.py:34-48, simpson_test.py
 (an independent formulation) and the brute-force tensor-product sum of
 oracle.c (tests/test_oracle.py).
 """
+import functools
+
 import numpy as np
+
+
+@functools.lru_cache(maxsize=None)
+def leggauss(n):
+    """numpy's Gauss-Legendre nodes and weights (the reference's _root_fn,
+    gaussian.py:25), cached per n; read-only arrays."""
+    r, w = np.polynomial.legendre.leggauss(n)
+    r.setflags(write=False)
+    w.setflags(write=False)
+    return r, w
 
 
 def n_per_dim(N, dim):
@@ -40,7 +52,7 @@ def adjust_N(rule, dim, N):
 
 def axis_nodes(rule, a, b, n):
     if rule == "gauss_legendre":
-        r, _ = np.polynomial.legendre.leggauss(n)
+        r, _ = leggauss(n)
         return (b - a) / 2.0 * r + (a + b) / 2.0
     return np.linspace(a, b, n)
 
@@ -79,7 +91,7 @@ def collapse(rule, values_nd, dim, hs, domain):
 
 
 def gl_weights(n, dim):
-    _, w = np.polynomial.legendre.leggauss(n)
+    _, w = leggauss(n)
     mesh = np.meshgrid(*([w] * dim), indexing="ij")
     return np.prod(np.stack(mesh), axis=0).ravel()
 
@@ -109,7 +121,7 @@ def integrate(rule, f, dim, N, domain):
 def axis_weights(rule, a, b, n):
     """Full per-axis weights (scale included): the 1-D rule."""
     if rule == "gauss_legendre":
-        _, w = np.polynomial.legendre.leggauss(n)
+        _, w = leggauss(n)
         return w * (b - a) / 2.0
     h = (b - a) / (n - 1)
     w = np.zeros(n)

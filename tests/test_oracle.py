@@ -136,3 +136,25 @@ def test_mc_oracle_same_stream_mean():
     assert s1 == 1000.0 and s2 == 1000.0
     s1, _ = native.mc_sums("float64", "exp_linear", 3, 200000, 7, 0, dom, a=[1.0, 1.0, 1.0])
     assert s1 / 200000 == pytest.approx((math.e - 1) ** 3, rel=5e-3)
+
+
+def test_torch_cpu_restatement_matches_oracle():
+    """The torch-CPU restatement timed as bench.py's CPU baseline computes the
+    same sums as the numpy oracle (FP64, small grids)."""
+    import torch
+
+    from oracle import families as fam
+    from oracle import grid as og
+    from oracle import torch_cpu
+
+    a, u = np.array([2.0, 3.0, 1.5]), np.array([0.1, -0.2, 0.3])
+    dom = [[-1.0, 1.0]] * 3
+    for rule, n in (("boole", 9), ("simpson", 7), ("trapezoid", 6)):
+        val, done, pts, _ = torch_cpu.grid_rule_slices(rule, torch_cpu.gaussian_peak_fn(a, u, torch.float64), dom, n,
+                                                       dtype=torch.float64)
+        ref = og.integrate(rule, lambda x: fam.gaussian_peak(x, a, u), 3, n**3, dom)
+        assert done == n and pts == n**3
+        assert abs(val - ref) <= 1e-13 * abs(ref)
+    # a bounded run stops early and reports no value
+    val, done, pts, _ = torch_cpu.grid_rule_slices("boole", torch_cpu.exp_sum_fn(), dom, 9, seconds=0.0)
+    assert val is None and done == 1 and pts == 81
