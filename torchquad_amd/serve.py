@@ -601,11 +601,14 @@ class IntegrationServer:
         for r in self._admitted_grid:  # start tables even while every slot is busy
             if r.gl_seq is None and len(r.gl_unique):
                 r.gl_seq = arena.place(r.gl_unique)
-                if r.gl_seq is None and idle and arena.idle():
-                    # over the arena's limit with nothing reading it: start afresh
+                # Over the arena's limit: start afresh, but only while nothing
+                # reads it (no batch in flight, no table being generated) and
+                # no admitted request holds a placement that it has not
+                # launched yet -- clearing those let two requests that do not
+                # fit together take the arena from each other forever.
+                if (r.gl_seq is None and idle and arena.idle()
+                        and all(q.gl_seq is None for q in self._admitted_grid)):
                     arena.clear()
-                    for q in self._admitted_grid:
-                        q.gl_seq = None
                     r.gl_seq = arena.place(r.gl_unique)
         slot = None
         for k in range(len(self._slots)):
