@@ -546,10 +546,32 @@ def bench_grad(c5, dev, steps):
         one(i)
     torch.cuda.synchronize()
     ms_loop = (time.perf_counter() - t0) * 1e3 * B / sub
+
+    # the other two Genz families of the batched workload on the same shape
+    from torchquad_amd.autograd import oscillatory_integrals, product_peak_integrals
+
+    c_osc = torch.tensor(rng.uniform(0.0, 5.0, (B, 4)), dtype=torch.float32, device=dev, requires_grad=True)
+    w_osc = torch.tensor(rng.uniform(0.0, 1.0, B), dtype=torch.float32, device=dev, requires_grad=True)
+    families = {}
+    for name, fn, p1, p2 in (("oscillatory", oscillatory_integrals, c_osc, w_osc),
+                              ("product_peak", product_peak_integrals, ta, tu)):
+        def fstep(fn=fn, p1=p1, p2=p2):
+            fn(p1, p2, td, n=c5["n"]).sum().backward()
+
+        for _ in range(3):
+            fstep()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fstep()
+        e1.record()
+        torch.cuda.synchronize()
+        families[name] = {"ms_forward_backward": e0.elapsed_time(e1) / reps}
     return {
         "vectors": B,
         "evals_per_pass": evals,
         "ms_forward_backward": ms,
+        "other_families_same_shape": families,
         "evals_per_s_forward_backward": 2 * evals / (ms / 1e3),
         "first_value": float(out[0]),
         "first_analytic": c5["first_analytic"],

@@ -277,6 +277,20 @@ int tq_sweep_gaussian_forward(const tq_grid_desc* g, int64_t B, const void* a, c
 int tq_sweep_gaussian_backward(const tq_grid_desc* g, int64_t B, const void* a, const void* u, const void* domains,
                                const void* grad_out, void* workspace, size_t workspace_bytes, void* grad_a,
                                void* grad_u, void* grad_domains, void* stream);
+/* The same sweep for other families (family: TQ_FAM_GAUSSIAN_PEAK,
+ * TQ_FAM_OSCILLATORY or TQ_FAM_PRODUCT_PEAK; others TQ_E_UNSUPPORTED), with
+ * the workspace of tq_sweep_workspace_size.  Parameters as the family's
+ * tq_integrand: Gaussian and product peak a[B][dim], u[B][dim] (grad_a,
+ * grad_u alike); oscillatory cos(2 pi w + sum c_k x_k): c[B][dim] in a and
+ * w[B] in u, so grad_a = dI/dc [B][dim] and grad_u = dI/dw [B].  The
+ * reference differentiates these integrands with torch autograd through the
+ * materialised grid (utils.py:24-58, tests/gradient_test.py:162-259). */
+int tq_sweep_forward(const tq_grid_desc* g, int32_t family, int64_t B, const void* a, const void* u,
+                     const void* domains, void* workspace, size_t workspace_bytes, void* out, double* out64,
+                     void* stream);
+int tq_sweep_backward(const tq_grid_desc* g, int32_t family, int64_t B, const void* a, const void* u,
+                      const void* domains, const void* grad_out, void* workspace, size_t workspace_bytes,
+                      void* grad_a, void* grad_u, void* grad_domains, void* stream);
 
 /* ---- VEGAS ----------------------------------------------------------------
  * Device-resident VEGAS+ state (reference: vegas_map.py, vegas_stratification.py,

@@ -141,6 +141,11 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.POINTER(GridDesc), _I64, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _P],
     ),
+    "tq_sweep_forward": (ctypes.c_int, [ctypes.POINTER(GridDesc), _I32, _I64, _P, _P, _P, _P, _SZ, _P, _P, _P]),
+    "tq_sweep_backward": (
+        ctypes.c_int,
+        [ctypes.POINTER(GridDesc), _I32, _I64, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _P],
+    ),
     "tq_batch_plan_sizes": (ctypes.c_int, [_P, _I64, _I32, _I64, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "tq_batch_plan": (ctypes.c_int, [_P, _I64, _I32, _I64, _P, _P, _P, _P, _P]),
     "tq_batch_run": (ctypes.c_int, [_I32, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _I64, _P, _P, _I64, _P, _P, _I32,

@@ -1,12 +1,14 @@
-"""Differentiable parameter sweeps with a hand-written backward kernel.
+"""Differentiable parameter sweeps with hand-written backward kernels.
 
 ``gaussian_peak_integrals(a, u, domains, N, rule)`` integrates
 f_b(x) = exp(-sum_k a_bk^2 (x_k - u_bk)^2) over the box domains[b] for every
 row b with one forward kernel, and its backward is one kernel that returns
-d/da, d/du and d/d(domain bounds) (see tq_sweep_gaussian_backward).  It
-replaces a Python loop of integrate() calls differentiated by torch autograd
+d/da, d/du and d/d(domain bounds) (tq_sweep_backward);
+``oscillatory_integrals(c, w, ...)`` and ``product_peak_integrals(a, u, ...)``
+do the same for the other two Genz families of the batched workload.  They
+replace a Python loop of integrate() calls differentiated by torch autograd
 through the materialised grid (the reference's way, gaussian.py:44-162,
-tests/gradient_test.py).
+utils.py:24-58, tests/gradient_test.py:162-259).
 """
 import ctypes
 
@@ -32,24 +34,26 @@ def _desc(rule, dim, n, dtype, device):
     return _ops.grid_desc(rule, dim, n, dtype, gl), gl
 
 
-class _GaussianPeakSweep(torch.autograd.Function):
+class _Sweep(torch.autograd.Function):
+    """Forward and hand-written backward of one family's parameter sweep."""
+
     @staticmethod
-    def forward(ctx, a, u, domains, rule, n):
+    def forward(ctx, a, u, domains, family, rule, n):
         B, dim = a.shape
         g, gl = _desc(rule, dim, n, a.dtype, a.device)
         nbytes = lib().tq_sweep_workspace_size(ctypes.byref(g), B)
         ws = workspace.get(max(nbytes, 8), a.device, tag="sweep")
         out = torch.empty(B, dtype=a.dtype, device=a.device)
-        call("tq_sweep_gaussian_forward", ctypes.byref(g), B, ptr(a), ptr(u), ptr(domains), ptr(ws), ws.numel(),
+        call("tq_sweep_forward", ctypes.byref(g), family, B, ptr(a), ptr(u), ptr(domains), ptr(ws), ws.numel(),
              ptr(out), None, stream_ptr(a.device))
         ctx.save_for_backward(a, u, domains)
-        ctx.meta = (rule, n, gl)
+        ctx.meta = (family, rule, n, gl)
         return out
 
     @staticmethod
     def backward(ctx, grad_out):
         a, u, domains = ctx.saved_tensors
-        rule, n, gl = ctx.meta
+        family, rule, n, gl = ctx.meta
         B, dim = a.shape
         g = _ops.grid_desc(rule, dim, n, a.dtype, gl)
         nbytes = lib().tq_sweep_workspace_size(ctypes.byref(g), B)
@@ -58,9 +62,28 @@ class _GaussianPeakSweep(torch.autograd.Function):
         ga = torch.empty_like(a) if ctx.needs_input_grad[0] else None
         gu = torch.empty_like(u) if ctx.needs_input_grad[1] else None
         gd = torch.empty_like(domains) if ctx.needs_input_grad[2] else None
-        call("tq_sweep_gaussian_backward", ctypes.byref(g), B, ptr(a), ptr(u), ptr(domains), ptr(grad_out), ptr(ws),
+        call("tq_sweep_backward", ctypes.byref(g), family, B, ptr(a), ptr(u), ptr(domains), ptr(grad_out), ptr(ws),
              ws.numel(), ptr(ga), ptr(gu), ptr(gd), stream_ptr(a.device))
-        return ga, gu, gd, None, None
+        return ga, gu, gd, None, None, None
+
+
+def _sweep(family, a, u, domains, N, n, rule, name, u_per_axis=True):
+    require_device(a)
+    if a.dim() != 2 or domains.shape != (*a.shape, 2):
+        raise ValueError(f"{name}: expected parameters of shape [B, dim] and domains of shape [B, dim, 2]")
+    if (u_per_axis and u.shape != a.shape) or (not u_per_axis and u.shape != a.shape[:1]):
+        raise ValueError(f"{name}: second parameter has shape {tuple(u.shape)}")
+    if a.shape[1] > 8:
+        raise ValueError(f"{name} supports dim <= 8")
+    code = RULE_NAMES[rule.lower()] if isinstance(rule, str) else int(rule)
+    if n is None:
+        if N is None:
+            raise ValueError("give N (total points) or n (points per axis)")
+        n = n_per_axis(code, a.shape[1], int(N))
+    dtype = a.dtype
+    u = u.to(dtype).contiguous()
+    domains = domains.to(dtype).contiguous()
+    return _Sweep.apply(a.contiguous(), u, domains, family, code, int(n))
 
 
 def gaussian_peak_integrals(a, u, domains, N=None, n=None, rule="gauss_legendre"):
@@ -76,17 +99,28 @@ def gaussian_peak_integrals(a, u, domains, N=None, n=None, rule="gauss_legendre"
     Returns:
         [B] tensor; differentiable w.r.t. a, u and domains.
     """
-    require_device(a)
-    if a.dim() != 2 or u.shape != a.shape or domains.shape != (*a.shape, 2):
-        raise ValueError("expected a, u of shape [B, dim] and domains of shape [B, dim, 2]")
-    if a.shape[1] > 8:
-        raise ValueError("gaussian_peak_integrals supports dim <= 8")
-    code = RULE_NAMES[rule.lower()] if isinstance(rule, str) else int(rule)
-    if n is None:
-        if N is None:
-            raise ValueError("give N (total points) or n (points per axis)")
-        n = n_per_axis(code, a.shape[1], int(N))
-    dtype = a.dtype
-    u = u.to(dtype).contiguous()
-    domains = domains.to(dtype).contiguous()
-    return _GaussianPeakSweep.apply(a.contiguous(), u, domains, code, int(n))
+    return _sweep(_lib.FAM_GAUSSIAN_PEAK, a, u, domains, N, n, rule, "gaussian_peak_integrals")
+
+
+def oscillatory_integrals(c, w, domains, N=None, n=None, rule="gauss_legendre"):
+    """Integrals of cos(2 pi w + sum_k c_k x_k) over domains[b] for each row b
+    (Genz's oscillatory family, as ``integrands.Oscillatory``).
+
+    Args:
+        c: [B, dim]; w: [B]; domains: [B, dim, 2] (float32 or float64, GPU).
+        N / n, rule: as for :func:`gaussian_peak_integrals`.
+
+    Returns:
+        [B] tensor; differentiable w.r.t. c, w and domains.
+    """
+    return _sweep(_lib.FAM_OSCILLATORY, c, w, domains, N, n, rule, "oscillatory_integrals", u_per_axis=False)
+
+
+def product_peak_integrals(a, u, domains, N=None, n=None, rule="gauss_legendre"):
+    """Integrals of prod_k 1/(a_k^-2 + (x_k - u_k)^2) over domains[b] for each
+    row b (Genz's product-peak family, as ``integrands.ProductPeak``).
+
+    Args and returns as :func:`gaussian_peak_integrals`; differentiable w.r.t.
+    a, u and domains.
+    """
+    return _sweep(_lib.FAM_PRODUCT_PEAK, a, u, domains, N, n, rule, "product_peak_integrals")
