@@ -280,6 +280,10 @@ class _VegasRequest:
         self.wait = False
 
 
+def _aligned(nbytes, to=256):
+    return (int(nbytes) + to - 1) // to * to
+
+
 class _Pinned:
     """A growable pinned host buffer and its device twin (bytes)."""
 
@@ -304,11 +308,7 @@ class _BatchSlot:
         self.device = device
         self.dtype = dtype
         self.stream = torch.cuda.Stream(device=device)
-        self.items = _Pinned(device)
-        self.desc = _Pinned(device)
-        self.tiles = _Pinned(device)
-        self.tb = _Pinned(device)
-        self.tc = _Pinned(device)
+        self.stage = _Pinned(device)  # plan outputs: descriptors | tiles | tile_begin | tile_count
         self.part = torch.empty(0, dtype=torch.float64, device=device)
         self.counter = torch.zeros(_lib.BATCH_COUNTER_WORDS, dtype=torch.int32, device=device)
         self.class_offsets = np.zeros(4, dtype=np.int64)
@@ -638,26 +638,28 @@ class IntegrationServer:
             items["gl_offset"][is_gl] = arena.offsets(gl_n[is_gl])
             gl_x, gl_w, gl_p, gl_len = arena.x, arena.w, arena.p, arena.used
         with torch.cuda.stream(slot.stream):
-            ih, idev = slot.items.reserve(count * ITEM_BYTES)
-            ih[: count * ITEM_BYTES].numpy()[:] = items.view(np.uint8)
-            base = ih.data_ptr()
+            # the host planner reads the records where they are; its outputs
+            # (descriptors, tiles, per-integral tile ranges) go into one pinned
+            # staging buffer and reach the device in one copy
+            base = ctypes.c_void_p(items.ctypes.data)
             desc_bytes = ctypes.c_int64()
             n_tiles = ctypes.c_int64()
-            call("tq_batch_plan_sizes", ctypes.c_void_p(base), count, self.dt, self.tile_points,
-                 ctypes.byref(desc_bytes), ctypes.byref(n_tiles))
+            call("tq_batch_plan_sizes", base, count, self.dt, self.tile_points, ctypes.byref(desc_bytes),
+                 ctypes.byref(n_tiles))
             nt = max(int(n_tiles.value), 1)
-            dh, dd = slot.desc.reserve(desc_bytes.value)
-            th, td = slot.tiles.reserve(nt * TILE_BYTES)
-            bh, bd = slot.tb.reserve(count * 8)
-            ch, cd = slot.tc.reserve(count * 8)
-            call("tq_batch_plan", ctypes.c_void_p(base), count, self.dt, self.tile_points, ptr(dh), ptr(th), ptr(bh),
-                 ptr(ch), slot.class_offsets.ctypes.data_as(ctypes.c_void_p))
-            dd[: desc_bytes.value].copy_(dh[: desc_bytes.value], non_blocking=True)
-            td[: nt * TILE_BYTES].copy_(th[: nt * TILE_BYTES], non_blocking=True)
-            bd[: count * 8].copy_(bh[: count * 8], non_blocking=True)
-            cd[: count * 8].copy_(ch[: count * 8], non_blocking=True)
+            o_tiles = _aligned(desc_bytes.value)
+            o_begin = o_tiles + _aligned(nt * TILE_BYTES)
+            o_count = o_begin + _aligned(count * 8)
+            total = o_count + count * 8
+            sh, sd = slot.stage.reserve(total)
+            hp, dp = sh.data_ptr(), sd.data_ptr()
+            call("tq_batch_plan", base, count, self.dt, self.tile_points, ctypes.c_void_p(hp),
+                 ctypes.c_void_p(hp + o_tiles), ctypes.c_void_p(hp + o_begin), ctypes.c_void_p(hp + o_count),
+                 slot.class_offsets.ctypes.data_as(ctypes.c_void_p))
+            sd[:total].copy_(sh[:total], non_blocking=True)
             slot.ensure_outputs(count, nt)
-            call("tq_batch_run", self.dt, ptr(dd), count, ptr(td), int(n_tiles.value), ptr(bd), ptr(cd), ptr(gl_x),
+            call("tq_batch_run", self.dt, ctypes.c_void_p(dp), count, ctypes.c_void_p(dp + o_tiles),
+                 int(n_tiles.value), ctypes.c_void_p(dp + o_begin), ctypes.c_void_p(dp + o_count), ptr(gl_x),
                  ptr(gl_w), ptr(gl_p), gl_len, ptr(slot.part), ptr(slot.counter), slot.counter.numel(), ptr(slot.out),
                  ptr(slot.out64), 0, slot.class_offsets.ctypes.data_as(ctypes.c_void_p), slot.stream.cuda_stream)
             slot.host_out[:count].copy_(slot.out64[:count], non_blocking=True)
