@@ -73,6 +73,38 @@ def load_valu():
         return None
 
 
+def probe_rates(dev):
+    """The device's own rates for the fused grid kernel's loop (tq_probe_rates):
+    cycles per wave instruction of FP32 FMA and exp2 chains at the nominal
+    clock, and the evaluations/s of the loop body alone."""
+    import ctypes
+
+    import torch
+
+    from torchquad_amd._lib import call, stream_ptr
+
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    blocks = 32 * cus  # 128 waves per CU
+    out = torch.empty(blocks * 256, dtype=torch.float32, device=dev)
+    waves_per_simd = blocks * 4 / (4 * cus)
+    res = {}
+    for kind, iters in ((0, 4000), (1, 1000), (2, 200)):
+        for _ in range(2):
+            call("tq_probe_rates", kind, blocks, 5, ctypes.c_void_p(out.data_ptr()), stream_ptr(dev))
+        torch.cuda.synchronize()
+        e0, e1 = _events()
+        e0.record()
+        call("tq_probe_rates", kind, blocks, iters, ctypes.c_void_p(out.data_ptr()), stream_ptr(dev))
+        e1.record()
+        torch.cuda.synchronize()
+        sec = e0.elapsed_time(e1) / 1e3
+        if kind < 2:
+            res["fma_cycles" if kind == 0 else "exp2_cycles"] = sec * CLOCK_GHZ * 1e9 / (waves_per_simd * iters * 32)
+        else:
+            res["loop_evals_per_s"] = blocks * 256 * iters * 256 / sec
+    return res
+
+
 def simd_count():
     import torch
 
@@ -743,8 +775,10 @@ def main():
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": None,
-        "bound_note": "vector ALU: FP32 157.3 TFLOP/s (packed FMA counted; packed FP32 is disabled in this "
-                      "build, so FMA-only code would reach 78.6); no HBM stream or matrix product on this path",
+        "bound_note": "vector ALU (no HBM stream or matrix product on this path): MI355X's FP32 157.3 TFLOP/s; "
+                      "each evaluation is one add, one exp2 and one FMA, and the exp2 issues at about a quarter "
+                      "of the FMA rate (see attainable), so the kernel is bound by its exp2 issue well below "
+                      "the FMA peak",
         "traffic": load_traffic(),
         "algorithmic_equivalent": {
             "flops_per_eval": FLOPS_PER_EVAL_GRID_GAUSSIAN,
@@ -753,10 +787,24 @@ def main():
                     "scratch; not a hardware rate (the kernel shares row and lane partial sums)",
         },
     }
+    rates = probe_rates(dev)
+    roofline["attainable"] = {
+        "loop_evals_per_s": rates["loop_evals_per_s"],
+        "frac_of_loop": evals_per_s / rates["loop_evals_per_s"],
+        "fma_cycles_per_wave_instruction": rates["fma_cycles"],
+        "exp2_cycles_per_wave_instruction": rates["exp2_cycles"],
+        "note": "tq_probe_rates on this GPU, cycles at the nominal 2.4 GHz: independent FP32 FMA and exp2 "
+                "chains, and the grid loop's body alone (state + LDS table term, exp2, weighted FMA per "
+                "point, 16 points per trip) with none of the kernel's tables, rows, flushes or tail",
+    }
     if valu:
         flops_eval = valu["executed_fp32_flops_per_eval"]
         achieved = flops_eval * evals_per_s / 1e12
-        issue_s = (valu["valu_insts_per_eval"] * 4 + valu["trans_f32_insts_per_eval"] * 4) * N / (
+        trans = valu["trans_f32_insts_per_eval"]
+        other = valu["valu_insts_per_eval"] - trans
+        # issue time of the executed instructions at the rates measured on this
+        # GPU (every non-transcendental instruction at the FMA rate)
+        issue_s = (other * rates["fma_cycles"] + trans * rates["exp2_cycles"]) * N / (
             simd_count() * CLOCK_GHZ * 1e9)
         roofline.update({
             "achieved": achieved,
@@ -764,8 +812,9 @@ def main():
             "executed_fp32_flops_per_eval": flops_eval,
             "valu_issue_frac": issue_s / kernel_s,
             "valu_note": "executed work per evaluation from " + VALU_FILE.name + " (" + valu["command"] +
-                         "); valu_issue_frac = (VALU wave instructions x 4 cycles + transcendentals x 4 more) / "
-                         "(SIMDs x 2.4 GHz x measured time per launch)",
+                         "); valu_issue_frac = (non-transcendental VALU wave instructions x the measured FMA "
+                         "cycles + transcendentals x the measured exp2 cycles) / (SIMDs x 2.4 GHz x measured "
+                         "time per launch)",
         })
 
     cpu = None

@@ -292,6 +292,16 @@ int tq_sweep_backward(const tq_grid_desc* g, int32_t family, int64_t B, const vo
                       const void* domains, const void* grad_out, void* workspace, size_t workspace_bytes,
                       void* grad_a, void* grad_u, void* grad_domains, void* stream);
 
+/* ---- measurement --------------------------------------------------------
+ * No counterpart in the reference: a measurement hook for the benchmark,
+ * which states the fused grid kernel's rate against the device's own rate
+ * for the kernel's inner loop.  kind 0: independent FP32 FMA chains, 32
+ * instructions per lane and trip; 1: FP32 exp2 chains, likewise; 2: the
+ * grid loop's body (state + LDS table term, exp2, weighted FMA), 256 points
+ * per lane and trip.  Launches blocks x 256 threads for iters trips; out:
+ * device float[blocks * 256]. */
+int tq_probe_rates(int32_t kind, int64_t blocks, int32_t iters, float* out, void* stream);
+
 /* ---- VEGAS ----------------------------------------------------------------
  * Device-resident VEGAS+ state (reference: vegas_map.py, vegas_stratification.py,
  * vegas.py).  Every buffer is caller-owned device memory; the library only

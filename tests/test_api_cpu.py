@@ -506,3 +506,12 @@ def test_submit_packing_on_the_host():
         s.submit_grid("gauss_legendre", g, 1, n=n, domain=[[0.0, 1.0]])
     singles = np.concatenate([r.items for r in s.queued[-4:]])
     assert many.items.tobytes() == singles.tobytes()
+
+
+def test_probe_rates_rejects_bad_arguments_on_the_host():
+    L = _lib.load_library()
+    fake = ctypes.c_void_p(256)
+    assert L.tq_probe_rates(3, 8, 1, fake, None) == -1
+    assert L.tq_probe_rates(0, 0, 1, fake, None) == -1
+    assert L.tq_probe_rates(0, 8, 0, fake, None) == -1
+    assert L.tq_probe_rates(0, 8, 1, None, None) == -2

@@ -141,6 +141,7 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.POINTER(GridDesc), _I64, _P, _P, _P, _P, _P, _SZ, _P, _P, _P, _P],
     ),
+    "tq_probe_rates": (ctypes.c_int, [_I32, _I64, _I32, _P, _P]),
     "tq_sweep_forward": (ctypes.c_int, [ctypes.POINTER(GridDesc), _I32, _I64, _P, _P, _P, _P, _SZ, _P, _P, _P]),
     "tq_sweep_backward": (
         ctypes.c_int,
