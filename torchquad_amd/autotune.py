@@ -98,7 +98,7 @@ def autotune(batch, tiles=TILE_CANDIDATES, blocks=BLOCK_CANDIDATES, reps=3, use_
 
 
 # ---- plain fused grid kernel: items per wave -------------------------------
-IPW_CANDIDATES = (0, 256, 512, 768, 1024, 1536, 2048, 3072, 4096, 8192)
+IPW_CANDIDATES = (0, 256, 512, 768, 1024, 1280, 1536, 1792, 2048, 2304, 2560, 3072, 4096, 8192)
 
 
 def _grid_key(rule, family, dim, n, dtype):
