@@ -215,11 +215,16 @@ constexpr long long kUnitOverheadItems = 128;
 static void align_ipw(GridGeom& G, const void* kernel, int bw, size_t shmem, long long max_blocks) {
     const long long W = resident_blocks(kernel, bw * kWave, shmem) * bw;
     if (W <= 0 || G.items <= 0) return;
+    // as plan_geom: at least 16 items per unit (all of them when fewer), so
+    // that a small grid is not spread over thousands of nearly empty waves,
+    // each of whose blocks pays a table build and the completion fence
+    const long long min_ipw = G.items < 16 ? G.items : 16;
     long long best_cost = -1, best_ipw = G.ipw;
     for (long long r = 1; r <= 16; ++r) {
         long long per_lw = (r * W) / G.lane_waves;
         if (per_lw < 1) per_lw = 1;
-        const long long ipw = (G.items + per_lw - 1) / per_lw;
+        long long ipw = (G.items + per_lw - 1) / per_lw;
+        if (ipw < min_ipw) ipw = min_ipw;
         const long long units = ((G.items + ipw - 1) / ipw) * (long long)G.lane_waves;
         if ((units + bw - 1) / bw > max_blocks) continue;
         const long long cost = ((units + W - 1) / W) * (ipw + kUnitOverheadItems);
