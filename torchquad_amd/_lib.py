@@ -232,6 +232,8 @@ _lib_lock = threading.Lock()
 def load_library(path=None):
     """Load and bind the shared library (no GPU needed to load it)."""
     global _lib
+    if _lib is not None and path is None:  # bound once; read without the lock
+        return _lib
     with _lib_lock:
         if _lib is not None and path is None:
             return _lib
@@ -308,9 +310,12 @@ class _Workspace:
         self._bufs = {}
         self._lock = threading.Lock()
 
-    def get(self, nbytes, device, tag="main"):
-        key = (device.index if device.index is not None else torch.cuda.current_device(),
-               torch.cuda.current_stream(device).cuda_stream, tag)
+    def get(self, nbytes, device, tag="main", stream=None):
+        """The buffer of (device, stream, tag); stream: a raw stream handle
+        already looked up by the caller (default: torch's current stream)."""
+        if stream is None:
+            stream = torch.cuda.current_stream(device).cuda_stream
+        key = (device.index if device.index is not None else torch.cuda.current_device(), stream, tag)
         with self._lock:
             buf = self._bufs.get(key)
             if buf is None or buf.numel() < nbytes:

@@ -45,8 +45,21 @@ def grid_workspace(rule, integrand, dim, n, domain, gl=None, lane_axes=0, items_
     """A new workspace tensor sized for grid_integrate with these arguments
     (for callers that must own theirs, e.g. captured graphs)."""
     g, _ = _fused_grid_desc(rule, integrand, dim, n, domain.dtype, gl, lane_axes, items_per_wave)
-    nbytes = lib().tq_grid_workspace_size(ctypes.byref(g))
-    return torch.zeros(max(int(nbytes), 8), dtype=torch.uint8, device=domain.device)
+    return torch.zeros(max(_grid_ws_bytes(g), 8), dtype=torch.uint8, device=domain.device)
+
+
+_ws_sizes = {}
+
+
+def _grid_ws_bytes(g):
+    """tq_grid_workspace_size, remembered per geometry (it depends on the
+    descriptor's scalar fields only)."""
+    key = (g.dtype, g.rule, g.dim, g.n, g.lane_axes, g.items_per_wave)
+    nbytes = _ws_sizes.get(key)
+    if nbytes is None:
+        nbytes = int(lib().tq_grid_workspace_size(ctypes.byref(g)))
+        _ws_sizes[key] = nbytes
+    return nbytes
 
 
 def grid_integrate(rule, integrand, dim, n, domain, gl=None, lane_axes=0, items_per_wave=0, out_f64=None, ws=None):
@@ -55,9 +68,10 @@ def grid_integrate(rule, integrand, dim, n, domain, gl=None, lane_axes=0, items_
     ws: the caller's workspace (uint8 tensor); default the per-stream shared one."""
     require_device(domain)
     g, desc = _fused_grid_desc(rule, integrand, dim, n, domain.dtype, gl, lane_axes, items_per_wave)
-    nbytes = lib().tq_grid_workspace_size(ctypes.byref(g))
+    nbytes = _grid_ws_bytes(g)
+    stream = stream_ptr(domain.device)
     if ws is None:
-        ws = workspace.get(max(nbytes, 8), domain.device)
+        ws = workspace.get(max(nbytes, 8), domain.device, stream=stream)
     elif ws.numel() < nbytes:
         raise ValueError(f"workspace of {ws.numel()} bytes, {nbytes} needed")
     out = torch.empty((), dtype=domain.dtype, device=domain.device)
@@ -70,7 +84,7 @@ def grid_integrate(rule, integrand, dim, n, domain, gl=None, lane_axes=0, items_
         ws.numel(),
         ptr(out),
         ptr(out_f64),
-        stream_ptr(domain.device),
+        stream,
     )
     return out
 

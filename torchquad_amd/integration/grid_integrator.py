@@ -239,8 +239,8 @@ class GridIntegrator(BaseIntegrator):
 
         The reference traces calculate_grid and calculate_result with
         torch.jit and runs the integrand eagerly between them.  Here a
-        built-in integrand's whole step (the fused kernel and its finalize)
-        is captured into a HIP graph on the first call per (integrand, domain
+        built-in integrand's whole step (one launch of the fused kernel,
+        whose last workgroup forms the result) is captured into a HIP graph on the first call per (integrand, domain
         dtype and device); later calls copy the domain into the graph's
         domain buffer, replay the graph and return a copy of its result, so a
         repeated integral costs no host work beyond the replay.  As with the
