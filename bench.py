@@ -616,7 +616,7 @@ def bench_grad(c5, dev, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=65, help="Boole points per axis of the timed step")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)

@@ -62,8 +62,11 @@ __device__ __forceinline__ double batch_tile_general(const BatchDev<T>& D, const
                                             int lw1, long long t0, long long t1) {
     if (table) {
         const int lane = threadIdx.x & (kWave - 1);
-        for (int j = lane; j < D.nl; j += kWave)
-            loop_table_entry<T, FAM>(tab, j, ax, D.L, D.q, D.n, D.rule, glx, glw, D.p);
+        if (D.q == 2 && D.n <= 32)
+            pair_table_wave<T, FAM>(tab, ax, D.L, D.n, D.rule, glx, glw, D.p);
+        else
+            for (int j = lane; j < D.nl; j += kWave)
+                loop_table_entry<T, FAM>(tab, j, ax, D.L, D.q, D.n, D.rule, glx, glw, D.p);
     }
     // make this wave's LDS writes visible to all of its lanes
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -90,8 +93,13 @@ __device__ __forceinline__ double batch_tile_rowtab(const BatchDev<T>& D, const 
     T* rt = tab + (size_t)TabStride<FAM>::v * D.nl;
     if (D.m > 0 && L >= 0) {
         const int lane = threadIdx.x & (kWave - 1);
-        for (int j = lane; j < D.nl; j += kWave)
-            loop_table_entry<T, FAM>(tab, j, ax, L, D.q, n, D.rule, glx, glw, D.p);
+        // two loop axes (n <= 32 in a batch): nodes and weights formed once
+        // per axis, not once per entry
+        if (D.q == 2 && n <= 32)
+            pair_table_wave<T, FAM>(tab, ax, L, n, D.rule, glx, glw, D.p);
+        else
+            for (int j = lane; j < D.nl; j += kWave)
+                loop_table_entry<T, FAM>(tab, j, ax, L, D.q, n, D.rule, glx, glw, D.p);
         for (int e = lane; e < L * n; e += kWave) {
             const int k = e / n;
             row_table_entry<T, FAM>(rt, rw, k, e - k * n, ax[k], axis_c(D.p, k), n, D.rule, glx, glw, D.p);

@@ -122,8 +122,12 @@ __global__ __launch_bounds__(BW * kWave) void k_sweep_fwd(SweepGeom S, const T* 
     load_sweep_params<T, FAM>(g.d, g.n, g.rule, b, a, u, dom, &s_p, s_axis);
     __syncthreads();
     if (TABLE) {
-        for (int j = threadIdx.x; j < g.nl; j += NT)
-            loop_table_entry<T, FAM>(s_tab, j, s_axis, g.L, g.q, g.n, g.rule, glx, glw, s_p);
+        if (g.q == 2 && g.n <= 32)
+            pair_table_wave<T, FAM>(s_tab, s_axis, g.L, g.n, g.rule, glx, glw, s_p,
+                                    __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * kWave, NT);
+        else
+            for (int j = threadIdx.x; j < g.nl; j += NT)
+                loop_table_entry<T, FAM>(s_tab, j, s_axis, g.L, g.q, g.n, g.rule, glx, glw, s_p);
         if (ROWTAB)
             for (int e = threadIdx.x; e < g.L * g.n; e += NT) {
                 const int k = e / g.n;

@@ -90,13 +90,19 @@ static int make_geom(const tq_grid_desc* gd, GridGeom& G, long long merge_cap = 
 // Fused kernel
 // ---------------------------------------------------------------------------
 // Dynamic LDS of the fused kernel: [row weights: L*n FP64][loop table:
-// S*nl T][row contributions: (S-1)*L*n T] (the row parts only when ROWTAB).
+// S*nl T][row contributions: (S-1)*L*n T] (the row parts only when ROWTAB),
+// then for a two-axis loop, 8-byte aligned, the loop axes' weights (2n FP64)
+// and nodes (2n T) from which the table is built.
 template <typename T, int FAM>
-inline size_t fused_lds_bytes(const GridGeom& G, bool rowtab) {
+__host__ __device__ inline size_t fused_pair_offset(const GridGeom& G, bool rowtab) {
     constexpr int S = TabStride<FAM>::v;
     size_t b = (size_t)S * G.nl * sizeof(T);
     if (rowtab) b += (size_t)G.L * G.n * (sizeof(double) + (S - 1) * sizeof(T));
-    return b;
+    return (b + 7) & ~(size_t)7;
+}
+template <typename T, int FAM>
+inline size_t fused_lds_bytes(const GridGeom& G, bool rowtab) {
+    return fused_pair_offset<T, FAM>(G, rowtab) + (G.q == 2 ? (size_t)2 * G.n * (sizeof(double) + sizeof(T)) : 0);
 }
 constexpr size_t kFusedLdsMax = 48 * 1024;
 
@@ -117,8 +123,22 @@ __global__ __launch_bounds__(BW * kWave) void k_grid_fused(GridArgs<T> A) {
     T* s_tab = reinterpret_cast<T*>(smem_raw + (ROWTAB ? (size_t)L * n * sizeof(double) : 0));
     T* s_rt = s_tab + (size_t)TabStride<FAM>::v * g.nl;
     if (TABLE) {
-        for (int j = tid; j < g.nl; j += NT)
-            loop_table_entry<T, FAM>(s_tab, j, s_axis, L, g.q, n, rule, A.glx, A.glw, A.p);
+        // Two loop axes: their 2n nodes and weights first, so that each of
+        // the n^2 entries costs two folds and a product instead of a
+        // division for its indices and two node and weight evaluations.
+        const bool pair = g.q == 2;
+        double* s_pw = reinterpret_cast<double*>(smem_raw + fused_pair_offset<T, FAM>(g, ROWTAB));
+        T* s_px = reinterpret_cast<T*>(s_pw + 2 * n);
+        if (pair) {
+            for (int e = tid; e < 2 * n; e += NT) {
+                const int j = e < n ? e : e - n;
+                s_px[e] = (T)node_of(s_axis[e < n ? L : L + 1], j, n, rule, A.glx);
+                s_pw[e] = weight_of(rule, j, n, A.glw);
+            }
+        } else {
+            for (int j = tid; j < g.nl; j += NT)
+                loop_table_entry<T, FAM>(s_tab, j, s_axis, L, 1, n, rule, A.glx, A.glw, A.p);
+        }
         if (ROWTAB)
             for (int e = tid; e < L * n; e += NT) {
                 const int k = e / n;
@@ -126,6 +146,22 @@ __global__ __launch_bounds__(BW * kWave) void k_grid_fused(GridArgs<T> A) {
                                         A.p);
             }
         __syncthreads();
+        if (pair) {
+            const AxisC<T> c1 = axis_c(A.p, L), c2 = axis_c(A.p, L + 1);
+            // entry j = j1 n + j2, stepped by NT without a division per entry
+            int j1 = tid / n, j2 = tid - j1 * n;
+            const int s1 = NT / n, s2 = NT - s1 * n;
+            for (int j = tid; j < g.nl; j += NT) {
+                pair_table_entry<T, FAM>(s_tab, j, s_px[j1], s_px[n + j2], s_pw[j1], s_pw[n + j2], c1, c2, A.p);
+                j1 += s1;
+                j2 += s2;
+                if (j2 >= n) {
+                    j2 -= n;
+                    ++j1;
+                }
+            }
+            __syncthreads();
+        }
     }
     // readfirstlane: the wave index is uniform, so the tile loop runs on SGPRs
     const long long wave = (long long)blockIdx.x * BW + __builtin_amdgcn_readfirstlane(tid >> 6);

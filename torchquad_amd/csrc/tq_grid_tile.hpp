@@ -106,6 +106,58 @@ __device__ inline void loop_table_entry(T* tab, int j, const Axis* ax, int L, in
     tab[S * j + S - 1] = (T)w;
 }
 
+// Two-axis loop-table entry j = j1 n + j2 from the nodes of its two axes
+// (x1 on axis L, x2 on axis L + 1, already rounded to T as loop_table_entry
+// rounds node_of) and their FP64 weights: the same folds and product as
+// loop_table_entry with q = 2, so the entry is bit-identical, while the
+// nodes and weights are formed once per axis instead of once per entry.
+template <typename T, int FAM>
+__device__ inline void pair_table_entry(T* tab, int j, T x1, T x2, double w1, double w2, const AxisC<T>& c1,
+                                        const AxisC<T>& c2, const Params<T>& p) {
+    using F = Family<FAM, T>;
+    constexpr int S = TabStride<FAM>::v;
+    State<T> s = F::fold_c(F::zero(), c1, x1, p);
+    s = F::fold_c(s, c2, x2, p);
+    tab[S * j] = s.s0;
+    if (S == 3) tab[S * j + 1] = s.s1;
+    tab[S * j + S - 1] = (T)(w1 * w2);
+}
+
+// Two-axis loop table (axes L and L + 1, n <= 32) built by whole waves: the
+// calling wave stores entries first + lane, first + stride + lane, ... (one
+// wave: first 0, stride 64; the waves of a block: first 64 w, stride 64 BW).
+// Lane l < 32 forms node l of axis L and lane 32 + l node l of axis L + 1,
+// and each entry takes its two nodes and weights from those lanes
+// (ds_bpermute).  Every lane runs every trip, so the lanes read from are
+// active; only entries j < n^2 are stored.
+template <typename T, int FAM>
+__device__ inline void pair_table_wave(T* tab, const Axis* ax, int L, int n, int rule, const double* glx,
+                                       const double* glw, const Params<T>& p, int first = 0, int stride = kWave) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int jn = lane & 31;
+    const bool has = jn < n;
+    const int js = has ? jn : 0;  // in-range index for the lanes past n
+    const T xn = (T)node_of(ax[lane < 32 ? L : L + 1], js, n, rule, glx);
+    const double wn = weight_of(rule, js, n, glw);
+    const T xa = has ? xn : T(0);
+    const double wa = has ? wn : 0.0;
+    const AxisC<T> c1 = axis_c(p, L), c2 = axis_c(p, L + 1);
+    const int nl = n * n;
+    int j1 = (first + lane) / n, j2 = first + lane - j1 * n;
+    const int s1 = stride / n, s2 = stride - s1 * n;
+    for (int jb = first; jb < nl; jb += stride) {
+        const T x1 = __shfl(xa, j1 & 31), x2 = __shfl(xa, 32 + j2);
+        const double w1 = __shfl(wa, j1 & 31), w2 = __shfl(wa, 32 + j2);
+        if (jb + lane < nl) pair_table_entry<T, FAM>(tab, jb + lane, x1, x2, w1, w2, c1, c2, p);
+        j1 += s1;
+        j2 += s2;
+        if (j2 >= n) {
+            j2 -= n;
+            ++j1;
+        }
+    }
+}
+
 // Row-axis tables (ROWTAB): for the row axes k < L, entry k*n + j holds the
 // axis's contribution at node j (the state slots of loop_table_entry) in `rt` and
 // its weight in FP64 in `rw`, so a row's state and weight are merged from
@@ -264,63 +316,48 @@ __device__ inline double grid_tile(int d, int n, int nl, int m, int L, int rule,
     }
     const AxisC<T> acL = axis_c(p, L);
     long long t = t_begin;
-    long long row;
     int j;
-    // row odometer digits for axes [0, L); 32-bit divisions when the item
-    // index fits (64-bit division is a long instruction sequence)
-    int rdig[TQ_MAX_DIM];
+    // Row odometer: lane k of `dig` holds the digit of row axis k (k < L <
+    // 16), read with readlane at a uniform k.  Digits kept as a register array
+    // with its loops unrolled over all 16 possible axes made the compiler
+    // hoist 32 "k < L" masks out of the row loop, spilling SGPRs into a VGPR
+    // and raising the kernel to 67 VGPRs (6 waves per SIMD instead of 8).
+    int dig = 0;
     if (t < 0x7fffffffLL) {
+        // 32-bit divisions when the item index fits
         const unsigned un = (unsigned)n, unl = (unsigned)nl;
         unsigned r = (unsigned)t / unl;
         j = (int)((unsigned)t - r * unl);
-        row = r;
-#pragma unroll
-        for (int k = TQ_MAX_DIM - 1; k >= 0; --k) {
-            if (k < L) {
-                const unsigned q = r / un;
-                rdig[k] = (int)(r - q * un);
-                r = q;
-            } else {
-                rdig[k] = 0;
-            }
+        for (int k = L - 1; k >= 0; --k) {
+            const unsigned q = r / un;
+            dig = (lane == k) ? (int)(r - q * un) : dig;
+            r = q;
         }
     } else {
-        row = t / nl;
-        j = (int)(t - row * nl);
-        long long r = row;
-#pragma unroll
-        for (int k = TQ_MAX_DIM - 1; k >= 0; --k) {
-            if (k < L) {
-                rdig[k] = (int)(r % n);
-                r /= n;
-            } else {
-                rdig[k] = 0;
-            }
+        long long r = t / nl;
+        j = (int)(t - r * nl);
+        for (int k = L - 1; k >= 0; --k) {
+            const long long q = r / n;
+            dig = (lane == k) ? (int)(r - q * n) : dig;
+            r = q;
         }
     }
     while (t < t_end) {
         State<T> s_row = F::start(p);
         double w_row = w_lane;
-        if (ROWTAB) {
-            constexpr int S1 = TabStride<FAM>::v - 1;
-#pragma unroll
-            for (int k = 0; k < TQ_MAX_DIM; ++k) {
-                if (k < L) {
-                    const int e = k * n + rdig[k];
-                    State<T> term;
-                    term.s0 = rt[S1 * e];
-                    term.s1 = (S1 == 2) ? rt[S1 * e + 1] : T(0);
-                    s_row = F::merge(s_row, term);
-                    w_row *= rw[e];
-                }
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < TQ_MAX_DIM; ++k) {
-                if (k < L) {
-                    s_row = F::fold_c(s_row, axis_c(p, k), (T)node_of(ax[k], rdig[k], n, rule, glx), p);
-                    w_row *= weight_of(rule, rdig[k], n, glw);
-                }
+        for (int k = 0; k < L; ++k) {
+            const int dk = __builtin_amdgcn_readlane(dig, k);
+            if (ROWTAB) {
+                constexpr int S1 = TabStride<FAM>::v - 1;
+                const int e = k * n + dk;
+                State<T> term;
+                term.s0 = rt[S1 * e];
+                term.s1 = (S1 == 2) ? rt[S1 * e + 1] : T(0);
+                s_row = F::merge(s_row, term);
+                w_row *= rw[e];
+            } else {
+                s_row = F::fold_c(s_row, axis_c(p, k), (T)node_of(ax[k], dk, n, rule, glx), p);
+                w_row *= weight_of(rule, dk, n, glw);
             }
         }
         const State<T> s_rl = F::merge(s_row, s_lane);
@@ -358,12 +395,11 @@ __device__ inline double grid_tile(int d, int n, int nl, int m, int L, int rule,
         j = jend;
         if (j == nl) {
             j = 0;
-#pragma unroll
-            for (int k = TQ_MAX_DIM - 1; k >= 0; --k) {
-                if (k < L) {
-                    if (++rdig[k] < n) break;
-                    rdig[k] = 0;
-                }
+            for (int k = L - 1; k >= 0; --k) {
+                const int dk = __builtin_amdgcn_readlane(dig, k) + 1;
+                const bool wrap = dk >= n;
+                dig = (lane == k) ? (wrap ? 0 : dk) : dig;
+                if (!wrap) break;
             }
         }
     }
