@@ -39,11 +39,11 @@ constexpr int kRegionBig = 8192;
 constexpr int kBatchClasses = 3;  // 0, 1: row-table kernels; 2: general kernel
 
 // LDS bytes of an integral's tables in the row-table kernels:
-// [row weights L*n FP64][loop table S*nl T][row terms (S-1)*L*n T]
+// [row weights L*n FP64][loop table S*nl T][row terms slots*L*n T] (S = tab_stride, slots = state_slots)
 inline long long rowtab_bytes(int fam, int n, int nl, int L, int tsize) {
     const int S = tab_stride(fam);
     if (L < 0) return 0;
-    return (long long)L * n * 8 + (long long)S * nl * tsize + (long long)(S - 1) * L * n * tsize;
+    return (long long)L * n * 8 + (long long)S * nl * tsize + (long long)state_slots(fam) * L * n * tsize;
 }
 
 // Tile queues per class and the distance between their counters (words).

@@ -92,16 +92,17 @@ __device__ inline void load_sweep_params(int d, int n, int rule, long long b, co
 constexpr int kSweepRowTab = 1024;  // row-axis table entries (L*n) kept in LDS
 constexpr int kBwdTable = 1024;     // largest loop axis tabulated by the backward kernel
 
-// Dynamic LDS: [row weights: L*n FP64][loop table: 2 nl T][row terms: L*n T]
-// (every sweep family has one state slot, so table entries are (term, weight))
-template <typename T>
+// Dynamic LDS: [row weights: L*n FP64][loop table: S nl T][row terms: L*n T]
+// (every sweep family has one state slot; S = TabStride: 1 when the family
+// folds the weight into the entry, else 2)
+template <typename T, int FAM>
 inline size_t sweep_fwd_lds(const GridGeom& g, bool table, bool rowtab) {
-    size_t b = table ? (size_t)2 * g.nl * sizeof(T) : 0;
+    size_t b = table ? (size_t)TabStride<FAM>::v * g.nl * sizeof(T) : 0;
     if (rowtab) b += (size_t)g.L * g.n * (sizeof(double) + sizeof(T));
     return b;
 }
-static_assert(TabStride<TQ_FAM_GAUSSIAN_PEAK>::v == 2 && TabStride<TQ_FAM_PRODUCT_PEAK>::v == 2 &&
-                  TabStride<TQ_FAM_OSCILLATORY>::v == 2,
+static_assert(StateSlots<TQ_FAM_GAUSSIAN_PEAK>::v == 1 && StateSlots<TQ_FAM_PRODUCT_PEAK>::v == 1 &&
+                  StateSlots<TQ_FAM_OSCILLATORY>::v == 1,
               "sweep families keep one state slot");
 
 // BW waves per block: 16 with a two-axis loop (one block per integral
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(BW * kWave) void k_sweep_fwd(SweepGeom S, const T* 
     __shared__ double s_red[BW];
     double* s_rw = reinterpret_cast<double*>(smem_raw);
     T* s_tab = reinterpret_cast<T*>(smem_raw + (ROWTAB ? (size_t)S.g.L * S.g.n * sizeof(double) : 0));
-    T* s_rt = s_tab + 2 * (size_t)S.g.nl;
+    T* s_rt = s_tab + (size_t)TabStride<FAM>::v * S.g.nl;
     const GridGeom& g = S.g;
     const long long b = blockIdx.y;
     load_sweep_params<T, FAM>(g.d, g.n, g.rule, b, a, u, dom, &s_p, s_axis);
@@ -1001,11 +1002,11 @@ static int launch_sweep_fwd(const tq_grid_desc* gd, int64_t B, const void* a, co
     const int bw = wide ? 16 : kWavesPerBlock;
     const unsigned gx = (unsigned)((S.g.num_waves + bw - 1) / bw);
     if (S.g.use_table && S.g.L >= 1 && (long long)S.g.L * S.g.n <= kSweepRowTab) {
-        const size_t sh = sweep_fwd_lds<T>(S.g, true, true);
+        const size_t sh = sweep_fwd_lds<T, FAM>(S.g, true, true);
         if (wide) launch_fwd_kernel<T, FAM, true, true, 16>(S, B, a, u, dom, gd, ws, sh, gx, s);
         else launch_fwd_kernel<T, FAM, true, true, kWavesPerBlock>(S, B, a, u, dom, gd, ws, sh, gx, s);
     } else if (S.g.use_table) {
-        const size_t sh = sweep_fwd_lds<T>(S.g, true, false);
+        const size_t sh = sweep_fwd_lds<T, FAM>(S.g, true, false);
         if (wide) launch_fwd_kernel<T, FAM, true, false, 16>(S, B, a, u, dom, gd, ws, sh, gx, s);
         else launch_fwd_kernel<T, FAM, true, false, kWavesPerBlock>(S, B, a, u, dom, gd, ws, sh, gx, s);
     } else {

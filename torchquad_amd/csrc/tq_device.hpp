@@ -212,6 +212,8 @@ struct Family<TQ_FAM_GAUSSIAN_PEAK, T> {
     }
     __device__ static inline State<T> merge(State<T> a, State<T> b) { return {a.s0 + b.s0, T(0)}; }
     __device__ static inline T finish(State<T> s, const Params<T>&) { return tq_exp2(s.s0); }
+    // w 2^s = 2^(s + log2 w)
+    __device__ static inline T fold_weight(T s0, double w) { return (T)((double)s0 + log2(w)); }
 };
 
 // cos(2*pi*c1 + sum a_k x_k);  start carries the phase 2*pi*c1 (in c2).  In
@@ -241,6 +243,8 @@ struct Family<TQ_FAM_PRODUCT_PEAK, T> {
     }
     __device__ static inline State<T> merge(State<T> a, State<T> b) { return {a.s0 * b.s0, T(0)}; }
     __device__ static inline T finish(State<T> s, const Params<T>&) { return tq_rcp(s.s0); }
+    // w / s = 1 / (s / w)
+    __device__ static inline T fold_weight(T s0, double w) { return (T)((double)s0 / w); }
 };
 
 // exp(c1 + sum a_k x_k) = 2^(c1' + sum a'_k x_k), a' = a log2 e, c1' = c1 log2 e
@@ -254,6 +258,7 @@ struct Family<TQ_FAM_EXP_LINEAR, T> {
     }
     __device__ static inline State<T> merge(State<T> a, State<T> b) { return {a.s0 + b.s0, T(0)}; }
     __device__ static inline T finish(State<T> s, const Params<T>&) { return tq_exp2(s.s0); }
+    __device__ static inline T fold_weight(T s0, double w) { return (T)((double)s0 + log2(w)); }
 };
 
 // sum sin(a_k x_k + u_k)

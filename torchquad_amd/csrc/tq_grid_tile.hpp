@@ -78,12 +78,46 @@ __device__ __host__ inline double axis_scale(double a, double b, int n, int rule
 // folded for two), then the point's weight; per point the tile loop only
 // merges it into the row/lane state.  Families with a two-component state
 // take three slots per entry.
+//
+// Families whose finish is exp2 of an additive state (Gaussian peak,
+// exp-linear) or a reciprocal of a multiplicative one (product peak) carry
+// the weight inside the state instead (Family::fold_weight: + log2 w, or
+// / w), so an entry is the state slot alone and a point is merge, finish and
+// add.  The table's LDS reads then move 4 bytes per point instead of 8 (the
+// loop of the d=5 n=65 Boole Gaussian peak was bound by them: 142 -> 130 us
+// per launch), at the cost of one more rounding of the entry (at most half an
+// ulp of the exponent, the size of the rounding of state + entry itself).
+template <int FAM>
+struct StateSlots {
+    static constexpr int v = (FAM == TQ_FAM_DOUBLE_GAUSSIAN) ? 2 : 1;
+};
+template <int FAM>
+struct FoldsWeight {
+    static constexpr bool v = FAM == TQ_FAM_GAUSSIAN_PEAK || FAM == TQ_FAM_EXP_LINEAR || FAM == TQ_FAM_PRODUCT_PEAK;
+};
 template <int FAM>
 struct TabStride {
-    static constexpr int v = (FAM == TQ_FAM_DOUBLE_GAUSSIAN) ? 3 : 2;
+    static constexpr int v = StateSlots<FAM>::v + (FoldsWeight<FAM>::v ? 0 : 1);
 };
 
-__host__ __device__ inline int tab_stride(int fam) { return fam == TQ_FAM_DOUBLE_GAUSSIAN ? 3 : 2; }
+__host__ __device__ inline int state_slots(int fam) { return fam == TQ_FAM_DOUBLE_GAUSSIAN ? 2 : 1; }
+__host__ __device__ inline bool folds_weight(int fam) {
+    return fam == TQ_FAM_GAUSSIAN_PEAK || fam == TQ_FAM_EXP_LINEAR || fam == TQ_FAM_PRODUCT_PEAK;
+}
+__host__ __device__ inline int tab_stride(int fam) { return state_slots(fam) + (folds_weight(fam) ? 0 : 1); }
+
+// Stores table entry j from its state and FP64 weight (see StateSlots).
+template <typename T, int FAM>
+__device__ inline void store_table_entry(T* tab, int j, const State<T>& s, double w) {
+    constexpr int S = TabStride<FAM>::v;
+    if constexpr (FoldsWeight<FAM>::v) {
+        tab[S * j] = Family<FAM, T>::fold_weight(s.s0, w);
+    } else {
+        tab[S * j] = s.s0;
+        if (StateSlots<FAM>::v == 2) tab[S * j + 1] = s.s1;
+        tab[S * j + S - 1] = (T)w;
+    }
+}
 
 // Loop-table entry j when the loop runs over q consecutive axes L .. L+q-1
 // (q = 1 or 2; for q = 2, j = j1 n + j2 with j1 on axis L): the axes'
@@ -92,7 +126,6 @@ template <typename T, int FAM>
 __device__ inline void loop_table_entry(T* tab, int j, const Axis* ax, int L, int q, int n, int rule,
                                         const double* glx, const double* glw, const Params<T>& p) {
     using F = Family<FAM, T>;
-    constexpr int S = TabStride<FAM>::v;
     const int j1 = q == 2 ? j / n : j;
     State<T> s = F::fold_c(F::zero(), axis_c(p, L), (T)node_of(ax[L], j1, n, rule, glx), p);
     double w = weight_of(rule, j1, n, glw);
@@ -101,9 +134,7 @@ __device__ inline void loop_table_entry(T* tab, int j, const Axis* ax, int L, in
         s = F::fold_c(s, axis_c(p, L + 1), (T)node_of(ax[L + 1], j2, n, rule, glx), p);
         w *= weight_of(rule, j2, n, glw);
     }
-    tab[S * j] = s.s0;
-    if (S == 3) tab[S * j + 1] = s.s1;
-    tab[S * j + S - 1] = (T)w;
+    store_table_entry<T, FAM>(tab, j, s, w);
 }
 
 // Two-axis loop-table entry j = j1 n + j2 from the nodes of its two axes
@@ -115,12 +146,9 @@ template <typename T, int FAM>
 __device__ inline void pair_table_entry(T* tab, int j, T x1, T x2, double w1, double w2, const AxisC<T>& c1,
                                         const AxisC<T>& c2, const Params<T>& p) {
     using F = Family<FAM, T>;
-    constexpr int S = TabStride<FAM>::v;
     State<T> s = F::fold_c(F::zero(), c1, x1, p);
     s = F::fold_c(s, c2, x2, p);
-    tab[S * j] = s.s0;
-    if (S == 3) tab[S * j + 1] = s.s1;
-    tab[S * j + S - 1] = (T)(w1 * w2);
+    store_table_entry<T, FAM>(tab, j, s, w1 * w2);
 }
 
 // Two-axis loop table (axes L and L + 1, n <= 32) built by whole waves: the
@@ -166,7 +194,7 @@ template <typename T, int FAM>
 __device__ inline void row_table_entry(T* rt, double* rw, int k, int j, const Axis& ax, const AxisC<T>& c, int n,
                                        int rule, const double* glx, const double* glw, const Params<T>& p) {
     using F = Family<FAM, T>;
-    constexpr int S1 = TabStride<FAM>::v - 1;
+    constexpr int S1 = StateSlots<FAM>::v;
     const State<T> s = F::fold_c(F::zero(), c, (T)node_of(ax, j, n, rule, glx), p);
     const int e = k * n + j;
     rt[S1 * e] = s.s0;
@@ -348,7 +376,7 @@ __device__ inline double grid_tile(int d, int n, int nl, int m, int L, int rule,
         for (int k = 0; k < L; ++k) {
             const int dk = __builtin_amdgcn_readlane(dig, k);
             if (ROWTAB) {
-                constexpr int S1 = TabStride<FAM>::v - 1;
+                constexpr int S1 = StateSlots<FAM>::v;
                 const int e = k * n + dk;
                 State<T> term;
                 term.s0 = rt[S1 * e];
@@ -373,20 +401,21 @@ __device__ inline double grid_tile(int d, int n, int nl, int m, int L, int rule,
             // (d=5 n=65: 157 -> 149 us per launch against 4 points per trip)
 #pragma unroll 16
             for (int jj = jb; jj < je; ++jj) {
-                T f, wL;
                 if (TABLE) {
                     constexpr int S = TabStride<FAM>::v;
                     State<T> term;
                     term.s0 = tab[S * jj];
-                    term.s1 = (S == 3) ? tab[S * jj + 1] : T(0);
-                    wL = tab[S * jj + S - 1];
-                    f = F::finish(F::merge(s_rl, term), p);
+                    term.s1 = (StateSlots<FAM>::v == 2) ? tab[S * jj + 1] : T(0);
+                    const T f = F::finish(F::merge(s_rl, term), p);
+                    if constexpr (FoldsWeight<FAM>::v)
+                        fa += f;  // the weight is inside f
+                    else
+                        fa = fma(tab[S * jj + S - 1], f, fa);
                 } else {
                     const T xL = (T)node_of(ax[L], jj, n, rule, glx);
-                    wL = (T)weight_of(rule, jj, n, glw);
-                    f = F::finish(F::fold_c(s_rl, acL, xL, p), p);
+                    const T wL = (T)weight_of(rule, jj, n, glw);
+                    fa = fma(wL, F::finish(F::fold_c(s_rl, acL, xL, p), p), fa);
                 }
-                fa = fma(wL, f, fa);
             }
             racc += (double)fa;
         }
