@@ -91,10 +91,10 @@ int tq_device_count(void);
  * boole.py:30-48, gaussian.py:132-144) for built-in integrands.
  * domain: device [dim][2] of dtype.  out: device scalar of dtype.
  * out_f64 (optional, may be NULL): device scalar receiving the FP64 value.
- * workspace: tq_grid_workspace_size() bytes; its first 256 bytes are a
- * completion counter (the last workgroup to finish sums the partials, so one
+ * workspace: tq_grid_workspace_size() bytes; its first 4352 bytes hold
+ * completion counters (the last workgroup to finish sums the partials, so one
  * launch does the whole integral) that must be zero before the first call
- * with that workspace -- clear it once after allocating -- and that every
+ * with that workspace -- clear them once after allocating -- and that every
  * call leaves zero. */
 size_t tq_grid_workspace_size(const tq_grid_desc* g);
 int tq_grid_integrate(const tq_grid_desc* g, const tq_integrand* f, const void* domain,
@@ -107,7 +107,7 @@ int tq_grid_integrate(const tq_grid_desc* g, const tq_integrand* f, const void* 
  * completion counter and block partials live in a static device array of
  * 1 MiB per device (zero from load; every call leaves the counter zero), each device's
  * address looked up once on that device's first call; the call allocates
- * nothing.  Grids with tq_grid_workspace_size() > 1 MiB (more than 131040
+ * nothing.  Grids with tq_grid_workspace_size() > 1 MiB (more than 130528
  * workgroups) return TQ_E_WORKSPACE.
  * Results equal tq_grid_integrate's with a workspace of
  * tq_grid_workspace_size() bytes.  Every call on a device shares that array:

@@ -16,9 +16,6 @@
 #include <mutex>
 
 #include "tq_host.hpp"
-#ifndef TQ_DONE_STRIDE_WORDS
-#define TQ_DONE_STRIDE_WORDS 64
-#endif
 #include "tq_grid_tile.hpp"
 
 namespace tq {
@@ -47,7 +44,7 @@ struct GridArgs {
 // for all workgroups serialised their atomics at about 15 ns each (10112
 // workgroups: 148 us for 3.9e7 evaluations).
 constexpr int kDoneGroups = 16;
-constexpr int kDoneStride = TQ_DONE_STRIDE_WORDS;  // words between counters
+constexpr int kDoneStride = 64;  // words between counters (256 bytes; 4 KB measured the same)
 constexpr int kDoneTop = kDoneGroups * kDoneStride;
 constexpr size_t kWsHeader = ((size_t)(kDoneTop + 1) * sizeof(unsigned int) + 255) & ~(size_t)255;
 
