@@ -88,7 +88,7 @@ def probe_rates(dev):
     out = torch.empty(blocks * 256, dtype=torch.float32, device=dev)
     waves_per_simd = blocks * 4 / (4 * cus)
     res = {}
-    for kind, iters in ((0, 4000), (1, 1000), (2, 200)):
+    for kind, iters in ((0, 4000), (1, 1000), (2, 200), (3, 200)):
         for _ in range(2):
             call("tq_probe_rates", kind, blocks, 5, ctypes.c_void_p(out.data_ptr()), stream_ptr(dev))
         torch.cuda.synchronize()
@@ -101,7 +101,7 @@ def probe_rates(dev):
         if kind < 2:
             res["fma_cycles" if kind == 0 else "exp2_cycles"] = sec * CLOCK_GHZ * 1e9 / (waves_per_simd * iters * 32)
         else:
-            res["loop_evals_per_s"] = blocks * 256 * iters * 256 / sec
+            res["loop_evals_per_s" if kind == 3 else "loop_weighted_evals_per_s"] = blocks * 256 * iters * 256 / sec
     return res
 
 
@@ -793,9 +793,12 @@ def main():
         "frac_of_loop": evals_per_s / rates["loop_evals_per_s"],
         "fma_cycles_per_wave_instruction": rates["fma_cycles"],
         "exp2_cycles_per_wave_instruction": rates["exp2_cycles"],
+        "loop_weighted_evals_per_s": rates["loop_weighted_evals_per_s"],
         "note": "tq_probe_rates on this GPU, cycles at the nominal 2.4 GHz: independent FP32 FMA and exp2 "
-                "chains, and the grid loop's body alone (state + LDS table term, exp2, weighted FMA per "
-                "point, 16 points per trip) with none of the kernel's tables, rows, flushes or tail",
+                "chains, and the grid loop's body alone as the kernel runs it for this family (state + "
+                "4-byte LDS table term with the weight folded in, exp2, add, 16 points per trip) with none "
+                "of the kernel's tables, rows, flushes or tail; loop_weighted_evals_per_s: the same body "
+                "with (term, weight) entries and a weighted FMA, the form of the non-folding families",
     }
     if valu:
         flops_eval = valu["executed_fp32_flops_per_eval"]

@@ -297,9 +297,10 @@ int tq_sweep_backward(const tq_grid_desc* g, int32_t family, int64_t B, const vo
  * which states the fused grid kernel's rate against the device's own rate
  * for the kernel's inner loop.  kind 0: independent FP32 FMA chains, 32
  * instructions per lane and trip; 1: FP32 exp2 chains, likewise; 2: the
- * grid loop's body (state + LDS table term, exp2, weighted FMA), 256 points
- * per lane and trip.  Launches blocks x 256 threads for iters trips; out:
- * device float[blocks * 256]. */
+ * grid loop's body with (term, weight) table entries (state + LDS table
+ * term, exp2, weighted FMA), 256 points per lane and trip; 3: the body with
+ * the weight folded into 4-byte entries (state + term, exp2, add).  Launches
+ * blocks x 256 threads for iters trips; out: device float[blocks * 256]. */
 int tq_probe_rates(int32_t kind, int64_t blocks, int32_t iters, float* out, void* stream);
 
 /* ---- VEGAS ----------------------------------------------------------------

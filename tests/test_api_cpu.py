@@ -511,7 +511,7 @@ def test_submit_packing_on_the_host():
 def test_probe_rates_rejects_bad_arguments_on_the_host():
     L = _lib.load_library()
     fake = ctypes.c_void_p(256)
-    assert L.tq_probe_rates(3, 8, 1, fake, None) == -1
+    assert L.tq_probe_rates(4, 8, 1, fake, None) == -1
     assert L.tq_probe_rates(0, 0, 1, fake, None) == -1
     assert L.tq_probe_rates(0, 8, 0, fake, None) == -1
     assert L.tq_probe_rates(0, 8, 1, None, None) == -2

@@ -230,15 +230,17 @@ __global__ __launch_bounds__(BW * kWave) void k_grid_fused(GridArgs<T> A) {
 }
 
 // Blocks of `kernel` (`threads` per block, `shmem` bytes of dynamic LDS)
-// that stay resident on the current device at once; 0 if unknown.  Answers are
-// kept per thread for the last few (device, kernel, shmem) keys, since the
-// runtime query costs more than a small launch.
-static long long resident_blocks(const void* kernel, int threads, size_t shmem) {
+// that stay resident on the current device at once, and the device's CU
+// count; 0 if unknown.  Answers are kept per thread for the last few
+// (device, kernel, shmem) keys, since the runtime query costs more than a
+// small launch.
+static long long resident_blocks(const void* kernel, int threads, size_t shmem, int* cus_out) {
     struct Entry {
         int dev;
         const void* kernel;
         size_t shmem;
         long long blocks;
+        int cus;
     };
     constexpr int kEntries = 16;
     thread_local Entry cache[kEntries] = {};
@@ -246,7 +248,10 @@ static long long resident_blocks(const void* kernel, int threads, size_t shmem) 
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     for (const Entry& e : cache)
-        if (e.kernel == kernel && e.dev == dev && e.shmem == shmem) return e.blocks;
+        if (e.kernel == kernel && e.dev == dev && e.shmem == shmem) {
+            *cus_out = e.cus;
+            return e.blocks;
+        }
     int cus = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, shmem) != hipSuccess) {
@@ -254,43 +259,70 @@ static long long resident_blocks(const void* kernel, int threads, size_t shmem) 
         return 0;
     }
     const long long r = (long long)cus * per_cu;
-    cache[next] = Entry{dev, kernel, shmem, r};
+    cache[next] = Entry{dev, kernel, shmem, r, cus};
     next = (next + 1) % kEntries;
+    *cus_out = cus;
     return r;
 }
 
 // Workspace of the fused kernel, in block partials: at least this many, so
 // that the automatic items per wave below can fill whole rounds.
 constexpr long long kFusedMinBlocks = 8192;
-// Per-wave cost of a work unit beyond its items (set-up, tables), in items.
-constexpr long long kUnitOverheadItems = 128;
 
-// Automatic items per wave for a launch of `kernel`: the number of
-// resident rounds r (1..16) whose grid of units, each lane wave's items cut
-// into floor(r W / lane_waves) chunks (W resident waves), minimises
-// rounds x (items per unit + overhead), so the last round is not left
-// mostly idle.  Keeps G as it is when the occupancy is unknown.
-static void align_ipw(GridGeom& G, const void* kernel, int bw, size_t shmem, long long max_blocks) {
-    const long long W = resident_blocks(kernel, bw * kWave, shmem) * bw;
-    if (W <= 0 || G.items <= 0) return;
-    // as plan_geom: at least 16 items per unit (all of them when fewer), so
-    // that a small grid is not spread over thousands of nearly empty waves,
-    // each of whose blocks pays a table build and the completion fence
-    const long long min_ipw = G.items < 16 ? G.items : 16;
-    long long best_cost = -1, best_ipw = G.ipw;
+// Launch-time model behind the automatic items per wave, fitted to the FP32
+// Boole Gaussian-peak kernel on MI355X (d = 3..6, n = 17..129, items per
+// wave 16..4096): a launch takes about
+//   blocks x kBlockNs  (each workgroup's completion fence and counter, which
+//                       the device serialises: 10112 workgroups of 16-item
+//                       waves took 112 us for 3.9e7 evaluations)
+//   + sum over rounds of ipw x kItemNs x (w + kItemLatency)
+// where a round is one fill of the W resident waves and w its waves per
+// SIMD: a wave's item (64 points) takes kItemNs (w + kItemLatency) when w
+// waves share the SIMD -- latency-bound alone, issue-bound with many.
+constexpr double kBlockNs = 10.0;
+constexpr double kItemNs = 5.6;
+constexpr double kItemLatency = 1.28;
+
+// Relative time per item of a family and dtype (FP64 transcendentals are
+// software sequences; the oscillatory cosine is a polynomial).
+inline double item_scale(int fam, int tsize) {
+    double s = fam == TQ_FAM_OSCILLATORY ? 1.5 : (fam == TQ_FAM_DOUBLE_GAUSSIAN ? 2.0 : 1.0);
+    return tsize == 8 ? 6.0 * s : s;
+}
+
+// Automatic items per wave for a launch of `kernel`: the candidate with the
+// least modelled time -- the counts that fill r = 1..16 resident rounds (each
+// lane wave's items cut into floor(r W / lane_waves) chunks) and the powers
+// of two up to the item count.  Keeps G as it is when the occupancy is unknown.
+static void align_ipw(GridGeom& G, const void* kernel, int bw, size_t shmem, long long max_blocks, double scale) {
+    int cus = 0;
+    const long long W = resident_blocks(kernel, bw * kWave, shmem, &cus) * bw;
+    if (W <= 0 || cus <= 0 || G.items <= 0) return;
+    const double simds = 4.0 * cus;
+    const double item_ns = kItemNs * scale;
+    double best_t = -1.0;
+    long long best_ipw = G.ipw;
+    auto consider = [&](long long ipw) {
+        if (ipw < 1) ipw = 1;
+        if (ipw > G.items) ipw = G.items;
+        const long long waves = ((G.items + ipw - 1) / ipw) * (long long)G.lane_waves;
+        const long long blocks = (waves + bw - 1) / bw;
+        if (blocks > max_blocks) return;
+        const long long full = waves / W, rest = waves % W;
+        double t = kBlockNs * (double)blocks + (double)full * (double)ipw * item_ns * ((double)W / simds + kItemLatency);
+        if (rest > 0) t += (double)ipw * item_ns * ((double)rest / simds + kItemLatency);
+        if (best_t < 0.0 || t < best_t) {
+            best_t = t;
+            best_ipw = ipw;
+        }
+    };
     for (long long r = 1; r <= 16; ++r) {
         long long per_lw = (r * W) / G.lane_waves;
         if (per_lw < 1) per_lw = 1;
-        long long ipw = (G.items + per_lw - 1) / per_lw;
-        if (ipw < min_ipw) ipw = min_ipw;
-        const long long units = ((G.items + ipw - 1) / ipw) * (long long)G.lane_waves;
-        if ((units + bw - 1) / bw > max_blocks) continue;
-        const long long cost = ((units + W - 1) / W) * (ipw + kUnitOverheadItems);
-        if (best_cost < 0 || cost < best_cost) {
-            best_cost = cost;
-            best_ipw = ipw;
-        }
+        consider((G.items + per_lw - 1) / per_lw);
     }
+    for (long long p = 16; p < G.items; p *= 2) consider(p);
+    consider(G.items);
     G.ipw = best_ipw;
     G.chunks = (G.items + best_ipw - 1) / best_ipw;
     G.num_waves = G.chunks * (long long)G.lane_waves;
@@ -303,7 +335,7 @@ template <typename T, int FAM, bool TABLE, bool ROWTAB, int BW>
 static void launch_fused(GridGeom& G, GridArgs<T>& A, size_t shmem, bool auto_ipw, long long max_blocks,
                          hipStream_t st) {
     const void* kern = (const void*)(k_grid_fused<T, FAM, TABLE, ROWTAB, BW>);
-    if (auto_ipw) align_ipw(G, kern, BW, shmem, max_blocks);
+    if (auto_ipw) align_ipw(G, kern, BW, shmem, max_blocks, item_scale(FAM, (int)sizeof(T)));
     G.blocks = (G.num_waves + BW - 1) / BW;
     A.g = G;
     hipLaunchKernelGGL((k_grid_fused<T, FAM, TABLE, ROWTAB, BW>), dim3((unsigned)G.blocks), dim3(BW * kWave), shmem,
