@@ -624,7 +624,8 @@ def main():
     ap.add_argument("--no-mc", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the VEGAS, batched and gradient lines")
     ap.add_argument("--ipw", type=int, default=0,
-                    help="items per wave of the timed fused kernel (0: tune before timing)")
+                    help="items per wave of the timed fused kernel (0: tune before timing; with "
+                    "--traffic-probe, the library's automatic choice)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run only a 256 MiB elementwise copy (counter calibration) and the timed step, "
                     "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes")
@@ -705,7 +706,9 @@ def main():
         from torchquad_amd.autotune import set_grid_ipw
 
         set_grid_ipw(_lib.RULE_BOOLE, f.descriptor(5).family, 5, n, dom.dtype, args.ipw)
-    ipw, _ = tune_grid(_lib.RULE_BOOLE, f, 5, n, dom)
+    # counter passes (--traffic-probe) keep the library's automatic choice
+    # unless --ipw fixes one, so that every pass sees the same launch
+    ipw = args.ipw if args.traffic_probe else tune_grid(_lib.RULE_BOOLE, f, 5, n, dom)[0]
 
     for _ in range(args.warmup):
         boole.integrate(f, 5, N=N, integration_domain=dom)

@@ -98,7 +98,7 @@ static int make_geom(const tq_grid_desc* gd, GridGeom& G, long long merge_cap = 
 // ---------------------------------------------------------------------------
 // Dynamic LDS of the fused kernel: [row weights: L*n FP64][loop table:
 // S*nl T][row contributions: slots*L*n T] (the row parts only when ROWTAB),
-// then for a two-axis loop, 8-byte aligned, the loop axes' weights (2n FP64)
+// then for a two-axis loop, 8-byte aligned, the loop axes' table weights (2n FP64: weights or their keys)
 // and nodes (2n T) from which the table is built.
 template <typename T, int FAM>
 __host__ __device__ inline size_t fused_pair_offset(const GridGeom& G, bool rowtab) {
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(BW * kWave) void k_grid_fused(GridArgs<T> A) {
             for (int e = tid; e < 2 * n; e += NT) {
                 const int j = e < n ? e : e - n;
                 s_px[e] = (T)node_of(s_axis[e < n ? L : L + 1], j, n, rule, A.glx);
-                s_pw[e] = weight_of(rule, j, n, A.glw);
+                s_pw[e] = table_weight<T, FAM>(weight_of(rule, j, n, A.glw));
             }
         } else {
             for (int j = tid; j < g.nl; j += NT)

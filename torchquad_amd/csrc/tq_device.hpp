@@ -212,8 +212,11 @@ struct Family<TQ_FAM_GAUSSIAN_PEAK, T> {
     }
     __device__ static inline State<T> merge(State<T> a, State<T> b) { return {a.s0 + b.s0, T(0)}; }
     __device__ static inline T finish(State<T> s, const Params<T>&) { return tq_exp2(s.s0); }
-    // w 2^s = 2^(s + log2 w)
-    __device__ static inline T fold_weight(T s0, double w) { return (T)((double)s0 + log2(w)); }
+    // the weight inside the state: w 2^s = 2^(s + log2 w); the key of a
+    // product of weights is the sum of their keys
+    __device__ static inline double weight_key(double w) { return log2(w); }
+    __device__ static inline double combine_keys(double a, double b) { return a + b; }
+    __device__ static inline T fold_key(T s0, double key) { return (T)((double)s0 + key); }
 };
 
 // cos(2*pi*c1 + sum a_k x_k);  start carries the phase 2*pi*c1 (in c2).  In
@@ -243,8 +246,10 @@ struct Family<TQ_FAM_PRODUCT_PEAK, T> {
     }
     __device__ static inline State<T> merge(State<T> a, State<T> b) { return {a.s0 * b.s0, T(0)}; }
     __device__ static inline T finish(State<T> s, const Params<T>&) { return tq_rcp(s.s0); }
-    // w / s = 1 / (s / w)
-    __device__ static inline T fold_weight(T s0, double w) { return (T)((double)s0 / w); }
+    // the weight inside the state: w / s = 1 / (s (1 / w))
+    __device__ static inline double weight_key(double w) { return 1.0 / w; }
+    __device__ static inline double combine_keys(double a, double b) { return a * b; }
+    __device__ static inline T fold_key(T s0, double key) { return (T)((double)s0 * key); }
 };
 
 // exp(c1 + sum a_k x_k) = 2^(c1' + sum a'_k x_k), a' = a log2 e, c1' = c1 log2 e
@@ -258,7 +263,9 @@ struct Family<TQ_FAM_EXP_LINEAR, T> {
     }
     __device__ static inline State<T> merge(State<T> a, State<T> b) { return {a.s0 + b.s0, T(0)}; }
     __device__ static inline T finish(State<T> s, const Params<T>&) { return tq_exp2(s.s0); }
-    __device__ static inline T fold_weight(T s0, double w) { return (T)((double)s0 + log2(w)); }
+    __device__ static inline double weight_key(double w) { return log2(w); }
+    __device__ static inline double combine_keys(double a, double b) { return a + b; }
+    __device__ static inline T fold_key(T s0, double key) { return (T)((double)s0 + key); }
 };
 
 // sum sin(a_k x_k + u_k)

@@ -81,9 +81,9 @@ __device__ __host__ inline double axis_scale(double a, double b, int n, int rule
 //
 // Families whose finish is exp2 of an additive state (Gaussian peak,
 // exp-linear) or a reciprocal of a multiplicative one (product peak) carry
-// the weight inside the state instead (Family::fold_weight: + log2 w, or
-// / w), so an entry is the state slot alone and a point is merge, finish and
-// add.  The table's LDS reads then move 4 bytes per point instead of 8 (the
+// the weight inside the state instead (Family::fold_key with the weight's
+// key: + log2 w, or * 1/w), so an entry is the state slot alone and a point
+// is merge, finish and add.  The table's LDS reads then move 4 bytes per point instead of 8 (the
 // loop of the d=5 n=65 Boole Gaussian peak was bound by them: 142 -> 130 us
 // per launch), at the cost of one more rounding of the entry (at most half an
 // ulp of the exponent, the size of the rounding of state + entry itself).
@@ -106,12 +106,25 @@ __host__ __device__ inline bool folds_weight(int fam) {
 }
 __host__ __device__ inline int tab_stride(int fam) { return state_slots(fam) + (folds_weight(fam) ? 0 : 1); }
 
-// Stores table entry j from its state and FP64 weight (see StateSlots).
+// A table build's FP64 value of a weight: the weight, or its key when the
+// family folds it into the state; the value of a product of weights.
+template <typename T, int FAM>
+__device__ inline double table_weight(double w) {
+    if constexpr (FoldsWeight<FAM>::v) return Family<FAM, T>::weight_key(w);
+    else return w;
+}
+template <typename T, int FAM>
+__device__ inline double combine_weights(double a, double b) {
+    if constexpr (FoldsWeight<FAM>::v) return Family<FAM, T>::combine_keys(a, b);
+    else return a * b;
+}
+
+// Stores table entry j from its state and the table_weight value w.
 template <typename T, int FAM>
 __device__ inline void store_table_entry(T* tab, int j, const State<T>& s, double w) {
     constexpr int S = TabStride<FAM>::v;
     if constexpr (FoldsWeight<FAM>::v) {
-        tab[S * j] = Family<FAM, T>::fold_weight(s.s0, w);
+        tab[S * j] = Family<FAM, T>::fold_key(s.s0, w);
     } else {
         tab[S * j] = s.s0;
         if (StateSlots<FAM>::v == 2) tab[S * j + 1] = s.s1;
@@ -134,21 +147,21 @@ __device__ inline void loop_table_entry(T* tab, int j, const Axis* ax, int L, in
         s = F::fold_c(s, axis_c(p, L + 1), (T)node_of(ax[L + 1], j2, n, rule, glx), p);
         w *= weight_of(rule, j2, n, glw);
     }
-    store_table_entry<T, FAM>(tab, j, s, w);
+    store_table_entry<T, FAM>(tab, j, s, table_weight<T, FAM>(w));
 }
 
 // Two-axis loop-table entry j = j1 n + j2 from the nodes of its two axes
 // (x1 on axis L, x2 on axis L + 1, already rounded to T as loop_table_entry
-// rounds node_of) and their FP64 weights: the same folds and product as
-// loop_table_entry with q = 2, so the entry is bit-identical, while the
-// nodes and weights are formed once per axis instead of once per entry.
+// rounds node_of) and their table_weight values: the same folds as
+// loop_table_entry with q = 2, while the nodes and weights (or weight keys)
+// are formed once per axis instead of once per entry.
 template <typename T, int FAM>
 __device__ inline void pair_table_entry(T* tab, int j, T x1, T x2, double w1, double w2, const AxisC<T>& c1,
                                         const AxisC<T>& c2, const Params<T>& p) {
     using F = Family<FAM, T>;
     State<T> s = F::fold_c(F::zero(), c1, x1, p);
     s = F::fold_c(s, c2, x2, p);
-    store_table_entry<T, FAM>(tab, j, s, w1 * w2);
+    store_table_entry<T, FAM>(tab, j, s, combine_weights<T, FAM>(w1, w2));
 }
 
 // Two-axis loop table (axes L and L + 1, n <= 32) built by whole waves: the
@@ -166,7 +179,7 @@ __device__ inline void pair_table_wave(T* tab, const Axis* ax, int L, int n, int
     const bool has = jn < n;
     const int js = has ? jn : 0;  // in-range index for the lanes past n
     const T xn = (T)node_of(ax[lane < 32 ? L : L + 1], js, n, rule, glx);
-    const double wn = weight_of(rule, js, n, glw);
+    const double wn = table_weight<T, FAM>(weight_of(rule, js, n, glw));
     const T xa = has ? xn : T(0);
     const double wa = has ? wn : 0.0;
     const AxisC<T> c1 = axis_c(p, L), c2 = axis_c(p, L + 1);
