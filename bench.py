@@ -211,7 +211,7 @@ def bench_batched(dev, steps):
 
     items = genz_batch(8192, seed=0)
     b = Batch(items, dtype=torch.float32, device=dev)
-    (tile, blocks), _ = autotune(b, reps=3, use_cache=False)
+    tile, _ = autotune(b, reps=3, use_cache=False)
     b.run()
     acc = batch_accuracy(items, b.out.double().cpu().numpy())
     g = b.graph()
@@ -246,7 +246,6 @@ def bench_batched(dev, steps):
         "evals_per_s": b.evals / (ms_batch / 1e3),
         "ms_per_batch": ms_batch,
         "tile_points": tile,
-        "blocks": blocks,
         "plain_loop_ms": ms_plain,
         "plain_loop_evals_per_s": b.evals / (ms_plain / 1e3),
         "speedup_vs_plain_loop": ms_plain / ms_batch,

@@ -203,7 +203,7 @@ int tq_uniform(uint64_t seed, uint64_t offset, int32_t dtype, int64_t rows, int3
 
 /* ---- batched form ------------------------------------------------------
  * Many independent tensor-product integrals (mixed rule, family, dim, n,
- * domain, parameters) in one persistent kernel.  The reference has no batched
+ * domain, parameters) in one launch per LDS class.  The reference has no batched
  * integrator; its nearest relatives are vectorised integrands
  * (base_integrator.py:77-89) and the tutorial's Batch1DIntegrator
  * (docs tutorial.rst:800-888).  Items carry the per-axis n already rounded
@@ -240,29 +240,23 @@ int tq_batch_plan_sizes(const tq_batch_item* items, int64_t count, int32_t dtype
  * wave (or no loop axis), 1: within 8 KB, 2: larger (general kernel). */
 int tq_batch_plan(const tq_batch_item* items, int64_t count, int32_t dtype, int64_t tile_points, void* desc_out,
                   tq_batch_tile* tiles_out, int64_t* tile_begin, int64_t* tile_count, int64_t* class_offsets);
-/* Work counters of tq_batch_run, in 32-bit words: per class, 16 counters
- * 4 KB apart, so that waves pulling tiles spread their atomics over 16
- * memory locations instead of serialising on one. */
-#define TQ_BATCH_COUNTER_WORDS (3 * 16 * 1024)
-/* Runs a plan whose arrays were copied to the device: clears the counters,
- * one persistent kernel per non-empty class of `blocks` workgroups (0 =
- * automatic) pulling that class's tiles, then out[i] (dtype) / out64[i] =
- * integral i.  tile_partials: [n_tiles]; counter: counter_words >=
- * TQ_BATCH_COUNTER_WORDS;
+/* Runs a plan whose arrays were copied to the device: one kernel per
+ * non-empty class, one wave per tile in the plan's order (longest integrals
+ * first, so the hardware dispatcher hands out the large tiles first), then
+ * out[i] (dtype) / out64[i] = integral i.  tile_partials: [n_tiles];
  * class_offsets: host [4] from tq_batch_plan, or NULL to run every tile on
- * the general kernel.  Each class's tiles form 16 interleaved queues (tile
- * t in queue t mod 16); a wave drains its home queue, then the others.
- * gl_nodes / gl_weights: every Gauss-Legendre item's table at its
- * gl_offset, gl_len entries each; gl_pairs32 (optional, F32 batches only,
- * else ignored): the same tables as interleaved FP32 (node, weight) pairs,
- * pair k at [2 (gl_offset + k)], read by one-axis items instead of the FP64
- * tables.  A Gauss-Legendre item whose gl_offset + n exceeds gl_len (or with
- * NULL tables) reads no table and its result is NaN. */
+ * the general kernel.  gl_nodes / gl_weights: every Gauss-Legendre item's
+ * table at its gl_offset, gl_len entries each; gl_pairs32 (optional, F32
+ * batches only, else ignored): the same tables as interleaved FP32 (node,
+ * weight) pairs, pair k at [2 (gl_offset + k)], read by one-axis items
+ * instead of the FP64 tables.  A Gauss-Legendre item whose gl_offset + n
+ * exceeds gl_len (or with NULL tables) reads no table and its result is
+ * NaN.  The run needs no workspace beyond tile_partials and no clearing, so
+ * it can be captured in a HIP graph and replayed. */
 int tq_batch_run(int32_t dtype, const void* desc, int64_t count, const tq_batch_tile* tiles, int64_t n_tiles,
                  const int64_t* tile_begin, const int64_t* tile_count, const double* gl_nodes,
                  const double* gl_weights, const float* gl_pairs32, int64_t gl_len, double* tile_partials,
-                 unsigned int* counter, int64_t counter_words, void* out, double* out64, int32_t blocks,
-                 const int64_t* class_offsets, void* stream);
+                 void* out, double* out64, const int64_t* class_offsets, void* stream);
 
 /* ---- differentiation ----------------------------------------------------
  * Parameter sweep of the Gaussian peak f = exp(-sum a_k^2 (x_k-u_k)^2) over B

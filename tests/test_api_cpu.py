@@ -349,8 +349,8 @@ def test_batch_run_rejects_bad_gl_tables_on_the_host():
     offs = np.zeros(4, dtype=np.int64)
 
     def run(gx, gw, gl_len):
-        return L.tq_batch_run(0, fake, 1, fake, 1, fake, fake, gx, gw, None, gl_len, fake, fake,
-                              _lib.BATCH_COUNTER_WORDS, fake, fake, 0, offs.ctypes.data_as(ctypes.c_void_p), None)
+        return L.tq_batch_run(0, fake, 1, fake, 1, fake, fake, gx, gw, None, gl_len, fake, fake, fake,
+                              offs.ctypes.data_as(ctypes.c_void_p), None)
 
     assert run(fake, fake, -1) == -1
     assert b"gl_len" in L.tq_last_error()

@@ -88,7 +88,7 @@ def test_batch_matches_plain_path_and_is_deterministic(dev):
         scale = scale_of(it, n, oracle_value(it, n))
         assert abs(plain - float(out[i])) <= 2e-6 * scale, (i, plain, float(out[i]))
     # a different tiling gives the same values up to FP32 partial rounding
-    b.retile(tile_points=2048, blocks=1024)
+    b.retile(tile_points=2048)
     b.run()
     np.testing.assert_allclose(b.out64.cpu().numpy(), first.cpu().numpy(), rtol=2e-7, atol=1e-9)
 
@@ -158,11 +158,11 @@ def test_autotune_picks_a_candidate(dev):
     b = Batch(genz_batch(200, seed=5), dtype=torch.float32)
     _, ref = b.run()
     ref = ref.clone()
-    (tp, bl), table = autotune.autotune(b, tiles=(2048, 8192), blocks=(1024, 2048), reps=2, use_cache=False)
-    assert (tp, bl) in {(t, q) for t, q, _ in table}
+    tp, table = autotune.autotune(b, tiles=(2048, 8192), reps=2, use_cache=False)
+    assert tp in {t for t, _ in table}
     b.run()
     # the tuned geometry gives exactly what a fresh batch with that geometry gives
-    same = Batch(genz_batch(200, seed=5), dtype=torch.float32, tile_points=tp, blocks=bl)
+    same = Batch(genz_batch(200, seed=5), dtype=torch.float32, tile_points=tp)
     assert torch.equal(b.out64, same.run()[1])
     # another tile size only regroups FP32 partial sums: FP32 bar
     np.testing.assert_allclose(b.out64.cpu().numpy(), ref.cpu().numpy(), rtol=1e-6, atol=1e-9)

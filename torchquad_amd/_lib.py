@@ -16,7 +16,6 @@ _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libtorchquad_amd.so"
 
 TQ_MAX_DIM = 16
-BATCH_COUNTER_WORDS = 3 * 16 * 1024  # TQ_BATCH_COUNTER_WORDS
 TQ_F32, TQ_F64, TQ_I32, TQ_I64 = 0, 1, 2, 3
 RULE_TRAPEZOID, RULE_SIMPSON, RULE_BOOLE, RULE_GAUSS_LEGENDRE = 0, 1, 2, 3
 
@@ -149,8 +148,7 @@ SIGNATURES = {
     ),
     "tq_batch_plan_sizes": (ctypes.c_int, [_P, _I64, _I32, _I64, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "tq_batch_plan": (ctypes.c_int, [_P, _I64, _I32, _I64, _P, _P, _P, _P, _P]),
-    "tq_batch_run": (ctypes.c_int, [_I32, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _I64, _P, _P, _I64, _P, _P, _I32,
-                                    _P, _P]),
+    "tq_batch_run": (ctypes.c_int, [_I32, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P]),
     "tq_mc_workspace_size": (_SZ, [ctypes.POINTER(McDesc)]),
     "tq_mc_integrate": (
         ctypes.c_int,

@@ -1,9 +1,8 @@
-"""Tile-size / grid-size autotuner for the batched kernel.
+"""Tile-size autotuner for the batched kernel.
 
-For a :class:`~torchquad_amd.batch.Batch` it times every (tile_points,
-blocks) candidate on the device with HIP events, keeps the fastest, and
-caches the choice per (dtype, integrand-family mix, dimension range, size
-bucket).  The cache lives in memory and, when ``TQ_AUTOTUNE_CACHE`` names a
+For a :class:`~torchquad_amd.batch.Batch` it times every tile_points
+candidate on the device with HIP events, keeps the fastest, and caches the
+choice per (dtype, integrand-family mix, dimension range, size bucket).  The cache lives in memory and, when ``TQ_AUTOTUNE_CACHE`` names a
 file, in that JSON file (so a serving process can reuse earlier searches).
 """
 import json
@@ -14,7 +13,6 @@ import threading
 import torch
 
 TILE_CANDIDATES = (16384, 32768, 65536, 131072)
-BLOCK_CANDIDATES = (1024, 2048, 4096)
 
 _cache = {}
 _lock = threading.Lock()
@@ -70,31 +68,29 @@ def time_batch(batch, reps=3):
     return times[len(times) // 2]
 
 
-def autotune(batch, tiles=TILE_CANDIDATES, blocks=BLOCK_CANDIDATES, reps=3, use_cache=True):
-    """Pick (tile_points, blocks) for `batch`, apply it and return it with the timing table."""
+def autotune(batch, tiles=TILE_CANDIDATES, reps=3, use_cache=True):
+    """Pick tile_points for `batch`, apply it and return (tile_points, timing table)."""
     key = signature(batch)
     with _lock:
         if not _cache:
             _load()
         hit = _cache.get(key) if use_cache else None
     if hit is not None:
-        batch.retile(tile_points=hit["tile_points"], blocks=hit["blocks"])
-        return (hit["tile_points"], hit["blocks"]), hit.get("table", [])
+        batch.retile(tile_points=hit["tile_points"])
+        return hit["tile_points"], hit.get("table", [])
     table = []
     best = None
     for tp in tiles:
         batch.retile(tile_points=tp)
-        for bl in blocks:
-            batch.retile(blocks=bl)
-            ms = time_batch(batch, reps)
-            table.append((tp, bl, ms))
-            if best is None or ms < best[2]:
-                best = (tp, bl, ms)
-    batch.retile(tile_points=best[0], blocks=best[1])
+        ms = time_batch(batch, reps)
+        table.append((tp, ms))
+        if best is None or ms < best[1]:
+            best = (tp, ms)
+    batch.retile(tile_points=best[0])
     with _lock:
-        _cache[key] = {"tile_points": best[0], "blocks": best[1], "ms": best[2], "table": table}
+        _cache[key] = {"tile_points": best[0], "ms": best[1], "table": table}
         _store()
-    return (best[0], best[1]), table
+    return best[0], table
 
 
 # ---- plain fused grid kernel: items per wave -------------------------------

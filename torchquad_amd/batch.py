@@ -1,12 +1,12 @@
-"""Batched form: many parameterised integrals in one persistent kernel.
+"""Batched form: many parameterised integrals in one launch per LDS class.
 
 A :class:`Batch` takes a list of integrals -- rule, built-in integrand,
 dimension, total point count N (rounded per axis exactly as the reference's
 integrators round it) and domain -- plans them once on the host (C ABI
 ``tq_batch_plan``), keeps the plan on the device and runs all of them with
-one persistent kernel plus one per-integral reduction (``tq_batch_run``).
-The pair of launches has no host synchronisation and no memset, so
-:meth:`Batch.graph` captures it in a HIP graph for replay.
+one wave per tile (one kernel per LDS class) plus one per-integral
+reduction (``tq_batch_run``).  The launches have no host synchronisation and
+no memset, so :meth:`Batch.graph` captures them in a HIP graph for replay.
 
 The reference has no batched integrator; this mirrors its vectorised /
 ``Batch1DIntegrator`` use (base_integrator.py:77-89, tutorial.rst:800-888)
@@ -32,7 +32,6 @@ RULE_NAMES = {
 }
 
 DEFAULT_TILE_POINTS = 8192
-DEFAULT_BLOCKS = 2048
 
 
 def n_per_axis(rule, dim, N):
@@ -79,13 +78,12 @@ class Batch:
             integrators' argument) or ``n`` (points per axis).
         dtype: torch.float32 or torch.float64 (points and integrand).
         tile_points: points per tile (None: autotuned / default).
-        blocks: persistent workgroups (None: autotuned / default).
         gl_tables: callable (n, device) -> (nodes, weights), FP64 device
             tensors on [-1, 1], for the Gaussian items (default: the
             Gauss-Legendre generator); a custom Gaussian rule's tables.
     """
 
-    def __init__(self, items, dtype=torch.float32, device=None, tile_points=None, blocks=None, gl_tables=None):
+    def __init__(self, items, dtype=torch.float32, device=None, tile_points=None, gl_tables=None):
         if not torch.cuda.is_available():
             raise _lib.NativeUnavailableError("the batched form runs on a ROCm GPU only")
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
@@ -148,7 +146,6 @@ class Batch:
                 arr[i].gl_offset = offsets[n]
         self._items = arr
         self.tile_points = int(tile_points or DEFAULT_TILE_POINTS)
-        self.blocks = int(blocks or DEFAULT_BLOCKS)
         self._plan(self.tile_points)
         self.out = torch.empty(self.count, dtype=dtype, device=self.device)
         self.out64 = torch.empty(self.count, dtype=torch.float64, device=self.device)
@@ -174,22 +171,18 @@ class Batch:
         self.tile_begin = torch.from_numpy(tb).to(dev)
         self.tile_count = torch.from_numpy(tc).to(dev)
         self.tile_part = torch.empty(max(self.n_tiles, 1), dtype=torch.float64, device=dev)
-        self.counter = torch.zeros(_lib.BATCH_COUNTER_WORDS, dtype=torch.int32, device=dev)
         self.tile_points = int(tile_points)
 
-    def retile(self, tile_points=None, blocks=None):
-        """Re-plan with another tile size and/or persistent grid size."""
+    def retile(self, tile_points=None):
+        """Re-plan with another tile size."""
         if tile_points is not None and int(tile_points) != self.tile_points:
             self._plan(int(tile_points))
-        if blocks is not None:
-            self.blocks = int(blocks)
 
     def run(self):
         """Launch the batch on torch's current stream; returns (out, out64)."""
         call("tq_batch_run", self.dt, ptr(self.desc), self.count, ptr(self.tiles), self.n_tiles,
              ptr(self.tile_begin), ptr(self.tile_count), ptr(self.gl_x), ptr(self.gl_w), ptr(self.gl_p32),
-             self.gl_x.numel() if self.gl_x is not None else 0, ptr(self.tile_part),
-             ptr(self.counter), self.counter.numel(), ptr(self.out), ptr(self.out64), self.blocks,
+             self.gl_x.numel() if self.gl_x is not None else 0, ptr(self.tile_part), ptr(self.out), ptr(self.out64),
              self.class_offsets.ctypes.data_as(ctypes.c_void_p), stream_ptr(self.device))
         return self.out, self.out64
 
@@ -252,8 +245,8 @@ def _split(fn):
     return [_OneSet(d) for d in fn.descriptors(fn.dim)]
 
 
-def integrate_batch(items, dtype=torch.float32, device=None, tile_points=None, blocks=None):
+def integrate_batch(items, dtype=torch.float32, device=None, tile_points=None):
     """Plan and run a batch once; returns the float64 results tensor."""
-    b = Batch(items, dtype=dtype, device=device, tile_points=tile_points, blocks=blocks)
+    b = Batch(items, dtype=dtype, device=device, tile_points=tile_points)
     _, out64 = b.run()
     return out64

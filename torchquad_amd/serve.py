@@ -11,8 +11,8 @@ any thread).  Each scheduling round (:meth:`IntegrationServer.step`):
    reservation fits the HBM budget; a request larger than the whole budget
    is rejected with :class:`AdmissionError`;
 3. packs every admitted grid request into one batch -- host plan
-   (``tq_batch_plan``), asynchronous upload from pinned buffers, one
-   persistent kernel plus the per-integral reduction (``tq_batch_run``),
+   (``tq_batch_plan``), asynchronous upload from pinned buffers, the
+   tile kernels plus the per-integral reduction (``tq_batch_run``),
    asynchronous download of the results -- on one of ``max_inflight``
    streams, so a batch is planned and launched while earlier ones still
    run.  Gauss-Legendre tables live at fixed offsets of one server-wide
@@ -310,7 +310,6 @@ class _BatchSlot:
         self.stream = torch.cuda.Stream(device=device)
         self.stage = _Pinned(device)  # plan outputs: descriptors | tiles | tile_begin | tile_count
         self.part = torch.empty(0, dtype=torch.float64, device=device)
-        self.counter = torch.zeros(_lib.BATCH_COUNTER_WORDS, dtype=torch.int32, device=device)
         self.class_offsets = np.zeros(4, dtype=np.int64)
         self.out = torch.empty(0, dtype=dtype, device=device)
         self.out64 = torch.empty(0, dtype=torch.float64, device=device)
@@ -660,8 +659,8 @@ class IntegrationServer:
             slot.ensure_outputs(count, nt)
             call("tq_batch_run", self.dt, ctypes.c_void_p(dp), count, ctypes.c_void_p(dp + o_tiles),
                  int(n_tiles.value), ctypes.c_void_p(dp + o_begin), ctypes.c_void_p(dp + o_count), ptr(gl_x),
-                 ptr(gl_w), ptr(gl_p), gl_len, ptr(slot.part), ptr(slot.counter), slot.counter.numel(), ptr(slot.out),
-                 ptr(slot.out64), 0, slot.class_offsets.ctypes.data_as(ctypes.c_void_p), slot.stream.cuda_stream)
+                 ptr(gl_w), ptr(gl_p), gl_len, ptr(slot.part), ptr(slot.out), ptr(slot.out64),
+                 slot.class_offsets.ctypes.data_as(ctypes.c_void_p), slot.stream.cuda_stream)
             slot.host_out[:count].copy_(slot.out64[:count], non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(slot.stream)
