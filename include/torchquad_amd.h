@@ -235,17 +235,18 @@ int tq_batch_plan_sizes(const tq_batch_item* items, int64_t count, int32_t dtype
 /* Host-side plan: descriptors (desc_out, host memory of desc_bytes), tiles
  * ordered by kernel class and, within a class, longest integral first, each
  * integral's tiles contiguous; tile_begin[i] / tile_count[i] for integral i.
- * class_offsets (optional, [4]): tiles [class_offsets[c], class_offsets[c+1])
- * belong to class c -- 0: loop and row-axis tables within 4 KB of LDS per
- * wave (or no loop axis), 1: within 8 KB, 2: larger (general kernel). */
+ * class_offsets (optional, [5]): tiles [class_offsets[c], class_offsets[c+1])
+ * belong to class c -- 0: one-axis integrals; multi-axis ones with 1: loop
+ * and row-axis tables within 4 KB of LDS per wave (or no loop axis), 2:
+ * within 8 KB, 3: larger (general kernel). */
 int tq_batch_plan(const tq_batch_item* items, int64_t count, int32_t dtype, int64_t tile_points, void* desc_out,
                   tq_batch_tile* tiles_out, int64_t* tile_begin, int64_t* tile_count, int64_t* class_offsets);
 /* Runs a plan whose arrays were copied to the device: one kernel per
  * non-empty class, one wave per tile in the plan's order (longest integrals
  * first, so the hardware dispatcher hands out the large tiles first), then
  * out[i] (dtype) / out64[i] = integral i.  tile_partials: [n_tiles];
- * class_offsets: host [4] from tq_batch_plan, or NULL to run every tile on
- * the general kernel.  gl_nodes / gl_weights: every Gauss-Legendre item's
+ * class_offsets: host [5] from tq_batch_plan, or NULL to run every tile on
+ * a general kernel that carries both the one-axis and multi-axis code.  gl_nodes / gl_weights: every Gauss-Legendre item's
  * table at its gl_offset, gl_len entries each; gl_pairs32 (optional, F32
  * batches only, else ignored): the same tables as interleaved FP32 (node,
  * weight) pairs, pair k at [2 (gl_offset + k)], read by one-axis items

@@ -226,15 +226,16 @@ def test_batch_plan_host_side():
     tiles = (_lib.BatchTile * nt.value)()
     tb = np.zeros(3, dtype=np.int64)
     tc = np.zeros(3, dtype=np.int64)
-    co = np.full(4, -1, dtype=np.int64)
+    co = np.full(6, -1, dtype=np.int64)  # one spare entry past the 5 the plan writes
     assert L.tq_batch_plan(ctypes.cast(items, ctypes.c_void_p), 3, _lib.TQ_F32, 4096,
                            desc.ctypes.data_as(ctypes.c_void_p), tiles, tb.ctypes.data_as(ctypes.c_void_p),
                            tc.ctypes.data_as(ctypes.c_void_p), co.ctypes.data_as(ctypes.c_void_p)) == 0
     assert tc.sum() == nt.value
-    # all three fit the small-table class: LPT order, the 1e5-point GL integral
-    # first, then 21^3 = 9261, then 33^2 = 1089
+    # class 0 holds the one-axis GL integral, class 1 (tables within 4 KB) the
+    # other two in LPT order: 21^3 = 9261 points, then 33^2 = 1089
     assert list(np.argsort(tb)) == [2, 1, 0]
-    assert list(co) == [0, nt.value, nt.value, nt.value]
+    assert list(co[:5]) == [0, tc[2], nt.value, nt.value, nt.value]
+    assert co[5] == -1
     for i in range(3):
         ts = [tiles[t] for t in range(tb[i], tb[i] + tc[i])]
         assert all(t.item == i for t in ts)
@@ -346,7 +347,7 @@ def test_batch_run_rejects_bad_gl_tables_on_the_host():
     node/weight tables TQ_E_NULL."""
     L = _lib.load_library()
     fake = ctypes.c_void_p(256)
-    offs = np.zeros(4, dtype=np.int64)
+    offs = np.zeros(5, dtype=np.int64)
 
     def run(gx, gw, gl_len):
         return L.tq_batch_run(0, fake, 1, fake, 1, fake, fake, gx, gw, None, gl_len, fake, fake, fake,

@@ -159,7 +159,7 @@ class Batch:
         tiles = np.zeros(max(n_tiles.value, 1) * ctypes.sizeof(_lib.BatchTile), dtype=np.uint8)
         tb = np.zeros(max(self.count, 1), dtype=np.int64)
         tc = np.zeros(max(self.count, 1), dtype=np.int64)
-        self.class_offsets = np.zeros(4, dtype=np.int64)
+        self.class_offsets = np.zeros(5, dtype=np.int64)
         call("tq_batch_plan", ctypes.cast(self._items, ctypes.c_void_p), self.count, self.dt, int(tile_points),
              desc.ctypes.data_as(ctypes.c_void_p), tiles.ctypes.data_as(ctypes.c_void_p),
              tb.ctypes.data_as(ctypes.c_void_p), tc.ctypes.data_as(ctypes.c_void_p),

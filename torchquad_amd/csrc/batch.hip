@@ -36,7 +36,7 @@ struct BatchTab {
 // tables, high occupancy) and class 1 (loop tables up to 8 KB).
 constexpr int kRegionSmall = 4096;
 constexpr int kRegionBig = 8192;
-constexpr int kBatchClasses = 3;  // 0, 1: row-table kernels; 2: general kernel
+constexpr int kBatchClasses = 4;  // 0: one-axis kernel; 1, 2: row-table kernels; 3: general kernel
 
 // LDS bytes of an integral's tables in the row-table kernels:
 // [row weights L*n FP64][loop table S*nl T][row terms slots*L*n T] (S = tab_stride, slots = state_slots)
@@ -48,7 +48,7 @@ inline long long rowtab_bytes(int fam, int n, int nl, int L, int tsize) {
 
 
 // General kernel: loop-axis table when it fits, row axes recomputed per row.
-template <typename T, int FAM>
+template <typename T, int FAM, int PATHS>
 __device__ __forceinline__ double batch_tile_general(const BatchDev<T>& D, const Axis* ax, T* tab, bool table,
                                             const double* glx, const double* glw, const float* gp32, int lw0,
                                             int lw1, long long t0, long long t1) {
@@ -66,15 +66,18 @@ __device__ __forceinline__ double batch_tile_general(const BatchDev<T>& D, const
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     double acc = 0.0;
     for (int lw = lw0; lw < lw1; ++lw)
-        acc += table ? grid_tile<T, FAM, true>(D.d, D.n, D.nl, D.m, D.L, D.rule, D.lane_count, D.p, ax, tab, glx,
-                                               glw, lw, t0, t1, nullptr, nullptr, gp32)
-                     : grid_tile<T, FAM, false>(D.d, D.n, D.nl, D.m, D.L, D.rule, D.lane_count, D.p, ax, tab, glx,
-                                                glw, lw, t0, t1, nullptr, nullptr, gp32);
+        acc += table ? grid_tile<T, FAM, true, false, true, PATHS>(D.d, D.n, D.nl, D.m, D.L, D.rule, D.lane_count,
+                                                                  D.p, ax, tab, glx, glw, lw, t0, t1, nullptr,
+                                                                  nullptr, gp32)
+                     : grid_tile<T, FAM, false, false, true, PATHS>(D.d, D.n, D.nl, D.m, D.L, D.rule, D.lane_count,
+                                                                   D.p, ax, tab, glx, glw, lw, t0, t1, nullptr,
+                                                                   nullptr, gp32);
     return acc;
 }
 
 // Row-table kernels: the planner guarantees that the loop and row tables of
-// every integral they see fit the region (or that it has no loop axis).
+// every integral they see fit the region (or that it has no loop axis), and
+// sends them multi-axis integrals only.
 template <typename T, int FAM>
 __device__ __forceinline__ double batch_tile_rowtab(const BatchDev<T>& D, const Axis* ax, unsigned char* region,
                                            const double* glx, const double* glw, const float* gp32, int lw0,
@@ -102,12 +105,12 @@ __device__ __forceinline__ double batch_tile_rowtab(const BatchDev<T>& D, const 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     double acc = 0.0;
     for (int lw = lw0; lw < lw1; ++lw)
-        acc += grid_tile<T, FAM, true, true>(D.d, n, D.nl, D.m, L, D.rule, D.lane_count, D.p, ax, tab, glx, glw, lw,
-                                             t0, t1, rt, rw, gp32);
+        acc += grid_tile<T, FAM, true, true, true, kPathsMultiAxis>(D.d, n, D.nl, D.m, L, D.rule, D.lane_count, D.p,
+                                                                     ax, tab, glx, glw, lw, t0, t1, rt, rw, gp32);
     return acc;
 }
 
-template <typename T, int REGION>
+template <typename T, int REGION, int PATHS>
 __device__ __forceinline__ double batch_tile(const BatchDev<T>& D, const Axis* ax, unsigned char* region, bool table,
                                     const double* glx, const double* glw, const float* gp32, int lw0, int lw1,
                                     long long t0, long long t1) {
@@ -115,8 +118,8 @@ __device__ __forceinline__ double batch_tile(const BatchDev<T>& D, const Axis* a
 #define TQ_BT(F)                                                                                                    \
     case F:                                                                                                         \
         return REGION > 0 ? batch_tile_rowtab<T, F>(D, ax, region, glx, glw, gp32, lw0, lw1, t0, t1)                \
-                          : batch_tile_general<T, F>(D, ax, reinterpret_cast<T*>(region), table, glx, glw, gp32, lw0, \
-                                                     lw1, t0, t1);
+                          : batch_tile_general<T, F, PATHS>(D, ax, reinterpret_cast<T*>(region), table, glx, glw,  \
+                                                            gp32, lw0, lw1, t0, t1);
         TQ_BT(TQ_FAM_GAUSSIAN_PEAK)
         TQ_BT(TQ_FAM_OSCILLATORY)
         TQ_BT(TQ_FAM_PRODUCT_PEAK)
@@ -132,17 +135,21 @@ __device__ __forceinline__ double batch_tile(const BatchDev<T>& D, const Axis* a
 // One wave per tile of one class, tiles [0, n_tiles) in the planner's order
 // (longest integrals first), so the hardware dispatcher hands them out
 // largest first as waves finish.  REGION > 0: row-table kernel with a
-// per-wave LDS region of REGION bytes; REGION == 0: general kernel.  A
+// per-wave LDS region of REGION bytes; REGION == 0: general kernel.  PATHS
+// (grid_tile): the one-axis class runs a kernel with only the one-axis code,
+// the others one with only the multi-axis code (83 VGPRs for the 4 KB
+// row-table kernel instead of 112 with both).  A
 // persistent form (each wave claiming tiles from 16 atomic queues) held
 // its claim loop's state and the hoisted per-tile invariants for the whole
 // kernel: 164 VGPRs against 112, and the 8192-integral Genz batch took
 // 0.441 ms against 0.344.
-template <typename T, int REGION>
+template <typename T, int REGION, int PATHS>
 __global__ __launch_bounds__(kBlock) void k_batch(const BatchDev<T>* descs, const tq_batch_tile* tiles,
                                                   long long n_tiles, const double* glx, const double* glw,
                                                   const float* glp32, long long gl_len, double* tile_part) {
     constexpr int NT = BatchTab<T>::N;
-    constexpr int RB = REGION > 0 ? REGION : (int)(2 * NT * sizeof(T));
+    // one-axis integrals use no LDS table
+    constexpr int RB = REGION > 0 ? REGION : (PATHS == kPathsOneAxis ? 8 : (int)(2 * NT * sizeof(T)));
     __shared__ Axis s_ax[kWavesPerBlock][TQ_MAX_DIM];
     __shared__ double s_region[kWavesPerBlock][RB / 8];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -168,9 +175,10 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchDev<T>* descs, cons
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        const bool table = REGION == 0 && D.m > 0 && D.L >= 0 && D.nl * tab_stride(D.fam) <= 2 * NT;
-        acc = batch_tile<T, REGION>(D, s_ax[wave], region, table, gx, gw, gp, tile.lane_wave,
-                                    tile.lane_wave_end, tile.t0, tile.t1);
+        const bool table = REGION == 0 && PATHS != kPathsOneAxis && D.m > 0 && D.L >= 0 &&
+                           D.nl * tab_stride(D.fam) <= 2 * NT;
+        acc = batch_tile<T, REGION, PATHS>(D, s_ax[wave], region, table, gx, gw, gp, tile.lane_wave,
+                                           tile.lane_wave_end, tile.t0, tile.t1);
         acc = wave_sum(acc);
     }
     if (lane == 0) tile_part[t] = acc;
@@ -281,10 +289,12 @@ static int plan_batch(const tq_batch_item* items, long long count, long long til
         group[i] = g;
         long long P;
         ipow_checked(it.n, it.dim, 1LL << 62, &P);
-        int cls = 0;
+        // class 0: one-axis integrals; 1, 2: multi-axis ones whose tables fit
+        // 4 KB / 8 KB (or that have no loop axis); 3: the rest
+        int cls = geo[i].m == 0 ? 0 : 1;
         if (geo[i].m > 0 && geo[i].L >= 0) {
             const long long b = rowtab_bytes(it.family, it.n, geo[i].nl, geo[i].L, (int)sizeof(T));
-            cls = b <= kRegionSmall ? 0 : (b <= kRegionBig ? 1 : 2);
+            cls = b <= kRegionSmall ? 1 : (b <= kRegionBig ? 2 : 3);
         }
         order[i] = {cls, P, i};
         total += chunks * ((geo[i].lane_waves + g - 1) / g);
@@ -362,13 +372,13 @@ static int plan_batch(const tq_batch_item* items, long long count, long long til
     return TQ_OK;
 }
 
-template <typename T, int REGION>
+template <typename T, int REGION, int PATHS>
 static int launch_class(const void* desc, const tq_batch_tile* tiles, long long n, const double* glx,
                         const double* glw, const float* glp32, long long gl_len, double* tile_part, hipStream_t s) {
     if (n <= 0) return TQ_OK;
     const long long blocks = (n + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL((k_batch<T, REGION>), dim3((unsigned)blocks), dim3(kBlock), 0, s, (const BatchDev<T>*)desc,
-                       tiles, n, glx, glw, glp32, gl_len, tile_part);
+    hipLaunchKernelGGL((k_batch<T, REGION, PATHS>), dim3((unsigned)blocks), dim3(kBlock), 0, s,
+                       (const BatchDev<T>*)desc, tiles, n, glx, glw, glp32, gl_len, tile_part);
     return check_launch("k_batch");
 }
 
@@ -377,21 +387,27 @@ static int run_batch(const void* desc, long long count, const tq_batch_tile* til
                      const int64_t* tile_begin, const int64_t* tile_count, const double* glx, const double* glw,
                      const float* glp32, long long gl_len, double* tile_part, void* out, double* out64,
                      const int64_t* class_offsets, hipStream_t s) {
-    long long off[kBatchClasses + 1] = {0, 0, 0, n_tiles};  // no classes: every tile on the general kernel
-    if (class_offsets) {
-        for (int c = 0; c <= kBatchClasses; ++c) off[c] = class_offsets[c];
+    int rc = TQ_OK;
+    if (!class_offsets) {
+        // no classes: every tile on the general kernel with both paths
+        rc = launch_class<T, 0, kPathsAll>(desc, tiles, n_tiles, glx, glw, glp32, gl_len, tile_part, s);
+    } else {
+        const int64_t* off = class_offsets;
         for (int c = 0; c < kBatchClasses; ++c)
-            if (off[c] < 0 || off[c] > off[c + 1] || off[kBatchClasses] != n_tiles)
+            if (off[0] != 0 || off[c] < 0 || off[c] > off[c + 1] || off[kBatchClasses] != n_tiles)
                 return fail(TQ_E_ARG, "class_offsets do not partition the tiles");
+        rc = launch_class<T, 0, kPathsOneAxis>(desc, tiles + off[0], off[1] - off[0], glx, glw, glp32, gl_len,
+                                               tile_part + off[0], s);
+        if (!rc)
+            rc = launch_class<T, kRegionSmall, kPathsMultiAxis>(desc, tiles + off[1], off[2] - off[1], glx, glw,
+                                                                glp32, gl_len, tile_part + off[1], s);
+        if (!rc)
+            rc = launch_class<T, kRegionBig, kPathsMultiAxis>(desc, tiles + off[2], off[3] - off[2], glx, glw,
+                                                              glp32, gl_len, tile_part + off[2], s);
+        if (!rc)
+            rc = launch_class<T, 0, kPathsMultiAxis>(desc, tiles + off[3], off[4] - off[3], glx, glw, glp32, gl_len,
+                                                     tile_part + off[3], s);
     }
-    int rc = launch_class<T, kRegionSmall>(desc, tiles + off[0], off[1] - off[0], glx, glw, glp32, gl_len,
-                                           tile_part + off[0], s);
-    if (!rc)
-        rc = launch_class<T, kRegionBig>(desc, tiles + off[1], off[2] - off[1], glx, glw, glp32, gl_len,
-                                         tile_part + off[1], s);
-    if (!rc)
-        rc = launch_class<T, 0>(desc, tiles + off[2], off[3] - off[2], glx, glw, glp32, gl_len, tile_part + off[2],
-                                s);
     if (rc) return rc;
     const long long fb = (count + kWavesPerBlock - 1) / kWavesPerBlock;
     hipLaunchKernelGGL((k_batch_finalize<T>), dim3((unsigned)(fb > 0 ? fb : 1)), dim3(kBlock), 0, s,

@@ -226,7 +226,15 @@ __device__ inline void row_table_entry(T* rt, double* rw, int k, int j, const Ax
 // the axes between the row and lane axes: one axis, nl = n, or two, nl = n^2,
 // which needs the table).  Returns the lane's weighted sum.  `ax` holds the
 // per-axis maps, `tab` the loop table (loop_table_entry) when TABLE.
-template <typename T, int FAM, bool TABLE, bool ROWTAB = false, bool GL_FOLD = true>
+// Code paths a grid_tile instantiation carries: one-axis integrals (m == 0),
+// multi-axis ones (m > 0), or both.  The batched kernels run the two kinds in
+// separate launches, each without the other's code and live registers; an
+// integral sent to an instantiation without its path yields NaN.
+constexpr int kPathsOneAxis = 1;
+constexpr int kPathsMultiAxis = 2;
+constexpr int kPathsAll = 3;
+
+template <typename T, int FAM, bool TABLE, bool ROWTAB = false, bool GL_FOLD = true, int PATHS = kPathsAll>
 __device__ inline double grid_tile(int d, int n, int nl, int m, int L, int rule, int lane_count, const Params<T>& p,
                                    const Axis* ax, const T* tab, const double* glx, const double* glw, int lw,
                                    long long t_begin, long long t_end, const T* rt = nullptr,
@@ -234,105 +242,110 @@ __device__ inline double grid_tile(int d, int n, int nl, int m, int L, int rule,
     using F = Family<FAM, T>;
     const int lane = threadIdx.x & (kWave - 1);
     double acc = 0.0;
-    if (m == 0 && GL_FOLD && rule == TQ_RULE_GAUSS_LEGENDRE) {
-        const State<T> s0 = F::start(p);
-        const AxisC<T> ac0 = axis_c(p, 0);
-        const Axis a0 = ax[0];
-        const int h = (n + 1) >> 1;  // nodes 0 .. h-1; for odd n node h-1 is the middle one, counted once
-        for (long long tb = t_begin; tb < t_end; tb += kFlush) {
-            const long long te = (tb + kFlush < t_end) ? (tb + kFlush) : t_end;
-            T fa = T(0);
-            if (gl_pairs32) {
-                // FP32 (node, weight) pairs: one 8-byte load per two points
-                // (FP32 results only; the node and weight carry 2^-24
-                // relative rounding, far inside the FP32 tolerance)
-                const float2* pairs = reinterpret_cast<const float2*>(gl_pairs32);
-                const float sa = (float)a0.a, sh = (float)a0.h;
-#pragma unroll 2
-                for (long long t = tb; t < te; ++t) {
-                    const int k = (int)(t * kWave) + lane;
-                    const float2 e = pairs[k < h ? k : h - 1];
-                    const T w1 = (k < h) ? (T)e.y : T(0);
-                    const T w2 = (k < h && 2 * k + 1 != n) ? (T)e.y : T(0);
-                    const T x1 = (T)fmaf(sa, e.x, sh);
-                    const T x2 = (T)fmaf(-sa, e.x, sh);
-                    fa = fma(w1, F::finish(F::fold_c(s0, ac0, x1, p), p), fa);
-                    fa = fma(w2, F::finish(F::fold_c(s0, ac0, x2, p), p), fa);
-                }
-            } else {
-#pragma unroll 2
-                for (long long t = tb; t < te; ++t) {
-                    const int k = (int)(t * kWave) + lane;
-                    const int kk = k < h ? k : h - 1;
-                    const double xn = glx[kk], wk = glw[kk];
-                    const T w1 = (k < h) ? (T)wk : T(0);
-                    const T w2 = (k < h && 2 * k + 1 != n) ? (T)wk : T(0);
-                    const T x1 = (T)fma(a0.a, xn, a0.h);
-                    const T x2 = (T)fma(-a0.a, xn, a0.h);
-                    fa = fma(w1, F::finish(F::fold_c(s0, ac0, x1, p), p), fa);
-                    fa = fma(w2, F::finish(F::fold_c(s0, ac0, x2, p), p), fa);
-                }
-            }
-            acc += (double)fa;
-        }
-        return acc;
-    }
-    if (m == 0) {
-        // one axis split into rows of 64 consecutive nodes: node k = 64 t + lane
-        const State<T> s0 = F::start(p);
-        const AxisC<T> ac0 = axis_c(p, 0);
-        const Axis a0 = ax[0];
-        // Items holding an end node (t = 0 and the item of node n - 1, which
-        // may also run past n) take the general body: clamped index, zero
-        // weight past n, end weights.
-        const long long t_end_item = (long long)(n - 1) / kWave;
-        auto edge_item = [&](long long t) -> double {
-            const long long k = t * kWave + lane;
-            const int kk = (int)(k < n ? k : n - 1);
-            const double w = (k < n) ? weight_of(rule, kk, n, glw) : 0.0;
-            const T f = F::finish(F::fold_c(s0, ac0, (T)node_of(a0, kk, n, rule, glx), p), p);
-            return w * (double)f;
-        };
-        long long ib = t_begin, ie = t_end;  // interior items [ib, ie)
-        if (ib == 0 && ie > 0) {
-            acc += edge_item(0);
-            ib = 1;
-        }
-        if (ie > t_end_item && t_end_item >= ib) {
-            acc += edge_item(t_end_item);
-            ie = t_end_item;
-        }
-        if (rule == TQ_RULE_GAUSS_LEGENDRE) {
-            for (long long tb = ib; tb < ie; tb += kFlush) {
-                const long long te = (tb + kFlush < ie) ? (tb + kFlush) : ie;
+    if constexpr ((PATHS & kPathsOneAxis) != 0) {
+        if (m == 0 && GL_FOLD && rule == TQ_RULE_GAUSS_LEGENDRE) {
+            const State<T> s0 = F::start(p);
+            const AxisC<T> ac0 = axis_c(p, 0);
+            const Axis a0 = ax[0];
+            const int h = (n + 1) >> 1;  // nodes 0 .. h-1; for odd n node h-1 is the middle one, counted once
+            for (long long tb = t_begin; tb < t_end; tb += kFlush) {
+                const long long te = (tb + kFlush < t_end) ? (tb + kFlush) : t_end;
                 T fa = T(0);
-#pragma unroll 4
-                for (long long t = tb; t < te; ++t) {
-                    const int k = (int)(t * kWave) + lane;
-                    const T x = (T)fma(a0.a, glx[k], a0.h);
-                    fa = fma((T)glw[k], F::finish(F::fold_c(s0, ac0, x, p), p), fa);
+                if (gl_pairs32) {
+                    // FP32 (node, weight) pairs: one 8-byte load per two points
+                    // (FP32 results only; the node and weight carry 2^-24
+                    // relative rounding, far inside the FP32 tolerance)
+                    const float2* pairs = reinterpret_cast<const float2*>(gl_pairs32);
+                    const float sa = (float)a0.a, sh = (float)a0.h;
+    #pragma unroll 2
+                    for (long long t = tb; t < te; ++t) {
+                        const int k = (int)(t * kWave) + lane;
+                        const float2 e = pairs[k < h ? k : h - 1];
+                        const T w1 = (k < h) ? (T)e.y : T(0);
+                        const T w2 = (k < h && 2 * k + 1 != n) ? (T)e.y : T(0);
+                        const T x1 = (T)fmaf(sa, e.x, sh);
+                        const T x2 = (T)fmaf(-sa, e.x, sh);
+                        fa = fma(w1, F::finish(F::fold_c(s0, ac0, x1, p), p), fa);
+                        fa = fma(w2, F::finish(F::fold_c(s0, ac0, x2, p), p), fa);
+                    }
+                } else {
+    #pragma unroll 2
+                    for (long long t = tb; t < te; ++t) {
+                        const int k = (int)(t * kWave) + lane;
+                        const int kk = k < h ? k : h - 1;
+                        const double xn = glx[kk], wk = glw[kk];
+                        const T w1 = (k < h) ? (T)wk : T(0);
+                        const T w2 = (k < h && 2 * k + 1 != n) ? (T)wk : T(0);
+                        const T x1 = (T)fma(a0.a, xn, a0.h);
+                        const T x2 = (T)fma(-a0.a, xn, a0.h);
+                        fa = fma(w1, F::finish(F::fold_c(s0, ac0, x1, p), p), fa);
+                        fa = fma(w2, F::finish(F::fold_c(s0, ac0, x2, p), p), fa);
+                    }
                 }
                 acc += (double)fa;
             }
-        } else {
-            // Newton-Cotes interior weights repeat with period 4 in the node
-            // index, and 64 t + lane = lane (mod 4): one weight per lane.
-            const double w_lane = weight_of(rule, kWave + lane, 0x7fffffff, glw);
-            for (long long tb = ib; tb < ie; tb += kFlush) {
-                const long long te = (tb + kFlush < ie) ? (tb + kFlush) : ie;
-                double kd = (double)(tb * kWave + lane);
-                T fa = T(0);
-#pragma unroll 4
-                for (long long t = tb; t < te; ++t) {
-                    const T x = (T)fma(kd, a0.h, a0.a);
-                    kd += (double)kWave;
-                    fa += F::finish(F::fold_c(s0, ac0, x, p), p);
-                }
-                acc += w_lane * (double)fa;
-            }
+            return acc;
         }
-        return acc;
+        if (m == 0) {
+            // one axis split into rows of 64 consecutive nodes: node k = 64 t + lane
+            const State<T> s0 = F::start(p);
+            const AxisC<T> ac0 = axis_c(p, 0);
+            const Axis a0 = ax[0];
+            // Items holding an end node (t = 0 and the item of node n - 1, which
+            // may also run past n) take the general body: clamped index, zero
+            // weight past n, end weights.
+            const long long t_end_item = (long long)(n - 1) / kWave;
+            auto edge_item = [&](long long t) -> double {
+                const long long k = t * kWave + lane;
+                const int kk = (int)(k < n ? k : n - 1);
+                const double w = (k < n) ? weight_of(rule, kk, n, glw) : 0.0;
+                const T f = F::finish(F::fold_c(s0, ac0, (T)node_of(a0, kk, n, rule, glx), p), p);
+                return w * (double)f;
+            };
+            long long ib = t_begin, ie = t_end;  // interior items [ib, ie)
+            if (ib == 0 && ie > 0) {
+                acc += edge_item(0);
+                ib = 1;
+            }
+            if (ie > t_end_item && t_end_item >= ib) {
+                acc += edge_item(t_end_item);
+                ie = t_end_item;
+            }
+            if (rule == TQ_RULE_GAUSS_LEGENDRE) {
+                for (long long tb = ib; tb < ie; tb += kFlush) {
+                    const long long te = (tb + kFlush < ie) ? (tb + kFlush) : ie;
+                    T fa = T(0);
+    #pragma unroll 4
+                    for (long long t = tb; t < te; ++t) {
+                        const int k = (int)(t * kWave) + lane;
+                        const T x = (T)fma(a0.a, glx[k], a0.h);
+                        fa = fma((T)glw[k], F::finish(F::fold_c(s0, ac0, x, p), p), fa);
+                    }
+                    acc += (double)fa;
+                }
+            } else {
+                // Newton-Cotes interior weights repeat with period 4 in the node
+                // index, and 64 t + lane = lane (mod 4): one weight per lane.
+                const double w_lane = weight_of(rule, kWave + lane, 0x7fffffff, glw);
+                for (long long tb = ib; tb < ie; tb += kFlush) {
+                    const long long te = (tb + kFlush < ie) ? (tb + kFlush) : ie;
+                    double kd = (double)(tb * kWave + lane);
+                    T fa = T(0);
+    #pragma unroll 4
+                    for (long long t = tb; t < te; ++t) {
+                        const T x = (T)fma(kd, a0.h, a0.a);
+                        kd += (double)kWave;
+                        fa += F::finish(F::fold_c(s0, ac0, x, p), p);
+                    }
+                    acc += w_lane * (double)fa;
+                }
+            }
+            return acc;
+        }
+    } else {
+        if (m == 0) return __longlong_as_double(0x7ff8000000000000ll);
     }
+    if constexpr ((PATHS & kPathsMultiAxis) == 0) return __longlong_as_double(0x7ff8000000000000ll);
     const int li = lw * kWave + lane;
     const bool active = li < lane_count;
 

@@ -310,7 +310,7 @@ class _BatchSlot:
         self.stream = torch.cuda.Stream(device=device)
         self.stage = _Pinned(device)  # plan outputs: descriptors | tiles | tile_begin | tile_count
         self.part = torch.empty(0, dtype=torch.float64, device=device)
-        self.class_offsets = np.zeros(4, dtype=np.int64)
+        self.class_offsets = np.zeros(5, dtype=np.int64)
         self.out = torch.empty(0, dtype=dtype, device=device)
         self.out64 = torch.empty(0, dtype=torch.float64, device=device)
         self.host_out = torch.empty(0, dtype=torch.float64, pin_memory=True)
