@@ -244,7 +244,10 @@ int tq_batch_plan(const tq_batch_item* items, int64_t count, int32_t dtype, int6
 /* Runs a plan whose arrays were copied to the device: one kernel per
  * non-empty class, one wave per tile in the plan's order (longest integrals
  * first, so the hardware dispatcher hands out the large tiles first), then
- * out[i] (dtype) / out64[i] = integral i.  tile_partials: [n_tiles];
+ * out[i] (dtype) / out64[i] = integral i.  With more than one non-empty
+ * class the classes run concurrently on two library-owned streams forked
+ * from `stream` and joined back into it with events (under graph capture
+ * the run becomes a forked subgraph).  tile_partials: [n_tiles];
  * class_offsets: host [5] from tq_batch_plan, or NULL to run every tile on
  * a general kernel that carries both the one-axis and multi-axis code.  gl_nodes / gl_weights: every Gauss-Legendre item's
  * table at its gl_offset, gl_len entries each; gl_pairs32 (optional, F32

@@ -12,6 +12,7 @@
 // tiling.  No memset is issued and no counter is kept, so the launches can be
 // captured in a graph and replayed.
 #include <algorithm>
+#include <mutex>
 
 #include "tq_grid_tile.hpp"
 
@@ -382,6 +383,69 @@ static int launch_class(const void* desc, const tq_batch_tile* tiles, long long 
     return check_launch("k_batch");
 }
 
+// Library-owned streams on which classes of a batch run beside the caller's
+// stream: the one-axis class (Gauss-Legendre table reads from HBM) and the
+// small classes overlap the large multi-axis class (VALU-bound) instead of
+// following it.  Created once per device (default priority) and kept; fork
+// and join go through events, so a captured run is one forked subgraph.
+// Runs on one device enqueue their fork, launches and join under that
+// device's lock, since the events are shared.
+constexpr int kMaxBatchDevices = 64;
+struct BatchAux {
+    hipStream_t st[2] = {nullptr, nullptr};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[2] = {nullptr, nullptr};
+    std::mutex mu;
+    bool ready = false;
+};
+static BatchAux g_aux[kMaxBatchDevices];
+static std::mutex g_aux_mu;
+
+static int hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return TQ_OK;
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return (int)e;
+}
+
+static int batch_aux(BatchAux** out) {
+    int dev = -1;
+    int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= kMaxBatchDevices) return fail(TQ_E_UNSUPPORTED, "device index beyond the batch stream table");
+    std::lock_guard<std::mutex> lock(g_aux_mu);
+    BatchAux& a = g_aux[dev];
+    if (!a.ready) {
+        for (int i = 0; i < 2 && !rc; ++i)
+            rc = hip_check(hipStreamCreateWithFlags(&a.st[i], hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.fork, hipEventDisableTiming), "hipEventCreateWithFlags");
+        for (int i = 0; i < 2 && !rc; ++i)
+            rc = hip_check(hipEventCreateWithFlags(&a.join[i], hipEventDisableTiming), "hipEventCreateWithFlags");
+        if (rc) return rc;
+        a.ready = true;
+    }
+    *out = &a;
+    return TQ_OK;
+}
+
+template <typename T>
+static int run_classes(const void* desc, const tq_batch_tile* tiles, const int64_t* off, const double* glx,
+                       const double* glw, const float* glp32, long long gl_len, double* tile_part, hipStream_t s0,
+                       hipStream_t s1, hipStream_t s2) {
+    // s0: the 4 KB multi-axis class; s1: one-axis integrals; s2: the 8 KB and general classes
+    int rc = launch_class<T, kRegionSmall, kPathsMultiAxis>(desc, tiles + off[1], off[2] - off[1], glx, glw, glp32,
+                                                            gl_len, tile_part + off[1], s0);
+    if (!rc)
+        rc = launch_class<T, 0, kPathsOneAxis>(desc, tiles + off[0], off[1] - off[0], glx, glw, glp32, gl_len,
+                                               tile_part + off[0], s1);
+    if (!rc)
+        rc = launch_class<T, kRegionBig, kPathsMultiAxis>(desc, tiles + off[2], off[3] - off[2], glx, glw, glp32,
+                                                          gl_len, tile_part + off[2], s2);
+    if (!rc)
+        rc = launch_class<T, 0, kPathsMultiAxis>(desc, tiles + off[3], off[4] - off[3], glx, glw, glp32, gl_len,
+                                                 tile_part + off[3], s2);
+    return rc;
+}
+
 template <typename T>
 static int run_batch(const void* desc, long long count, const tq_batch_tile* tiles, long long n_tiles,
                      const int64_t* tile_begin, const int64_t* tile_count, const double* glx, const double* glw,
@@ -396,17 +460,23 @@ static int run_batch(const void* desc, long long count, const tq_batch_tile* til
         for (int c = 0; c < kBatchClasses; ++c)
             if (off[0] != 0 || off[c] < 0 || off[c] > off[c + 1] || off[kBatchClasses] != n_tiles)
                 return fail(TQ_E_ARG, "class_offsets do not partition the tiles");
-        rc = launch_class<T, 0, kPathsOneAxis>(desc, tiles + off[0], off[1] - off[0], glx, glw, glp32, gl_len,
-                                               tile_part + off[0], s);
-        if (!rc)
-            rc = launch_class<T, kRegionSmall, kPathsMultiAxis>(desc, tiles + off[1], off[2] - off[1], glx, glw,
-                                                                glp32, gl_len, tile_part + off[1], s);
-        if (!rc)
-            rc = launch_class<T, kRegionBig, kPathsMultiAxis>(desc, tiles + off[2], off[3] - off[2], glx, glw,
-                                                              glp32, gl_len, tile_part + off[2], s);
-        if (!rc)
-            rc = launch_class<T, 0, kPathsMultiAxis>(desc, tiles + off[3], off[4] - off[3], glx, glw, glp32, gl_len,
-                                                     tile_part + off[3], s);
+        int busy = 0;
+        for (int c = 0; c < kBatchClasses; ++c) busy += off[c + 1] > off[c];
+        if (busy <= 1) {
+            rc = run_classes<T>(desc, tiles, off, glx, glw, glp32, gl_len, tile_part, s, s, s);
+        } else {
+            BatchAux* aux = nullptr;
+            rc = batch_aux(&aux);
+            if (rc) return rc;
+            std::lock_guard<std::mutex> lock(aux->mu);
+            rc = hip_check(hipEventRecord(aux->fork, s), "hipEventRecord");
+            for (int i = 0; i < 2 && !rc; ++i) rc = hip_check(hipStreamWaitEvent(aux->st[i], aux->fork, 0), "hipStreamWaitEvent");
+            if (!rc) rc = run_classes<T>(desc, tiles, off, glx, glw, glp32, gl_len, tile_part, s, aux->st[0], aux->st[1]);
+            for (int i = 0; i < 2 && !rc; ++i) {
+                rc = hip_check(hipEventRecord(aux->join[i], aux->st[i]), "hipEventRecord");
+                if (!rc) rc = hip_check(hipStreamWaitEvent(s, aux->join[i], 0), "hipStreamWaitEvent");
+            }
+        }
     }
     if (rc) return rc;
     const long long fb = (count + kWavesPerBlock - 1) / kWavesPerBlock;
