@@ -159,11 +159,26 @@ __device__ inline double node_r(int j, int n, int rule, const double* glx) {
 
 // One loop-axis table entry of the backward kernel: exponent term
 // -(a x + b)^2 + log2(w) in log2 units (the node weight folded into the
-// exponent, so w f is one exp2), t = x - u, t^2 and t r (r: normalised node).
+// exponent, so w f is one exp2) and the normalised node r.  The loop axis's
+// sums of W f t, W f t^2 and W f t r follow from the moments of r, since
+// t = x - u = alpha r + gamma on the axis (see loop_axis_affine): the loop
+// keeps sum W f r and sum W f r^2, and its LDS reads move 8 bytes per point
+// instead of the 16 of (term, t, t^2, t r) entries.
 template <typename T>
-struct alignas(4 * sizeof(T) < 16 ? 4 * sizeof(T) : 16) BwdEntry {
-    T term, t, t2, tr;
+struct alignas(2 * sizeof(T)) BwdEntry {
+    T term, r;
 };
+
+// x = alpha r + beta on an axis (r the normalised node of node_r).
+__device__ inline void loop_axis_affine(const Axis& ax, int n, int rule, double* alpha, double* beta) {
+    if (rule == TQ_RULE_GAUSS_LEGENDRE) {
+        *alpha = ax.a;
+        *beta = ax.h;
+    } else {
+        *alpha = 0.5 * (ax.b - ax.a);
+        *beta = 0.5 * (ax.b + ax.a);
+    }
+}
 
 // Backward tile for D axes: flat walk of the tile's points (lane axes per
 // lane, the rest decoded per row).  Only the loop axis is accumulated per
@@ -250,13 +265,12 @@ __device__ inline void sweep_bwd_tile(const GridGeom& g, const Params<T>& p, con
             }
         }
     }
-    T aL = T(0), bL = T(0), uL = T(0), s_lane = T(0);
+    T aL = T(0), bL = T(0), s_lane = T(0);
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         if (k == L) {
             aL = p.a[k];
             bL = p.b[k];
-            uL = p.u[k];
         }
         if (k > L) {
             const T e = fma(xk[k], p.a[k], p.b[k]);
@@ -267,7 +281,7 @@ __device__ inline void sweep_bwd_tile(const GridGeom& g, const Params<T>& p, con
     // row factor: sum W f t_k = t_k F_row, sum W f t_k^2 = t_k^2 F_row,
     // sum W f t_k r_k = t_k r_k F_row.  Lane axes (k > L) keep their node for
     // the whole tile, so theirs are t_k F_tile etc., formed after the walk.
-    double F_tile = 0.0, EL = 0.0, QL = 0.0, RL = 0.0;  // loop-axis sums
+    double F_tile = 0.0, M1 = 0.0, M2 = 0.0;  // loop-axis sums of W f r and W f r^2
     while (t < t_end) {
         double w_row = (double)w_lane;
         T s_fix = s_lane;
@@ -286,35 +300,30 @@ __device__ inline void sweep_bwd_tile(const GridGeom& g, const Params<T>& p, con
         double Frow = 0.0;
         for (int jb = j; jb < jend; jb += kFlush) {
             const int je = (jb + kFlush < jend) ? (jb + kFlush) : jend;
-            T fF = T(0), fE = T(0), fQ = T(0), fR = T(0);
+            T fF = T(0), f1 = T(0), f2 = T(0);
 #pragma unroll 4
             for (int jj = jb; jj < je; ++jj) {
+                T wf, r;
                 if (TABLE) {
-                    // one add, one exp2 and four accumulates per point
+                    // one add, one exp2, then F, W f r and W f r^2
                     const BwdEntry<T> en = tab[jj];
-                    const T wf = tq_exp2(s_fix + en.term);
-                    fF += wf;
-                    fE = fma(wf, en.t, fE);
-                    fQ = fma(wf, en.t2, fQ);
-                    fR = fma(wf, en.tr, fR);
+                    wf = tq_exp2(s_fix + en.term);
+                    r = en.r;
                 } else {
                     const T xL = (T)node_of(ax[L], jj, n, rule, glx);
                     const T e = fma(xL, aL, bL);
                     const T wL = (T)weight_of(rule, jj, n, glw);
-                    const T tL = xL - uL;
-                    const T rL = (T)node_r(jj, n, rule, glx);
-                    const T wf = wL * tq_exp2(s_fix - e * e);
-                    fF += wf;
-                    const T wt = wf * tL;
-                    fE += wt;
-                    fQ = fma(wt, tL, fQ);
-                    fR = fma(wt, rL, fR);
+                    wf = wL * tq_exp2(s_fix - e * e);
+                    r = (T)node_r(jj, n, rule, glx);
                 }
+                fF += wf;
+                const T wr = wf * r;
+                f1 += wr;
+                f2 = fma(wr, r, f2);
             }
             Frow += w_row * (double)fF;
-            EL += w_row * (double)fE;
-            QL += w_row * (double)fQ;
-            RL += w_row * (double)fR;
+            M1 += w_row * (double)f1;
+            M2 += w_row * (double)f2;
         }
         F_tile += Frow;
 #pragma unroll
@@ -344,9 +353,14 @@ __device__ inline void sweep_bwd_tile(const GridGeom& g, const Params<T>& p, con
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         if (k == L) {
-            accE[k] += EL;
-            accQ[k] += QL;
-            accR[k] += RL;
+            // t = alpha r + gamma on the loop axis: sums of W f t, W f t^2 and
+            // W f t r from F_tile = sum W f and the moments M1, M2 of r
+            double alpha, beta;
+            loop_axis_affine(ax[L], n, rule, &alpha, &beta);
+            const double gam = beta - (double)p.u[L];
+            accE[k] += alpha * M1 + gam * F_tile;
+            accQ[k] += alpha * alpha * M2 + 2.0 * alpha * gam * M1 + gam * gam * F_tile;
+            accR[k] += alpha * M2 + gam * M1;
         }
         if (k > L) {
             const double tt = (double)(xk[k] - p.u[k]);
@@ -379,12 +393,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_bwd(SweepGeom S, const T* a, c
             const T e = fma(x, s_p.a[L], s_p.b[L]);
             // composite-rule and Gauss-Legendre weights are positive
             const double w = weight_of(g.rule, j, g.n, glw);
-            const T t = x - s_p.u[L];
             BwdEntry<T> en;
             en.term = (T)((double)(-e * e) + log2(w));
-            en.t = t;
-            en.t2 = t * t;
-            en.tr = t * (T)node_r(j, g.n, g.rule, glx);
+            en.r = (T)node_r(j, g.n, g.rule, glx);
             s_tab[j] = en;
         }
         __syncthreads();
