@@ -254,9 +254,13 @@ def lib():
     return load_library()
 
 
+_device_ready = False  # a GPU was seen and the library bound (checked once)
+
+
 def require_device(tensor):
     """Raise unless `tensor` lives on a ROCm GPU that the library can use."""
-    if not torch.cuda.is_available():
+    global _device_ready
+    if not _device_ready and not torch.cuda.is_available():
         raise NativeUnavailableError(
             "torchquad_amd computes on a ROCm GPU only and none is visible to torch"
         )
@@ -266,7 +270,9 @@ def require_device(tensor):
             f"{tensor.device}. Call enable_cuda() / set_up_backend('torch') or pass a "
             "domain tensor on the GPU."
         )
-    lib()
+    if not _device_ready:
+        lib()
+        _device_ready = True
 
 
 def call(name, *args):
@@ -279,7 +285,21 @@ def call(name, *args):
     return rc
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def stream_ptr(device=None):
+    """Raw handle of torch's current stream on `device` (a torch.device, an
+    index or None for the current device)."""
+    if _raw_stream is not None:
+        if device is None:
+            idx = torch.cuda.current_device()
+        elif isinstance(device, int):
+            idx = device
+        else:
+            device = torch.device(device)
+            idx = device.index if device.index is not None else torch.cuda.current_device()
+        return _raw_stream(idx)
     return torch.cuda.current_stream(device).cuda_stream
 
 

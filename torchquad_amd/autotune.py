@@ -101,14 +101,20 @@ def _grid_key(rule, family, dim, n, dtype):
     return f"grid|{int(rule)}|{int(family)}|{int(dim)}|{int(n)}|{dtype}"
 
 
+_file_tried = False  # grid_ipw looked for TQ_AUTOTUNE_CACHE once
+
+
 def grid_ipw(rule, family, dim, n, dtype):
     """Tuned items-per-wave of the fused grid kernel for this geometry, or 0 (automatic)."""
+    global _file_tried
     key = _grid_key(rule, family, dim, n, dtype)
     hit = _cache.get(key)
-    if hit is None and not _cache and _cache_file():
-        with _lock:
-            _load()
-        hit = _cache.get(key)
+    if hit is None and not _cache and not _file_tried:
+        _file_tried = True
+        if _cache_file():
+            with _lock:
+                _load()
+            hit = _cache.get(key)
     return int(hit["items_per_wave"]) if hit else 0
 
 
