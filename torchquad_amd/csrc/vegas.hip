@@ -1256,6 +1256,260 @@ __global__ __launch_bounds__(kMapBlock) void k_vegas_map_edges(tq_vegas_state st
     }
 }
 
+// ---- update_map for small maps (vegas_map.py:113-261) ----
+// At most 64 bins per axis and 16 axes.  Histogram mode: G blocks sum the
+// sample blocks' histogram rows into G partial rows (two levels of about
+// sqrt(rows) loads per thread instead of one long walk), then one block sums
+// those and runs the update with wave k < dim on axis k, one lane per bin:
+// weight/count, zero-count fill, smoothing, and the new edges from wave sums
+// and scans.  These are the one-wave case of the block kernels above (a
+// block sum over one busy wave adds only zeros to that wave's sum), so both
+// give the same values.
+constexpr int kSmallMapBins = kWave;
+constexpr int kSmallMapAxes = kMapWaves;
+constexpr int kSmallMapEdges = kSmallMapBins + 1;
+
+// Inclusive scan over the wave's lanes (Hillis-Steele, as map_block_scan).
+__device__ inline double wave_scan(double v, int lane) {
+    for (int off = 1; off < kWave; off <<= 1) {
+        const double o = __shfl_up(v, off, kWave);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
+// Column e of rows r0, r0 + step, ... < end of a [rows][W] histogram, added
+// in row order onto (w, c); loads go out kRowBatch at a time, so a thread
+// waits about once per batch instead of once per row.
+constexpr int kRowBatch = 8;
+__device__ inline void column_sum(const double* hw, const uint32_t* hc, long long W, int e, long long r0,
+                                  long long step, long long end, double& w, unsigned long long& c) {
+    const long long stride = step * W;
+    const double* pw = hw + r0 * W + e;
+    const uint32_t* pc = hc + r0 * W + e;
+    for (long long r = r0; r < end; r += kRowBatch * step, pw += kRowBatch * stride, pc += kRowBatch * stride) {
+        double a[kRowBatch];
+        uint32_t b[kRowBatch];
+#pragma unroll
+        for (int i = 0; i < kRowBatch; ++i) {
+            const bool in = r + i * step < end;
+            a[i] = in ? pw[i * stride] : 0.0;
+            b[i] = in ? pc[i * stride] : 0u;
+        }
+#pragma unroll
+        for (int i = 0; i < kRowBatch; ++i) {
+            w += a[i];
+            c += b[i];
+        }
+    }
+}
+
+// Histogram mode, first kernel: block g of G sums rows g, g+G, g+2G, ... of
+// the sample blocks' hist_w/hist_c (S = 1024 / W threads per column, each
+// taking every S-th of those rows; the S sums added in order) and stores the
+// result over its own row g, which no other block reads.
+__global__ __launch_bounds__(kMapBlock) void k_vegas_map_rows(tq_vegas_state st, long long nsamp, int it) {
+    __shared__ double s_pw[kMapBlock], s_pc[kMapBlock];
+    const int tid = threadIdx.x;
+    const int W = st.dim * st.n_intervals, S = kMapBlock / W, e = tid % W, sub = tid / W;
+    long long total = nsamp >= 0 ? nsamp : (long long)st.iter_stats[4 * it + 2];
+    if (nsamp < 0 && total > st.max_samples) total = st.max_samples;
+    long long nb = (total + kSB - 1) / kSB;
+    if (nb > st.max_blocks) nb = st.max_blocks;
+    const long long G = gridDim.x, g = blockIdx.x;
+    double w = 0.0;
+    unsigned long long c = 0;
+    if (sub < S) column_sum(st.hist_w, st.hist_c, W, e, g + sub * G, S * G, nb, w, c);
+    s_pw[tid] = w;
+    s_pc[tid] = (double)c;
+    __syncthreads();
+    if (tid < W) {
+        double wr = 0.0;
+        unsigned long long cr = 0;
+        for (int q = 0; q < S; ++q) {
+            wr += s_pw[q * W + tid];
+            cr += (unsigned long long)s_pc[q * W + tid];
+        }
+        st.hist_w[g * W + tid] = wr;
+        st.hist_c[g * W + tid] = (uint32_t)cr;  // < 2^32: the host requires max_samples < 2^32
+    }
+}
+
+// Second kernel, one block: rows > 0 sums the first `rows` rows of
+// hist_w/hist_c into weights/counts; rows == 0 takes the weights and counts
+// as they stand and zeroes them (_reset_weight).  Then the map update.
+__global__ __launch_bounds__(kMapBlock) void k_vegas_map_small(tq_vegas_state st, int rows) {
+    __shared__ double s_tw[kMapBlock], s_tc[kMapBlock];  // totals per (axis, bin)
+    __shared__ double s_rows[kMapWaves][4][kSmallMapEdges];  // per-wave rows; first the sub-sums
+    __shared__ double s_sum[kMapWaves];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), k = tid >> 6;
+    const int d = st.dim, n = st.n_intervals, W = d * n;
+    if (rows > 0) {
+        double* s_pw = &s_rows[0][0][0];
+        double* s_pc = s_pw + kMapBlock;
+        const int S = kMapBlock / W, e = tid % W, sub = tid / W;
+        double w = 0.0;
+        unsigned long long c = 0;
+        if (sub < S) column_sum(st.hist_w, st.hist_c, W, e, sub, S, rows, w, c);
+        s_pw[tid] = w;
+        s_pc[tid] = (double)c;
+        __syncthreads();
+        if (tid < W) {
+            double wr = 0.0, cr = 0.0;
+            for (int q = 0; q < S; ++q) {
+                wr += s_pw[q * W + tid];
+                cr += s_pc[q * W + tid];
+            }
+            s_tw[tid] = wr;
+            s_tc[tid] = cr;
+            st.weights[tid] = wr;
+            st.counts[tid] = cr;
+        }
+    } else {
+        for (int e = tid; e < W; e += kMapBlock) {
+            s_tw[e] = st.weights[e];
+            s_tc[e] = st.counts[e];
+            st.weights[e] = 0.0;
+            st.counts[e] = 0.0;
+        }
+    }
+    const bool any_zero = __syncthreads_or(tid < W && s_tc[tid] == 0.0) != 0;
+
+    // smoothing (vegas_map.py:113-172), lane j = bin j of axis k
+    const bool axis = k < d;
+    const int j = lane;
+    const bool bin = axis && j < n;
+    double w = 0.0;
+    bool z = false;
+    if (bin) {
+        const double wt = s_tw[k * n + j], ct = s_tc[k * n + j];
+        z = ct == 0.0;
+        w = (!any_zero || ct != 0.0) ? wt / ct : wt;
+    }
+    if (axis && any_zero) {
+        // up to 10 passes, each reading only values from before the pass
+        for (int pass = 0; pass < 10; ++pass) {
+            const double wn = __shfl_down(w, 1, kWave);
+            const int zn = __shfl_down(z ? 1 : 0, 1, kWave);
+            const double w2 = (j < n - 1 && z) ? wn : w;
+            const bool z2 = (j < n - 1) ? (z && zn != 0) : z;
+            const double wp = __shfl_up(w2, 1, kWave);
+            const int zp = __shfl_up(z2 ? 1 : 0, 1, kWave);
+            w = (j >= 1 && z2) ? wp : w2;
+            z = (j >= 1) ? (z2 && zp != 0) : z2;
+            if (__ballot(bin && z) == 0) break;
+        }
+    }
+    const double sum = wave_sum(bin ? w : 0.0);
+    const double wl = __shfl_up(w, 1, kWave), wr = __shfl_down(w, 1, kWave);
+    double sw = 0.0;
+    if (bin) {
+        // [1,6,1]/8 smoothing (edges [7,1]/8) and range compression (EQ 18-19)
+        double dd;
+        if (j == 0) dd = 7.0 * w + wr;
+        else if (j == n - 1) dd = wl + 7.0 * w;
+        else dd = wl + 6.0 * w + wr;
+        dd = dd / (8.0 * sum);
+        if (dd != 0.0) {
+            const double r = (dd - 1.0) / log(dd);
+            dd = (st.alpha == 0.5) ? sqrt(r) : pow(r, st.alpha);
+        }
+        sw = dd;
+    }
+    if (axis && lane == 0) s_sum[k] = sum;
+    __syncthreads();
+    bool cannot = false;
+    for (int a = 0; a < d; ++a) cannot = cannot || (s_sum[a] == 0.0);
+    if (cannot) {
+        if (tid == 0) st.flags[0] += 1;
+        return;
+    }
+    if (!axis) return;
+
+    // new edges (vegas_map.py:198-259); from here on each wave works alone
+    double* num = s_rows[k][0];
+    double* val = s_rows[k][1];
+    double* newx = s_rows[k][2];
+    double* tmp = s_rows[k][3];
+    double* xe = st.x_edges + k * (n + 1);
+    double* dxe = st.dx_edges + k * n;
+    const double total = wave_sum(sw);
+    const double delta_d = total / (double)n;
+    const bool first = j < n - 1;  // the n-1 cumulative sums
+    const double cum = wave_scan(first ? sw : 0.0, lane);
+    int mlt = 0;
+    bool bad = false;
+    if (first) {
+        const double q = cum / delta_d;
+        if (q >= 0.0 && q <= (double)n) mlt = (int)q;
+        else bad = true;
+    }
+    for (int m = lane; m <= n; m += kWave) {
+        num[m] = 0.0;
+        val[m] = 0.0;
+    }
+    if (__ballot(bad) != 0) {
+        if (lane == 0) {
+            atomicAdd(&st.flags[1], 1);
+            atomicAdd(&st.flags[2], 1);
+        }
+        return;
+    }
+    wave_lds_sync();
+    // count and weight sum of each cumulative bin: runs of equal mlt are
+    // contiguous (cum is nondecreasing); DPP run scan, written at run ends
+    unsigned key = first ? (unsigned)mlt : 0xFFFFFFFEu;
+    double rs = first ? sw : 0.0;
+    int cnt = first ? 1 : 0;
+    run_scan_step<0x111, 0xF>(key, rs, cnt);
+    run_scan_step<0x112, 0xF>(key, rs, cnt);
+    run_scan_step<0x114, 0xF>(key, rs, cnt);
+    run_scan_step<0x118, 0xF>(key, rs, cnt);
+    run_scan_step<0x142, 0xA>(key, rs, cnt);
+    run_scan_step<0x143, 0xC>(key, rs, cnt);
+    const int next_mlt = __shfl_down(mlt, 1, kWave);
+    if (first && (j == n - 2 || next_mlt != mlt)) {
+        num[key] = (double)cnt;
+        val[key] = rs;
+    }
+    wave_lds_sync();
+    // new edge j+1 from the running count and remainder (vegas_map.py:238-249)
+    const double idx = wave_scan(first ? num[j] : 0.0, lane);
+    const double accu = wave_scan(first ? delta_d - val[j] : 0.0, lane);
+    const long long ix0 = first ? (long long)idx : 0;
+    const int ix = (int)(ix0 > n - 1 ? n - 1 : ix0);
+    const double sw_ix = __shfl(sw, ix, kWave);
+    if (first) tmp[j] = xe[ix] + accu / sw_ix * dxe[ix];
+    wave_lds_sync();
+    for (int m = lane; m <= n; m += kWave) newx[m] = (m == 0) ? xe[0] : ((m == n) ? xe[n] : tmp[m - 1]);
+    wave_lds_sync();
+    bool nonfinite = false;
+    for (int m = lane; m <= n; m += kWave) nonfinite = nonfinite || !isfinite(newx[m]);
+    int fl = 0;
+    if (__ballot(nonfinite) != 0) {
+        fl |= 1;
+        // replace by the mean of the neighbours (vegas_map.py:252-255)
+        for (int m = lane; m <= n; m += kWave)
+            tmp[m] = (m >= 1 && m < n && !isfinite(newx[m])) ? 0.5 * (newx[m - 1] + newx[m + 1]) : newx[m];
+        wave_lds_sync();
+        bool still = false;
+        for (int m = lane; m <= n; m += kWave) {
+            newx[m] = tmp[m];
+            still = still || !isfinite(tmp[m]);
+        }
+        if (__ballot(still) != 0) fl |= 2;
+        wave_lds_sync();
+    }
+    if (!(fl & 2)) {
+        for (int m = lane; m <= n; m += kWave) xe[m] = newx[m];
+        for (int m = lane; m < n; m += kWave) dxe[m] = newx[m + 1] - newx[m];
+    }
+    if (lane == 0) {
+        if (fl & 1) atomicAdd(&st.flags[1], 1);
+        if (fl & 2) atomicAdd(&st.flags[2], 1);
+    }
+}
+
 __global__ __launch_bounds__(64) void k_vegas_smooth(int d, int n, const double* weights, const double* counts,
                                                      double alpha, double* out, double* scratch, int32_t* flag) {
     __shared__ double s_sum[TQ_MAX_DIM];
@@ -1334,9 +1588,28 @@ static int launch_strat(const tq_vegas_state* st, const double* w, int it, int u
     return check_launch("k_vegas_strat_finalize");
 }
 
+// Maps that k_vegas_map_small updates: at most 64 bins and 16 axes, and
+// per-bin counts below 2^32 (its partial rows keep them in hist_c).
+static bool small_map(const tq_vegas_state* st) {
+    return st->n_intervals <= kSmallMapBins && st->dim <= kSmallMapAxes && st->max_samples < (1LL << 32);
+}
+
+// Partial rows of the one-launch update: about sqrt(sample blocks), so the
+// blocks' row sums and the last block's sum of the rows take similar time.
+static int small_map_grid(const tq_vegas_state* st) {
+    long long g = (long long)std::ceil(std::sqrt((double)st->max_blocks));
+    if (g > 256) g = 256;
+    if (g > st->max_blocks) g = st->max_blocks;
+    return (int)(g < 1 ? 1 : g);
+}
+
 // update_map on the accumulated weights/counts: smoothing, then the edges;
 // `reset` zeroes the weights and counts afterwards (_reset_weight).
 static int launch_map_kernels(const tq_vegas_state* st, int reset, hipStream_t s) {
+    if (reset && small_map(st)) {
+        hipLaunchKernelGGL(k_vegas_map_small, dim3(1), dim3(kMapBlock), 0, s, *st, 0);
+        return check_launch("k_vegas_map_small");
+    }
     const size_t scr = (size_t)5 * (st->n_intervals + 1) * sizeof(double);
     const int lds = scr <= kMapUpdateLds ? 1 : 0;
     hipLaunchKernelGGL(k_vegas_map_smooth, dim3((unsigned)st->dim), dim3(kMapBlock), lds ? scr : 0, s, *st, lds);
@@ -1349,6 +1622,14 @@ static int launch_map_kernels(const tq_vegas_state* st, int reset, hipStream_t s
 
 static int launch_map_update(const tq_vegas_state* st, long long nsamp, int it, hipStream_t s) {
     const int W = st->dim * st->n_intervals;
+    if (W <= kHistLds && small_map(st)) {
+        const int rows = small_map_grid(st);
+        hipLaunchKernelGGL(k_vegas_map_rows, dim3(rows), dim3(kMapBlock), 0, s, *st, nsamp, it);
+        const int rc = check_launch("k_vegas_map_rows");
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_vegas_map_small, dim3(1), dim3(kMapBlock), 0, s, *st, rows);
+        return check_launch("k_vegas_map_small");
+    }
     if (W <= kHistLds) {
         const int rb = (W + kWavesPerBlock - 1) / kWavesPerBlock;
         hipLaunchKernelGGL(k_vegas_map_reduce, dim3(rb), dim3(kBlock), 0, s, *st, nsamp, it, 0);
