@@ -155,10 +155,31 @@ __global__ __launch_bounds__(1024) void k_vegas_scan(int64_t* sums, int nb) {
     if (tid == 1023) sums[nb] = s[1023];
 }
 
-__global__ __launch_bounds__(kBlock) void k_vegas_offsets(tq_vegas_state st, int it, int nb_scan) {
+// raw != 0: scan_sums holds k_vegas_nh's block sums unscanned (at most kBlock
+// of them) and each block adds the ones before its cubes itself (integer
+// sums: any order gives the same value), which saves the k_vegas_scan launch.
+__global__ __launch_bounds__(kBlock) void k_vegas_offsets(tq_vegas_state st, int it, int nb_scan, int raw) {
+    __shared__ long long s_part[kWavesPerBlock];
     const long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long prefix = 0, total = 0;
+    if (raw) {
+        // this block's cubes all lie in nh block j0 (kBlock divides kScanBlockCubes)
+        const long long j0 = ((long long)blockIdx.x * kBlock) / kScanBlockCubes;
+        const long long lim = blockIdx.x == 0 ? nb_scan : j0;
+        long long v = (long long)threadIdx.x < lim ? st.scan_sums[threadIdx.x] : 0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+        if ((threadIdx.x & (kWave - 1)) == 0) s_part[threadIdx.x >> 6] = v;
+        __syncthreads();
+        long long r = 0;
+        for (int w = 0; w < kWavesPerBlock; ++w) r += s_part[w];
+        if (blockIdx.x == 0) total = r;
+        else prefix = r;
+    } else if (h < st.n_cubes) {
+        prefix = st.scan_sums[h / kScanBlockCubes];
+    }
     if (h < st.n_cubes) {
-        const long long off = st.offsets[h] + st.scan_sums[h / kScanBlockCubes];
+        const long long off = st.offsets[h] + prefix;
         st.offsets[h] = off;
         const long long n = st.nh[h];
         if (n > 0) {
@@ -168,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_vegas_offsets(tq_vegas_state st, int
         }
     }
     if (h == 0) {
-        const long long total = st.scan_sums[nb_scan];
+        if (!raw) total = st.scan_sums[nb_scan];
         st.offsets[st.n_cubes] = total;
         st.iter_stats[4 * it + 2] = (double)total;
         if (total > st.max_samples || (total + kSB - 1) / kSB > st.max_blocks) st.flags[3] = 1;
@@ -826,8 +847,9 @@ __global__ __launch_bounds__(kBlock) void k_vegas_update_dh(tq_vegas_state st) {
     }
 }
 
-__global__ __launch_bounds__(1024) void k_vegas_strat_finalize(tq_vegas_state st, long long nb, int it) {
-    __shared__ double s_red[1024 / kWave];
+// The iteration statistics from k_vegas_strat's block partials, by one
+// 1024-thread block (its own kernel, or an extra block of k_vegas_map_rows).
+__device__ inline void strat_finalize_block(const tq_vegas_state& st, long long nb, int it, double* s_red) {
     double a = 0.0, b = 0.0, c = 0.0;
     for (long long i = threadIdx.x; i < nb; i += blockDim.x) {
         a += st.block_sums[4 * i];
@@ -842,6 +864,11 @@ __global__ __launch_bounds__(1024) void k_vegas_strat_finalize(tq_vegas_state st
         st.iter_stats[4 * it + 1] = b;
         st.iter_stats[4 * it + 3] = c;
     }
+}
+
+__global__ __launch_bounds__(1024) void k_vegas_strat_finalize(tq_vegas_state st, long long nb, int it) {
+    __shared__ double s_red[1024 / kWave];
+    strat_finalize_block(st, nb, it, s_red);
 }
 
 __global__ __launch_bounds__(kBlock) void k_vegas_normalize(tq_vegas_state st, int it) {
@@ -1308,15 +1335,23 @@ __device__ inline void column_sum(const double* hw, const uint32_t* hc, long lon
 // the sample blocks' hist_w/hist_c (S = 1024 / W threads per column, each
 // taking every S-th of those rows; the S sums added in order) and stores the
 // result over its own row g, which no other block reads.
-__global__ __launch_bounds__(kMapBlock) void k_vegas_map_rows(tq_vegas_state st, long long nsamp, int it) {
+// Block `rows` (present when nb_strat > 0) computes the iteration statistics
+// from nb_strat k_vegas_strat partials instead, as k_vegas_strat_finalize.
+__global__ __launch_bounds__(kMapBlock) void k_vegas_map_rows(tq_vegas_state st, long long nsamp, int it, int rows,
+                                                              long long nb_strat) {
     __shared__ double s_pw[kMapBlock], s_pc[kMapBlock];
+    __shared__ double s_red[kMapWaves];
+    if ((int)blockIdx.x == rows) {
+        strat_finalize_block(st, nb_strat, it, s_red);
+        return;
+    }
     const int tid = threadIdx.x;
     const int W = st.dim * st.n_intervals, S = kMapBlock / W, e = tid % W, sub = tid / W;
     long long total = nsamp >= 0 ? nsamp : (long long)st.iter_stats[4 * it + 2];
     if (nsamp < 0 && total > st.max_samples) total = st.max_samples;
     long long nb = (total + kSB - 1) / kSB;
     if (nb > st.max_blocks) nb = st.max_blocks;
-    const long long G = gridDim.x, g = blockIdx.x;
+    const long long G = rows, g = blockIdx.x;
     double w = 0.0;
     unsigned long long c = 0;
     if (sub < S) column_sum(st.hist_w, st.hist_c, W, e, g + sub * G, S * G, nb, w, c);
@@ -1565,11 +1600,14 @@ static int launch_nh(const tq_vegas_state* st, int64_t n_exp, int normalize_it, 
     hipLaunchKernelGGL(k_vegas_nh, dim3(nb), dim3(kBlock), 0, s, *st, (double)n_exp, normalize_it, nh_in);
     int rc = check_launch("k_vegas_nh");
     if (rc) return rc;
-    hipLaunchKernelGGL(k_vegas_scan, dim3(1), dim3(1024), 0, s, st->scan_sums, nb);
-    rc = check_launch("k_vegas_scan");
-    if (rc) return rc;
+    const int raw = nb <= kBlock ? 1 : 0;
+    if (!raw) {
+        hipLaunchKernelGGL(k_vegas_scan, dim3(1), dim3(1024), 0, s, st->scan_sums, nb);
+        rc = check_launch("k_vegas_scan");
+        if (rc) return rc;
+    }
     const int ob = (int)((st->n_cubes + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_vegas_offsets, dim3(ob), dim3(kBlock), 0, s, *st, it, nb);
+    hipLaunchKernelGGL(k_vegas_offsets, dim3(ob), dim3(kBlock), 0, s, *st, it, nb, raw);
     return check_launch("k_vegas_offsets");
 }
 
@@ -1579,11 +1617,16 @@ static long long sample_blocks(const tq_vegas_state* st, int warmup, long long n
     return b < 1 ? 1 : b;
 }
 
-static int launch_strat(const tq_vegas_state* st, const double* w, int it, int update_dh, hipStream_t s) {
-    const long long nb = (st->n_cubes + kBlock - 1) / kBlock;
+static long long strat_blocks(const tq_vegas_state* st) { return (st->n_cubes + kBlock - 1) / kBlock; }
+
+// finalize == false leaves the iteration statistics to the caller's next
+// launch (the extra block of k_vegas_map_rows).
+static int launch_strat(const tq_vegas_state* st, const double* w, int it, int update_dh, hipStream_t s,
+                        bool finalize = true) {
+    const long long nb = strat_blocks(st);
     hipLaunchKernelGGL(k_vegas_strat, dim3((unsigned)nb), dim3(kBlock), 0, s, *st, w, update_dh);
     int rc = check_launch("k_vegas_strat");
-    if (rc) return rc;
+    if (rc || !finalize) return rc;
     hipLaunchKernelGGL(k_vegas_strat_finalize, dim3(1), dim3(1024), 0, s, *st, nb, it);
     return check_launch("k_vegas_strat_finalize");
 }
@@ -1620,11 +1663,15 @@ static int launch_map_kernels(const tq_vegas_state* st, int reset, hipStream_t s
     return check_launch("k_vegas_map_edges");
 }
 
-static int launch_map_update(const tq_vegas_state* st, long long nsamp, int it, hipStream_t s) {
+// nb_strat > 0: also finish the iteration statistics from that many
+// k_vegas_strat partials (small maps only; see launch_iteration_tail).
+static int launch_map_update(const tq_vegas_state* st, long long nsamp, int it, hipStream_t s,
+                             long long nb_strat = 0) {
     const int W = st->dim * st->n_intervals;
     if (W <= kHistLds && small_map(st)) {
         const int rows = small_map_grid(st);
-        hipLaunchKernelGGL(k_vegas_map_rows, dim3(rows), dim3(kMapBlock), 0, s, *st, nsamp, it);
+        hipLaunchKernelGGL(k_vegas_map_rows, dim3(rows + (nb_strat > 0 ? 1 : 0)), dim3(kMapBlock), 0, s, *st, nsamp,
+                           it, rows, nb_strat);
         const int rc = check_launch("k_vegas_map_rows");
         if (rc) return rc;
         hipLaunchKernelGGL(k_vegas_map_small, dim3(1), dim3(kMapBlock), 0, s, *st, rows);
@@ -1637,6 +1684,21 @@ static int launch_map_update(const tq_vegas_state* st, long long nsamp, int it, 
         if (rc) return rc;
     }
     return launch_map_kernels(st, W <= kHistLds ? 0 : 1, s);
+}
+
+// The end of an iteration: per-cube sums and the iteration statistics (not
+// in warm-up), then the map update.  With the small-map update the
+// statistics' one-block reduction runs as an extra block of the row-sum
+// kernel: both only need the sample kernel's output.
+static int launch_iteration_tail(const tq_vegas_state* st, int it, int hist, int warmup, long long n_warm,
+                                 hipStream_t s) {
+    const bool fold = hist && !warmup && st->dim * st->n_intervals <= kHistLds && small_map(st);
+    if (!warmup) {
+        const int rc = launch_strat(st, st->jf, it, 1, s, !fold);
+        if (rc) return rc;
+    }
+    if (hist) return launch_map_update(st, warmup ? n_warm : -1, it, s, fold ? strat_blocks(st) : 0);
+    return TQ_OK;
 }
 
 __global__ void k_zero(double* p, long long n) {
@@ -1688,12 +1750,7 @@ static int vegas_iterate(const tq_vegas_state* st, const tq_integrand* f, const 
     });
     int rc = check_launch("k_vegas_sample");
     if (rc) return rc;
-    if (!warmup) {
-        rc = launch_strat(st, st->jf, it, 1, s);
-        if (rc) return rc;
-    }
-    if (hist) return launch_map_update(st, warmup ? n_warm : -1, it, s);
-    return TQ_OK;
+    return launch_iteration_tail(st, it, hist, warmup, n_warm, s);
 }
 
 }  // namespace tq
@@ -1799,12 +1856,7 @@ int tq_vegas_consume(const tq_vegas_state* st, const void* fvals, const void* do
     }
     rc = check_launch("k_vegas_consume");
     if (rc) return rc;
-    if (!warmup) {
-        rc = launch_strat(st, st->jf, it, 1, s);
-        if (rc) return rc;
-    }
-    if (hist) return launch_map_update(st, warmup ? n_warm : -1, it, s);
-    return TQ_OK;
+    return launch_iteration_tail(st, it, hist, warmup, n_warm, s);
 }
 
 int tq_vegas_update_map(const tq_vegas_state* st, int64_t n_samples, int32_t it, void* stream) {
