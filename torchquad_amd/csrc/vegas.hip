@@ -1290,8 +1290,10 @@ __global__ __launch_bounds__(kMapBlock) void k_vegas_map_edges(tq_vegas_state st
 // those and runs the update with wave k < dim on axis k, one lane per bin:
 // weight/count, zero-count fill, smoothing, and the new edges from wave sums
 // and scans.  These are the one-wave case of the block kernels above (a
-// block sum over one busy wave adds only zeros to that wave's sum), so both
-// give the same values.
+// block sum over one busy wave adds only zeros to that wave's sum), so from
+// the same weights and counts both give the same edges.  The histogram row
+// sums group the blocks differently from k_vegas_map_reduce (last-bit
+// differences in the weights).
 constexpr int kSmallMapBins = kWave;
 constexpr int kSmallMapAxes = kMapWaves;
 constexpr int kSmallMapEdges = kSmallMapBins + 1;
