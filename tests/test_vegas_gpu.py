@@ -110,6 +110,28 @@ def test_stratification_bit_exact(dtype, dev, ref_vectors):
 
 # (dim, bins): the sample kernel's instantiations -- dimension bound 8 or 16,
 # wave histograms in LDS (dim * bins <= 768) or global atomics (default bins)
+@pytest.mark.parametrize("n_strat", [724, 725])
+def test_nh_offsets_both_scan_forms(n_strat, dev):
+    """get_NH and the sample offsets around the switch between the two scan
+    forms: 724^2 cubes give 256 nh blocks of 2048 cubes (block sums added by
+    the offsets kernel itself), 725^2 give 257 (separate scan kernel).  nh
+    bit-exact against the oracle for a non-uniform dh, offsets exactly the
+    exclusive prefix sums of nh."""
+    N = 4 * n_strat * n_strat
+    st = tq.VEGASStratification(N, 2, tq.RNG(seed=3), "torch", torch.float64)
+    ost = ov.Strat(N, 2)
+    assert st.N_strat == ost.N_strat == n_strat
+    dh = np.random.default_rng(n_strat).exponential(1.0, n_strat * n_strat)
+    dh /= dh.sum()
+    st._dev.dh.copy_(torch.tensor(dh))
+    ost.dh = dh.copy()
+    for n_exp in (10**6, 3 * 10**7):
+        nh = st.get_NH(n_exp).cpu().numpy()
+        assert np.array_equal(nh, ost.get_NH(n_exp))
+        off = st._dev.offsets.cpu().numpy()
+        assert off[0] == 0 and np.array_equal(off[1:], np.cumsum(nh))
+
+
 @pytest.mark.parametrize("dim,bins", [(3, None), (9, None), (3, 40), (9, 40)])
 def test_fused_end_to_end_fp64_matches_oracle(dim, bins, dev):
     N, seed = 60000, 5
